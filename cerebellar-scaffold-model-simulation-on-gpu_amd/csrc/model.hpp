@@ -1,0 +1,160 @@
+// model.hpp -- host-side network description, planner output and engine interface.
+#pragma once
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/scaffold.h"
+#include "device_types.hpp"
+
+namespace sc {
+
+// Reference time constants (option.h:4-7).
+constexpr double kDT = 0.1;
+constexpr double kSec = 1000.;
+constexpr int kStepMax = 10;
+
+// Thread-local error reporting behind scaffold_last_error().
+int fail(int code, const char *fmt, ...);
+void clear_error();
+
+struct Population {
+    bool set = false;
+    std::string name;
+    int type = -1, duplicate = 0, num = 0, kind = SCAFFOLD_NORMAL, c_num = 0;
+    float Cm = 0, El = 0, dt_ref = 0, Ie = 0, Vr = 0, Vth = 0, gL = 0;
+    float w[2] = {0, 0}, E[2] = {0, 0}, g_bar[2] = {0, 0}, tau[2] = {1, 1};
+};
+
+struct Projection {
+    bool set = false;
+    int type = -1;
+    std::string file;
+    int preNum = 0, postNum = 0, pre = -1, post = -1;
+    float weight = 0.f;   // initial_weight (sign selects the buffer)
+    float delay_ms = 0.f;
+    int use_pr = 0;
+    // CSR by post local id over the post population (rows beyond it dropped)
+    std::vector<int64_t> row_ptr;
+    std::vector<int32_t> col;   // pre local ids
+    std::vector<float> val;     // empty: unit multipliers
+    int max_conv = 0;
+    int delay() const { return (int)delay_ms; }
+};
+
+struct Stimulus {
+    int pop = -1;
+    std::vector<uint32_t> targets;
+    std::vector<int64_t> fanout;   // empty: entry i drives targets[i]
+    scaffold_stim_func f{};
+    float rate = 0.f;              // current Poisson rate (SetInputFreq may change it)
+    int n_entries() const { return fanout.empty() ? (int)targets.size() : (int)fanout.size() - 1; }
+};
+
+struct Options {
+    uint64_t neuron_seed = 0x5eed;
+    int weight_precision = SCAFFOLD_WEIGHTS_F32;
+    int index_bits = 32;
+    int lane_max_fanin = 48;
+    int chunk = 1024;
+    int graph = 1;
+    int graph_steps = 25;
+    int record_fired = 0;
+    int reset_g = 0;
+};
+
+// Everything the engine uploads, built once by plan_build().
+struct HostProjArrays {
+    std::vector<uint32_t> col;
+    std::vector<uint16_t> col16;
+    std::vector<uint8_t> val;       // raw bytes of f32 / bf16 / fp8 multipliers
+    std::vector<int64_t> row_ptr;
+    std::vector<int64_t> sell_off;
+    std::vector<int32_t> sell_w;
+    std::vector<int32_t> sell_base;
+};
+
+struct Plan {
+    int Ndev = 0, n_groups = 0, Wr = 0, R = 0;
+    int total_ref = 0;
+    std::vector<ScPop> pops;
+    std::vector<ScProj> projs;        // pointers filled by the engine
+    std::vector<HostProjArrays> parr;
+    std::vector<ScGroup> groups;
+    std::vector<ScStim> stims;
+    std::vector<int32_t> stim_s, stim_e;   // per device index
+    std::vector<uint32_t> outdeg;         // per device index
+    std::vector<int32_t> dev_to_ref;      // -1 for padding
+    std::vector<int32_t> ref_to_dev;
+    std::vector<ScTask> tasks;            // gather kernel work list
+    std::vector<int32_t> fin_groups;      // groups finalized by the second kernel
+    int64_t n_slots = 0;
+    int64_t nnz = 0;
+    int64_t algo_bytes = 0;
+    int max_delay = 0;
+    std::vector<int> out_pops;            // OUTPUT populations
+    int n_output = 0;
+};
+
+struct Engine;  // device side (engine.hip)
+
+// Engine entry points used by the C ABI layer.
+int engine_create(Engine **e, Plan &plan);
+void engine_destroy(Engine *e);
+int engine_init_params(Engine *e, uint64_t seed, int64_t call_index);
+int engine_zero_g(Engine *e);
+int engine_set_ctl(Engine *e, int64_t n, int64_t m);
+int engine_clear_counts_row(Engine *e, int row);
+int engine_upload_stims(Engine *e, const std::vector<ScStim> &stims);
+int engine_run(Engine *e, int n_steps, int use_graph, int graph_steps);
+int engine_sync(Engine *e);
+int engine_get_state(Engine *e, float *V, float *g, int32_t *refr);
+int engine_set_state(Engine *e, const float *V, const float *g, const int32_t *refr);
+int engine_get_ring_row(Engine *e, int row, uint64_t *words);
+int engine_get_counts(Engine *e, int row, int32_t *counts, unsigned long long *events);
+int engine_get_fired(Engine *e, int row, int32_t *ids, int32_t cap, int32_t *n);
+int engine_time(Engine *e, int n_steps, int use_graph, int graph_steps,
+                double *ms_total, double *ms_kernel_mean);
+int64_t engine_device_bytes(Engine *e);
+int engine_copy_probe(int64_t bytes, int iters, double *ms);
+
+// Batching (engine.hip).
+struct Batch;
+int batch_build(Batch **b, std::vector<Engine *> &engines, const std::vector<const Plan *> &plans);
+int batch_run(Batch *b, int n_steps, int graph_steps, double *ms);
+void batch_destroy(Batch *b);
+
+}  // namespace sc
+
+// The opaque environment behind the C ABI.
+struct scaffold_env {
+    int n_types = 0, n_conn = 0, n_inputs = 0, n_plast = 0;
+    std::vector<sc::Population> pops;
+    int normal_next = 0, other_next = 0;
+    std::vector<sc::Projection> projs;
+    int gpu_next = 0, cpu_next = 0;
+    std::vector<sc::Stimulus> stims;
+    sc::Options opt;
+    bool initialized = false;
+    bool finalized = false;
+    int T_print = 0, T_cpu_sim = 0;
+    int64_t init_calls = 0;
+    int64_t draws = 0;      // stimulus draws since init (mirror of the device counter)
+    int64_t cur_step = 0;   // trial-relative step of the next launch
+    sc::Plan plan;
+    sc::Engine *eng = nullptr;
+    std::vector<char> out_train;  // output_spike_train buffer
+    std::vector<uint64_t> tmp_words;
+    int in_batches = 0;           // batches referencing this environment's device state
+};
+
+namespace sc {
+// describe.cpp
+int plan_build(scaffold_env *env, Plan &plan);
+int load_dat(const char *file, int preNum, int postNum, int postPopNum, Projection &p);
+int synth_describe(scaffold_env *env, const scaffold_synth_config *cfg);
+int describe_reference(scaffold_env *env, const char *dir, uint64_t stim_seed);
+// capi.cpp / planner.cpp
+std::vector<ScStim> dev_stims(scaffold_env *env);
+float plan_multiplier(const scaffold_env *env, int proj, int64_t e);
+}  // namespace sc

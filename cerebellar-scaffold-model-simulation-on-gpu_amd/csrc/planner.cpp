@@ -1,0 +1,405 @@
+// planner.cpp -- .dat loader and the host planner that turns a network description
+// into the device layout (device_types.hpp): device order, sliced-ELL / chunked-CSR
+// projection streams, weight storage, partial-sum slots, the wavefront work list,
+// stimulus ownership and out-degrees.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "model.hpp"
+
+namespace sc {
+
+// ------------------------------------------------------------------ .dat loader
+// GetMaxConv + LoadConnectivityFile_ELL (init_params.cu:13-67,96-128): text lines
+// "pre post" (population-local ids).  Rows keep the file order of their entries
+// (h_col[post]++ at :43-48).  Unlike the reference, which reads out of bounds for
+// such lines, ids outside [0,preNum) x [0,postNum) and posts beyond the post
+// population are rejected.
+int load_dat(const char *file, int preNum, int postNum, int postPopNum, Projection &p)
+{
+    FILE *fp = fopen(file, "r");
+    if (!fp) return fail(SCAFFOLD_ERR_IO, "can't open file :  %s", file);
+    std::vector<int32_t> pre, post;
+    char line[256];
+    int lineno = 0;
+    while (fgets(line, sizeof(line), fp)) {
+        ++lineno;
+        char *s = line;
+        while (*s == ' ' || *s == '\t') ++s;
+        if (*s == '\n' || *s == '\r' || *s == 0) continue;
+        char *end1, *end2;
+        long a = strtol(s, &end1, 10);
+        long b = strtol(end1, &end2, 10);
+        if (end1 == s || end2 == end1) {
+            fclose(fp);
+            return fail(SCAFFOLD_ERR_FORMAT, "%s:%d: expected 'pre<TAB>post'", file, lineno);
+        }
+        if (a < 0 || a >= preNum || b < 0 || b >= postNum) {
+            fclose(fp);
+            return fail(SCAFFOLD_ERR_FORMAT, "%s:%d: pair (%ld, %ld) outside [0,%d) x [0,%d)", file, lineno, a, b,
+                        preNum, postNum);
+        }
+        if (b >= postPopNum) {
+            fclose(fp);
+            return fail(SCAFFOLD_ERR_FORMAT, "%s:%d: post id %ld beyond the post population (%d)", file, lineno,
+                        b, postPopNum);
+        }
+        pre.push_back((int32_t)a);
+        post.push_back((int32_t)b);
+    }
+    fclose(fp);
+    p.row_ptr.assign(postPopNum + 1, 0);
+    for (int32_t b : post) p.row_ptr[b + 1]++;
+    for (int j = 0; j < postPopNum; ++j) p.row_ptr[j + 1] += p.row_ptr[j];
+    p.col.assign(pre.size(), 0);
+    std::vector<int64_t> fillp(p.row_ptr.begin(), p.row_ptr.end() - 1);
+    for (size_t i = 0; i < pre.size(); ++i) p.col[fillp[post[i]]++] = pre[i];
+    p.val.clear();  // every stored ELL value is 1 (init_params.cu:46)
+    p.max_conv = 0;
+    for (int j = 0; j < postPopNum; ++j)
+        p.max_conv = std::max(p.max_conv, (int)(p.row_ptr[j + 1] - p.row_ptr[j]));
+    return SCAFFOLD_OK;
+}
+
+// ------------------------------------------------------------ weight storage
+static inline uint32_t f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static inline float u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+
+static uint16_t f32_to_bf16(float f)  // round to nearest even
+{
+    uint32_t u = f2u(f);
+    uint32_t r = u + 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(r >> 16);
+}
+static float bf16_to_f32(uint16_t h) { return u2f((uint32_t)h << 16); }
+
+// OCP FP8 e4m3: bias 7, no infinities, 0x7F/0xFF NaN, largest finite 448.
+static float e4m3_to_f32(uint8_t c)
+{
+    int s = c >> 7, e = (c >> 3) & 15, m = c & 7;
+    float v = e == 0 ? std::ldexp((float)m, -9) : std::ldexp(1.f + (float)m / 8.f, e - 7);
+    return s ? -v : v;
+}
+
+static uint8_t f32_to_e4m3(float x)  // nearest, ties to even code, saturating at 448
+{
+    static float table[127];
+    static bool init = false;
+    if (!init) {
+        for (int c = 0; c < 127; ++c) table[c] = e4m3_to_f32((uint8_t)c);
+        init = true;
+    }
+    uint8_t sign = x < 0 ? 0x80 : 0;
+    float a = std::fabs(x);
+    if (a >= 448.f) return sign | 126;
+    int hi = (int)(std::upper_bound(table, table + 127, a) - table);  // first > a
+    int lo = hi - 1;
+    if (hi >= 127) return sign | 126;
+    float dl = a - table[lo], dh = table[hi] - a;
+    int c = (dl < dh) ? lo : (dh < dl) ? hi : ((lo & 1) ? hi : lo);
+    return sign | (uint8_t)c;
+}
+
+// Effective multiplier of synapse e of projection `proj` as the engine applies it.
+float plan_multiplier(const scaffold_env *env, int proj, int64_t e)
+{
+    const Projection &p = env->projs[proj];
+    if (p.val.empty()) return 1.f;
+    const ScProj &sp = env->plan.projs[proj];
+    switch (sp.wprec) {
+    case SC_W_BF16: return bf16_to_f32(f32_to_bf16(p.val[e]));
+    case SC_W_FP8: return e4m3_to_f32(f32_to_e4m3(p.val[e] / sp.scale)) * sp.scale;
+    default: return p.val[e];
+    }
+}
+
+// Multipliers are summed in f64.  The sum of a row is exact when every value is a
+// multiple of 2^q (q from the smallest magnitude) and the row's absolute sum stays
+// below 2^(q+52); then the order of accumulation cannot change the result.
+static bool row_sums_exact(const std::vector<float> &v, const std::vector<int64_t> &row_ptr)
+{
+    int q = 1 << 30;
+    for (float x : v) {
+        if (x == 0.f) continue;
+        if (!std::isfinite(x)) return false;
+        int e;
+        std::frexp(x, &e);  // |x| = f * 2^e, f in [0.5, 1)
+        int qx = std::max(e - 24, -149);
+        q = std::min(q, qx);
+    }
+    if (q == (1 << 30)) return true;
+    const double bound = std::ldexp(1.0, q + 52);
+    for (size_t j = 0; j + 1 < row_ptr.size(); ++j) {
+        double s = 0.0;
+        for (int64_t e = row_ptr[j]; e < row_ptr[j + 1]; ++e) s += std::fabs((double)v[e]);
+        if (s >= bound) return false;
+    }
+    return true;
+}
+
+// ------------------------------------------------------------------- planner
+int plan_build(scaffold_env *env, Plan &P)
+{
+    const Options &opt = env->opt;
+    if (opt.index_bits != 32)
+        return fail(SCAFFOLD_ERR_UNSUPPORTED, "16-bit column offsets are not available in this build");
+    const int npop = env->n_types;
+    P.pops.assign(npop, ScPop{});
+    int dev = 0, ref = 0;
+    for (int p = 0; p < npop; ++p) {
+        const Population &src = env->pops[p];
+        ScPop &d = P.pops[p];
+        const int num = src.set ? src.num : 0;
+        d.dev_base = dev;
+        d.group_base = dev / SC_WAVE;
+        d.ref_base = ref;
+        d.num = num;
+        d.kind = src.set ? src.kind : SCAFFOLD_INPUT;
+        d.c_num = src.c_num;
+        d.dtc = src.Cm > 0 ? kDT / (double)src.Cm : 0.0;
+        d.El = src.El; d.Vr = src.Vr; d.Vth = src.Vth; d.Ie = src.Ie; d.gL = src.gL;
+        d.Vr_span = src.Vth - src.Vr;
+        d.ref_step = (int)((double)src.dt_ref / kDT);
+        for (int c = 0; c < 2; ++c) {
+            d.dtt[c] = (c < src.c_num) ? kDT / (double)src.tau[c] : 0.0;
+            d.E[c] = src.E[c];
+            d.gw[c] = src.g_bar[c] * src.w[c];
+            d.exc_ch[c] = src.E[c] > src.El ? 1 : 0;
+        }
+        dev += (num + SC_WAVE - 1) / SC_WAVE * SC_WAVE;
+        ref += num;
+        if (src.set && src.kind == SCAFFOLD_OUTPUT) P.out_pops.push_back(p);
+    }
+    P.Ndev = dev;
+    P.n_groups = dev / SC_WAVE;
+    P.Wr = P.n_groups + 1;
+    P.total_ref = ref;
+    P.n_output = 0;
+    for (int p : P.out_pops) P.n_output += P.pops[p].num;
+    if ((int64_t)dev >= (1ll << 31) - 64) return fail(SCAFFOLD_ERR_CAPACITY, "too many neurons");
+    const uint32_t sentinel = (uint32_t)P.n_groups * SC_WAVE;  // a bit of the always-zero word
+
+    P.dev_to_ref.assign(dev, -1);
+    P.ref_to_dev.assign(ref, 0);
+    P.groups.clear();
+    for (int p = 0; p < npop; ++p) {
+        const ScPop &d = P.pops[p];
+        for (int i = 0; i < d.num; ++i) {
+            P.dev_to_ref[d.dev_base + i] = d.ref_base + i;
+            P.ref_to_dev[d.ref_base + i] = d.dev_base + i;
+        }
+        const int ng = (d.num + SC_WAVE - 1) / SC_WAVE;
+        for (int gi = 0; gi < ng; ++gi) {
+            ScGroup g{};
+            g.pop = p;
+            g.first_local = gi * SC_WAVE;
+            g.nvalid = std::min(SC_WAVE, d.num - gi * SC_WAVE);
+            g.self_final = 1;
+            P.groups.push_back(g);
+        }
+    }
+
+    // projections (index order fixes the accumulation order per post population)
+    const int nproj = env->n_conn;
+    P.projs.assign(nproj, ScProj{});
+    P.parr.assign(nproj, HostProjArrays{});
+    P.outdeg.assign(dev, 0);
+    P.nnz = 0;
+    P.max_delay = 0;
+    int64_t weight_bytes = 0;
+    for (int pi = 0; pi < nproj; ++pi) {
+        const Projection &src = env->projs[pi];
+        ScProj &sp = P.projs[pi];
+        sp.slot = -1;
+        sp.scale = 1.f;
+        if (!src.set) { sp.mode = -1; continue; }
+        const ScPop &pre = P.pops[src.pre];
+        ScPop &post = P.pops[src.post];
+        sp.delay = src.delay();
+        sp.exc = src.weight > 0 ? 1 : 0;                       // GPUScaffoldSimulation.cu:732
+        sp.W = src.weight > 0 ? src.weight : src.weight * (-1.f); // :733
+        sp.pre_pop = src.pre;
+        sp.post_pop = src.post;
+        sp.chunk = opt.chunk;
+        if (sp.delay > P.R) return fail(SCAFFOLD_ERR_INVALID_ARG, "delay beyond the spike ring");
+        P.max_delay = std::max(P.max_delay, sp.delay);
+        if (post.n_in >= SC_MAX_IN)
+            return fail(SCAFFOLD_ERR_CAPACITY, "population %d receives more than %d projections", src.post, SC_MAX_IN);
+        post.in_proj[post.n_in++] = pi;
+        const int rows = post.num;
+        const int64_t nnz = (int64_t)src.col.size();
+        P.nnz += nnz;
+        // weight storage
+        std::vector<float> eff;  // effective multipliers (empty: unit)
+        if (src.val.empty()) {
+            sp.wprec = SC_W_UNIT;
+        } else if (opt.weight_precision == SCAFFOLD_WEIGHTS_BF16) {
+            sp.wprec = SC_W_BF16;
+            eff.resize(nnz);
+            for (int64_t e = 0; e < nnz; ++e) eff[e] = bf16_to_f32(f32_to_bf16(src.val[e]));
+        } else if (opt.weight_precision == SCAFFOLD_WEIGHTS_FP8) {
+            sp.wprec = SC_W_FP8;
+            float mx = 0.f;
+            for (float x : src.val) mx = std::max(mx, std::fabs(x));
+            sp.scale = mx > 0.f ? std::ldexp(1.f, (int)std::ceil(std::log2((double)mx / 448.0))) : 1.f;
+            eff.resize(nnz);
+            for (int64_t e = 0; e < nnz; ++e) eff[e] = e4m3_to_f32(f32_to_e4m3(src.val[e] / sp.scale)) * sp.scale;
+        } else {
+            sp.wprec = SC_W_F32;
+            eff = src.val;
+        }
+        if (!eff.empty() && !row_sums_exact(eff, src.row_ptr))
+            return fail(SCAFFOLD_ERR_WEIGHT_RANGE,
+                        "projection %d: multipliers span too wide a range for exact f64 row sums", pi);
+        const int wbytes = sp.wprec == SC_W_F32 ? 4 : sp.wprec == SC_W_BF16 ? 2 : sp.wprec == SC_W_FP8 ? 1 : 0;
+        weight_bytes += nnz * wbytes;
+        auto put_w = [&](std::vector<uint8_t> &out, int64_t pos, int64_t e) {
+            if (sp.wprec == SC_W_F32) memcpy(&out[pos * 4], &src.val[e], 4);
+            else if (sp.wprec == SC_W_BF16) { uint16_t h = f32_to_bf16(src.val[e]); memcpy(&out[pos * 2], &h, 2); }
+            else if (sp.wprec == SC_W_FP8) out[pos] = f32_to_e4m3(src.val[e] / sp.scale);
+        };
+        for (int64_t e = 0; e < nnz; ++e) P.outdeg[pre.dev_base + src.col[e]]++;
+        const double mean_fanin = rows > 0 ? (double)nnz / rows : 0.0;
+        HostProjArrays &A = P.parr[pi];
+        if (mean_fanin <= (double)opt.lane_max_fanin) {
+            sp.mode = SC_MODE_LANE;
+            const int ng = (rows + SC_WAVE - 1) / SC_WAVE;
+            A.sell_off.assign(ng, 0);
+            A.sell_w.assign(ng, 0);
+            int64_t total = 0;
+            for (int gi = 0; gi < ng; ++gi) {
+                int w = 0;
+                for (int l = 0; l < SC_WAVE; ++l) {
+                    int j = gi * SC_WAVE + l;
+                    if (j < rows) w = std::max(w, (int)(src.row_ptr[j + 1] - src.row_ptr[j]));
+                }
+                A.sell_off[gi] = total;
+                A.sell_w[gi] = w;
+                total += (int64_t)w * SC_WAVE;
+            }
+            A.col.assign(total, sentinel);
+            if (wbytes) A.val.assign(total * wbytes, 0);
+            for (int gi = 0; gi < ng; ++gi) {
+                for (int l = 0; l < SC_WAVE; ++l) {
+                    int j = gi * SC_WAVE + l;
+                    if (j >= rows) continue;
+                    int k = 0;
+                    for (int64_t e = src.row_ptr[j]; e < src.row_ptr[j + 1]; ++e, ++k) {
+                        int64_t pos = A.sell_off[gi] + (int64_t)k * SC_WAVE + l;
+                        A.col[pos] = (uint32_t)(pre.dev_base + src.col[e]);
+                        if (wbytes) put_w(A.val, pos, e);
+                    }
+                }
+            }
+        } else {
+            sp.mode = SC_MODE_ROW;
+            A.row_ptr = src.row_ptr;
+            A.col.resize(nnz);
+            if (wbytes) A.val.assign(nnz * wbytes, 0);
+            for (int64_t e = 0; e < nnz; ++e) {
+                A.col[e] = (uint32_t)(pre.dev_base + src.col[e]);
+                if (wbytes) put_w(A.val, e, e);
+            }
+            int maxch = 0;
+            for (int j = 0; j < rows; ++j) {
+                int64_t len = src.row_ptr[j + 1] - src.row_ptr[j];
+                maxch = std::max(maxch, (int)((len + sp.chunk - 1) / sp.chunk));
+            }
+            sp.maxch = maxch;
+        }
+    }
+    P.algo_bytes = P.nnz * 4 + weight_bytes + (int64_t)ref * 32 + ref / 8;
+
+    // groups whose population has any row-mode input are finalized by the second kernel
+    std::vector<int> mixed(npop, 0);
+    for (int p = 0; p < npop; ++p) {
+        const ScPop &d = P.pops[p];
+        for (int q = 0; q < d.n_in; ++q)
+            if (P.projs[d.in_proj[q]].mode == SC_MODE_ROW) mixed[p] = 1;
+    }
+    P.n_slots = 0;
+    for (int p = 0; p < npop; ++p) {
+        if (!mixed[p]) continue;
+        const ScPop &d = P.pops[p];
+        for (int q = 0; q < d.n_in; ++q) {
+            ScProj &sp = P.projs[d.in_proj[q]];
+            sp.slot = P.n_slots;
+            P.n_slots += (int64_t)d.num * (sp.mode == SC_MODE_LANE ? 1 : std::max(sp.maxch, 1));
+        }
+    }
+    for (auto &g : P.groups) g.self_final = mixed[g.pop] ? 0 : 1;
+
+    // work list: row chunks first (largest), then lane groups, then stimulus groups
+    P.tasks.clear();
+    P.fin_groups.clear();
+    for (int pi = 0; pi < nproj; ++pi) {
+        const ScProj &sp = P.projs[pi];
+        if (sp.mode != SC_MODE_ROW) continue;
+        const ScPop &post = P.pops[sp.post_pop];
+        const Projection &src = env->projs[pi];
+        for (int j = 0; j < post.num; ++j) {
+            int64_t len = src.row_ptr[j + 1] - src.row_ptr[j];
+            int nch = (int)((len + sp.chunk - 1) / sp.chunk);
+            for (int c = 0; c < nch; ++c) {
+                ScTask t{};
+                t.kind = SC_TASK_ROW;
+                t.group = post.group_base + j / SC_WAVE;
+                t.proj = pi;
+                t.post = j;
+                t.chunk_idx = c;
+                t.e0 = src.row_ptr[j] + (int64_t)c * sp.chunk;
+                t.len = (int)std::min<int64_t>(sp.chunk, len - (int64_t)c * sp.chunk);
+                P.tasks.push_back(t);
+            }
+        }
+    }
+    for (int gid = 0; gid < P.n_groups; ++gid) {
+        const ScGroup &g = P.groups[gid];
+        const ScPop &d = P.pops[g.pop];
+        if (d.kind == SCAFFOLD_INPUT) continue;
+        bool has_lane = false;
+        for (int q = 0; q < d.n_in; ++q)
+            if (P.projs[d.in_proj[q]].mode == SC_MODE_LANE) has_lane = true;
+        if (g.self_final || has_lane) {
+            ScTask t{};
+            t.kind = SC_TASK_LANE;
+            t.group = gid;
+            P.tasks.push_back(t);
+        }
+        if (!g.self_final) P.fin_groups.push_back(gid);
+    }
+    for (int gid = 0; gid < P.n_groups; ++gid) {
+        const ScGroup &g = P.groups[gid];
+        if (P.pops[g.pop].kind != SCAFFOLD_INPUT) continue;
+        ScTask t{};
+        t.kind = SC_TASK_STIM;
+        t.group = gid;
+        P.tasks.push_back(t);
+    }
+
+    // stimulus ownership: a later stimulus (and a later entry) overwrites an earlier
+    // one for a shared id (UserInputFunctions.cu:118)
+    P.stim_s.assign(dev, -1);
+    P.stim_e.assign(dev, 0);
+    for (size_t s = 0; s < env->stims.size(); ++s) {
+        const Stimulus &st = env->stims[s];
+        const ScPop &d = P.pops[st.pop];
+        const int ne = st.n_entries();
+        for (int i = 0; i < ne; ++i) {
+            int64_t b = st.fanout.empty() ? i : st.fanout[i];
+            int64_t e = st.fanout.empty() ? i + 1 : st.fanout[i + 1];
+            for (int64_t t = b; t < e; ++t) {
+                int di = d.dev_base + (int)st.targets[t];
+                P.stim_s[di] = (int32_t)s;
+                P.stim_e[di] = i;
+            }
+        }
+    }
+    P.stims = dev_stims(env);
+    return SCAFFOLD_OK;
+}
+
+}  // namespace sc

@@ -1,0 +1,327 @@
+/*
+ * scaffold.h -- C ABI of the MI355X (gfx950) cerebellar-scaffold exchange-step engine.
+ *
+ * The entry points mirror the reference simulator's lifecycle API and its
+ * network-description plug-ins so that a caller of the reference can switch
+ * over call by call.  Each declaration names the reference interface it
+ * replaces (paths relative to the reference repository,
+ * SimulationModule/Scaffold/ unless stated otherwise).
+ *
+ * Conventions
+ *   - Every function returns SCAFFOLD_OK (0) or a positive SCAFFOLD_ERR_* code;
+ *     scaffold_last_error() returns a thread-local message for the last failure.
+ *     (The reference returns nothing and calls exit() on errors:
+ *     Simulation_header.h:11-18, init_params.cu:33-36.)
+ *   - Plain pointers and sizes only.  All device memory is owned by the library.
+ *   - Host pointers returned through out-parameters stay valid until the next
+ *     call on the same environment.
+ *   - An environment is not re-entrant: one caller thread at a time
+ *     (as the reference's static state, GPUScaffoldSimulation.cu:697).
+ *   - Neuron indices in queries are the reference's global ids: populations in
+ *     internal index order, each occupying [base_id, base_id + num)
+ *     (init_params.cu:334-341).
+ */
+#ifndef SCAFFOLD_H
+#define SCAFFOLD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define SCAFFOLD_OK                0
+#define SCAFFOLD_ERR_INVALID_ARG   1  /* bad argument value or null pointer           */
+#define SCAFFOLD_ERR_STATE         2  /* call out of lifecycle order                   */
+#define SCAFFOLD_ERR_IO            3  /* file cannot be opened / read                  */
+#define SCAFFOLD_ERR_FORMAT        4  /* .dat malformed, unsorted or ids out of range  */
+#define SCAFFOLD_ERR_UNSUPPORTED   5  /* feature outside this engine (e.g. >1 device)  */
+#define SCAFFOLD_ERR_HIP           6  /* HIP runtime error                             */
+#define SCAFFOLD_ERR_NO_DEVICE     7  /* no gfx950 device visible                      */
+#define SCAFFOLD_ERR_WEIGHT_RANGE  8  /* weights whose row sums cannot be exact in f64 */
+#define SCAFFOLD_ERR_CAPACITY      9  /* a size limit of the engine is exceeded        */
+
+const char *scaffold_last_error(void);
+/* Library version string, e.g. "scaffold-mi355x 0.1 gfx950". */
+const char *scaffold_version(void);
+
+/* ------------------------------------------------------------ enums/types */
+/* DeviceType (struct_enum_def.h:54-58). */
+#define SCAFFOLD_NORMAL 0
+#define SCAFFOLD_INPUT  1
+#define SCAFFOLD_OUTPUT 2
+
+/* Synaptic weight storage (the per-synapse multiplier of w_bar). */
+#define SCAFFOLD_WEIGHTS_F32  0
+#define SCAFFOLD_WEIGHTS_BF16 1
+#define SCAFFOLD_WEIGHTS_FP8  2   /* OCP e4m3 with a per-projection power-of-two scale */
+
+/* Stimulus function expressed as data.  It replaces the device function-pointer
+ * table d_pInputFunctions[] (UserInputFunctions.cu:83) and its entries
+ * background_noise / tone_stim (UserInputFunctions.cu:19-29), built from
+ * PoissonProcess / PeriodicFiring (UserInputFunctions.cu:4-10).
+ *
+ * For a list entry and sub-step time t (ms, float), with uniform r:
+ *   fire = (poisson_on  <= t < poisson_off  && r < poisson_hz/1000*0.1)
+ *       || (burst_on    <= t < burst_off    && r < burst_hz/1000*0.1)
+ *       || (periodic_on <= t < periodic_off(+inclusive) && PeriodicFiring(t, periodic_hz, periodic_start))
+ * A rate of 0 disables a component.  Windows are compared in double precision. */
+typedef struct {
+    float  poisson_hz;
+    double poisson_on, poisson_off;
+    float  burst_hz;
+    double burst_on, burst_off;
+    float  periodic_hz;
+    float  periodic_start;
+    double periodic_on, periodic_off;
+    int    periodic_off_inclusive;
+    /* 1: scaffold_set_input_freq() overrides poisson_hz for this stimulus.
+     * 0: the rate is fixed, as in the reference kernel, which ignores the
+     *    SetInputFreq value (UserInputFunctions.cu:89,116). */
+    int    poisson_hz_from_freq;
+    /* Philox4x32-10 key of this stimulus: curand_init(seed, subsequence=i, 0)
+     * per list entry i (GPUScaffoldSimulation.cu:329, Simulation_header.cu:29-34). */
+    uint64_t seed;
+} scaffold_stim_func;
+
+/* Fill *out with the reference's table entry func_id:
+ * 0 = background_noise (UserInputFunctions.cu:19-23), 1 = tone_stim (:25-29). */
+int scaffold_stim_func_reference(int func_id, uint64_t seed, scaffold_stim_func *out);
+
+typedef struct scaffold_env   scaffold_env;
+typedef struct scaffold_batch scaffold_batch;
+
+/* ------------------------------------------------------------- lifecycle */
+/* CreateNetworkEnv (GPUScaffoldSimulation.h:41, GPUScaffoldSimulation.cu:634-657).
+ * num_of_dev must be 1: this engine drives one GPU. */
+int scaffold_create_network_env(scaffold_env **env, int num_of_dev,
+                                int num_of_neuron_types, int num_of_connectivity_types,
+                                int num_of_inputs, int num_of_plasticities);
+
+/* set_neuron_params (init_params.cu:255-303).  NORMAL populations receive the
+ * internal indices 0,1,2,... in call order; INPUT/OUTPUT populations receive
+ * num_of_neuron_types-1, -2, ... (init_params.cu:259-265).  `channels` holds
+ * c_num x {w, E, g_bar, tau} (the variadic doubles of the reference); c_num <= 2.
+ * *index_out receives the internal index (the reference's NeuronTypeID[type]). */
+int scaffold_set_neuron_params(scaffold_env *env, int type, const char *name,
+                               int duplicate, int num,
+                               float Cm, float El, float dt_ref, float Ie,
+                               float Vr, float Vth, float gL,
+                               int device_type, int c_num, const double *channels,
+                               int *index_out);
+
+/* set_connectivity_params (init_params.cu:304-332) with the .dat loader
+ * GetMaxConv + LoadConnectivityFile_ELL (init_params.cu:13-67,96-128).
+ * File format: one "pre<TAB>post" pair per line, population-local ids, sorted by
+ * (post, pre) (convertHDF52dat/convert.py:79-81).  preType/postType are internal
+ * population indices.  initial_weight > 0 feeds the excitatory buffer, < 0 the
+ * inhibitory one with |w| (GPUScaffoldSimulation.cu:732-733).  delay is in ms
+ * (whole exchange steps, truncated as (int)delay, :734).  Every stored synapse
+ * has weight multiplier 1.0 as in the reference (init_params.cu:46).
+ * postNum may exceed the post population size (rows beyond it are ignored,
+ * as in init_params.cu:743); pairs with pre >= preNum or post >= postNum are a
+ * format error here (the reference reads out of bounds). */
+int scaffold_set_connectivity_params(scaffold_env *env, int type, const char *filename,
+                                     int preNum, int postNum, int preType, int postType,
+                                     float initial_weight, float delay,
+                                     int use_parallel_reduction, int *index_out);
+
+/* In-memory variant: CSR by post row.  row_ptr has postNum+1 entries (int64),
+ * col holds population-local pre ids, val the per-synapse multipliers of
+ * |initial_weight| (NULL = all 1.0, the reference loader's value). */
+int scaffold_set_connectivity_csr(scaffold_env *env, int type,
+                                  int preNum, int postNum, int preType, int postType,
+                                  float initial_weight, float delay,
+                                  const int64_t *row_ptr, const int32_t *col, const float *val,
+                                  int *index_out);
+
+/* setInputStim (UserInputFunctions.cu:41-70): read one unsigned id per line from
+ * id_file; entry i drives neuron id_i of population neuron_index.  Stimuli are
+ * applied in registration order; a later stimulus overwrites an earlier one for
+ * a shared id (UserInputFunctions.cu:118). */
+int scaffold_set_input_stim(scaffold_env *env, int neuron_index, const char *id_file,
+                            const scaffold_stim_func *func, int *index_out);
+
+/* In-memory variant.  With fanout_ptr == NULL entry i drives ids[i] (n entries).
+ * Otherwise entry i drives ids[fanout_ptr[i] .. fanout_ptr[i+1]) (a mossy fibre
+ * feeding several glomeruli); fanout_ptr has n+1 entries. */
+int scaffold_set_input_stim_ids(scaffold_env *env, int neuron_index,
+                                const uint32_t *ids, int n, const int64_t *fanout_ptr,
+                                const scaffold_stim_func *func, int *index_out);
+
+/* Engine options; set before scaffold_initialize_network_env.  Keys:
+ *   "neuron_seed"       Philox key of the initial-potential draw (InitParams,
+ *                       init_params.cu:343-352); default 0x5eed
+ *   "weight_precision"  SCAFFOLD_WEIGHTS_*; default F32
+ *   "index_bits"        32 or 16 (16-bit pre offsets within 65,536-neuron tiles)
+ *   "lane_max_fanin"    projections with mean fan-in <= this are gathered one
+ *                       lane per post neuron; others one wavefront per row chunk
+ *   "chunk"             synapses per row chunk (multiple of 64)
+ *   "graph"             1: replay captured hipGraphs of the step kernels
+ *   "graph_steps"       exchange steps per captured graph
+ *   "record_fired"      1: keep a compacted fired-neuron list per step
+ *   "reset_g"           1: also zero conductances in scaffold_reset_network_env
+ *                       (the reference keeps them, GPUScaffoldSimulation.cu:802-854) */
+int scaffold_set_option(scaffold_env *env, const char *key, int64_t value);
+int scaffold_get_option(scaffold_env *env, const char *key, int64_t *value);
+
+/* InitializeNetworkEnv (GPUScaffoldSimulation.cu:659-691): builds the device
+ * layout, uploads it, draws the initial state twice (Initialization :381 and the
+ * ResetNetworkEnv(env,0) at :689), so trial 0 uses each neuron's second Philox
+ * output.  T_print is raised to the maximum delay; T_print must be a multiple of
+ * T_cpu_sim (:674-684).  The spike ring has 2*T_print rows (:361). */
+int scaffold_initialize_network_env(scaffold_env *env, int T_print, int T_cpu_sim);
+
+/* loop_n_steps (GPUScaffoldSimulation.cu:693-791).  Advances n_steps exchange
+ * steps (1 ms = 10 Euler sub-steps of 0.1 ms each) starting at trial-relative
+ * step start_step.  n_steps must be a multiple of T_cpu_sim.  If
+ * output_spike_train is not NULL it receives a library-owned host buffer of
+ * T_cpu_sim x (number of OUTPUT neurons) bytes: the OUTPUT-population spikes of
+ * the last T_cpu_sim steps of this call (row-major by step), valid until the
+ * next call.  (The reference hands over the buffer of its lagging host thread,
+ * :777.) */
+int scaffold_loop_n_steps(scaffold_env *env, int start_step, int n_steps, int trial,
+                          const char **output_spike_train);
+
+/* SetInputFreq (GPUScaffoldSimulation.cu:883-890): freq_list[i] is the Poisson
+ * rate (Hz) of stimulus i; it takes effect for stimuli registered with
+ * poisson_hz_from_freq = 1. */
+int scaffold_set_input_freq(scaffold_env *env, const float *freq_list);
+
+/* ResetNetworkEnv (GPUScaffoldSimulation.cu:802-854): redraws the membrane
+ * potentials (next Philox output), zeroes dg and refractory counters and ring
+ * row 0.  Conductances and stimulus streams carry over, as in the reference. */
+int scaffold_reset_network_env(scaffold_env *env, int trial);
+
+/* FinalizeNetworkEnv (GPUScaffoldSimulation.cu:856-880); also frees *env. */
+int scaffold_finalize_network_env(scaffold_env *env);
+
+/* ---------------------------------------------------- description helpers */
+/* The reference's hard-coded description plug-ins init_neurons_params +
+ * init_connectivity_params (init_params.cu:367-766) and InitInputStimulation
+ * (UserInputFunctions.cu:72-82), reading the .dat files from data_dir.
+ * seeds: neuron_seed applies to InitParams, stim_seed to stimulus i as
+ * stim_seed + i. */
+int scaffold_describe_reference(scaffold_env *env, const char *data_dir,
+                                uint64_t stim_seed);
+
+/* Seeded synthetic scaffold (the network the benchmark configurations name):
+ * populations placed on an x-z sheet of side 400*sqrt(scale) um, rule-based
+ * connectivity, mossy fibres each feeding a set of glomeruli. */
+typedef struct {
+    double   scale;              /* sheet area / (400 um x 400 um)                       */
+    int32_t  counts[8];          /* 0 = derive from scale.  Order: granule, glomerulus,
+                                    purkinje, golgi, stellate, basket, dcn, mossy fibre */
+    uint64_t seed;               /* connectome seed                                      */
+    int32_t  weight_jitter;      /* 1: multipliers U(0.8,1.2); 0: all 1.0                */
+    float    mf_rate_hz;         /* background rate of every mossy fibre                 */
+    float    burst_rate_hz;      /* burst rate of the patch                              */
+    double   burst_on_ms, burst_off_ms;
+    int32_t  burst_n_mf;         /* patch size (mossy fibres nearest the sheet centre)   */
+    double   burst_fraction;     /* used when burst_n_mf == 0                            */
+    uint64_t stim_seed;
+    int32_t  max_delay;          /* delays are clipped to [1, max_delay] steps           */
+    int32_t  reference_delays;   /* 1: use the reference's delays (1..5 ms)              */
+} scaffold_synth_config;
+
+/* Presets: which = 0 (small, ~9.9k cells), 1 (reference size, ~96.7k cells),
+ * 2 (16x area), 4 (100x area).  Other fields get the benchmark defaults. */
+int scaffold_synth_config_preset(int which, scaffold_synth_config *cfg);
+int scaffold_describe_synthetic(scaffold_env *env, const scaffold_synth_config *cfg);
+
+/* Write projection `index` in the reference .dat format (pre<TAB>post, sorted). */
+int scaffold_write_dat(scaffold_env *env, int index, const char *filename);
+
+/* ----------------------------------------------------------------- queries */
+typedef struct {
+    int32_t  n_populations, n_projections, n_stimuli;
+    int32_t  total_neurons;        /* reference global id space                */
+    int32_t  device_neurons;       /* padded to 64 per population on the GPU   */
+    int32_t  T_print, T_cpu_sim, ring_rows, max_delay;
+    int64_t  total_synapses;
+    int64_t  steps_since_init;     /* stimulus Philox draws so far             */
+    int32_t  current_step;         /* trial-relative step of the next call     */
+    int32_t  n_output_neurons;
+    int64_t  device_bytes;         /* device memory held by this environment   */
+    int64_t  algorithmic_bytes_per_step; /* nnz*(idx+w bytes) + N*32 + N/8     */
+    int32_t  n_tasks, n_groups;
+} scaffold_info;
+int scaffold_get_info(scaffold_env *env, scaffold_info *out);
+
+typedef struct {
+    char     name[32];
+    int32_t  type, device_type, num, base_id, c_num;
+    float    Cm, El, dt_ref, Ie, Vr, Vth, gL;
+    float    w[2], E[2], g_bar[2], tau[2];
+    int32_t  ref_step;
+} scaffold_population_info;
+int scaffold_get_population(scaffold_env *env, int index, scaffold_population_info *out);
+
+typedef struct {
+    int32_t type, pre, post, pre_num, post_num, delay;
+    float   initial_weight, w_bar;
+    int64_t nnz;
+    int32_t max_conv;          /* largest row (the reference's ELL width)      */
+    int32_t lane_mode;         /* 1: gathered one lane per post neuron         */
+    float   fp8_scale;         /* power-of-two scale of FP8 multipliers        */
+} scaffold_projection_info;
+int scaffold_get_projection(scaffold_env *env, int index, scaffold_projection_info *out);
+/* Copy the CSR of a projection (row_ptr: post_num+1, col, val: nnz).  val is the
+ * multiplier as the engine applies it (after BF16/FP8 rounding when selected). */
+int scaffold_get_projection_csr(scaffold_env *env, int index,
+                                int64_t *row_ptr, int32_t *col, float *val);
+
+typedef struct {
+    int32_t  pop, n_entries, n_targets;
+    scaffold_stim_func func;
+} scaffold_stim_info;
+int scaffold_get_stimulus(scaffold_env *env, int index, scaffold_stim_info *out,
+                          int64_t *fanout_ptr /* n_entries+1 or NULL */,
+                          uint32_t *targets /* n_targets or NULL */);
+
+/* Neuron state in global-id order (V: total_neurons, g: total_neurons x 2,
+ * refr: total_neurons).  Any pointer may be NULL. */
+int scaffold_get_state(scaffold_env *env, float *V, float *g, int32_t *refr);
+int scaffold_set_state(scaffold_env *env, const float *V, const float *g, const int32_t *refr);
+
+/* Spike row of trial-relative step `step` (one byte per neuron, global ids);
+ * the step must be among the last ring_rows steps. */
+int scaffold_get_spike_row(scaffold_env *env, int step, uint8_t *spikes);
+/* Per-population spike counts (n_populations) and synaptic events
+ * (sum over fired neurons of their out-degree) of step `step`. */
+int scaffold_get_counts(scaffold_env *env, int step, int32_t *counts, uint64_t *events);
+/* Compacted fired-neuron list of step `step` (global ids, ascending), built on
+ * the GPU with wavefront ballot/popcount and a prefix sum. */
+int scaffold_get_fired(scaffold_env *env, int step, int32_t *ids, int32_t cap, int32_t *n);
+/* Write the spikes of trial-relative steps [step0, step0+n) in the reference's
+ * text format "%f\t%d\t%d\n" = (ms, population-local id, population type)
+ * (GPUScaffoldSimulation.cu:81), appending to filename. */
+int scaffold_write_spike_text(scaffold_env *env, int step0, int n, const char *filename);
+
+/* Timing helper for benchmarks: advance n_steps (continuing the trial) and
+ * return the device time measured with HIP events on the engine stream, and the
+ * mean duration of one step kernel measured around individual launches. */
+int scaffold_time_steps(scaffold_env *env, int n_steps, double *ms_total,
+                        double *ms_step_kernel_mean);
+
+/* --------------------------------------------------------------- batching */
+/* Many initialized environments of uneven size advanced together: every step
+ * kernel covers all instances, and runs of steps are replayed as hipGraphs. */
+int scaffold_batch_create(scaffold_batch **batch);
+int scaffold_batch_add(scaffold_batch *batch, scaffold_env *env);
+int scaffold_batch_build(scaffold_batch *batch);
+int scaffold_batch_run(scaffold_batch *batch, int n_steps);
+int scaffold_batch_time(scaffold_batch *batch, int n_steps, double *ms_total);
+int scaffold_batch_destroy(scaffold_batch *batch);
+
+/* ------------------------------------------------------------ diagnostics */
+/* Device copy of `bytes` bytes between two library-allocated scratch buffers;
+ * used to calibrate HBM counters against a transfer of known size. */
+int scaffold_copy_probe(int64_t bytes, int iters, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SCAFFOLD_H */
