@@ -136,6 +136,7 @@ def load_library(path=None):
         "scaffold_get_spike_row": (ctypes.c_int, [vp, ctypes.c_int, P(ctypes.c_uint8)]),
         "scaffold_get_counts": (ctypes.c_int, [vp, ctypes.c_int, P(i32), P(u64)]),
         "scaffold_get_fired": (ctypes.c_int, [vp, ctypes.c_int, P(i32), i32, P(i32)]),
+        "scaffold_get_totals": (ctypes.c_int, [vp, P(u64), P(u64)]),
         "scaffold_write_spike_text": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, cs]),
         "scaffold_time_steps": (ctypes.c_int, [vp, ctypes.c_int, P(f64), P(f64)]),
         "scaffold_batch_create": (ctypes.c_int, [P(vp)]),
@@ -367,6 +368,14 @@ class NetworkEnvironment:
         ev = ctypes.c_uint64()
         _check(self._lib.scaffold_get_counts(self._h, step, _ptr(c, ctypes.c_int32), ctypes.byref(ev)))
         return c, ev.value
+
+    def totals(self):
+        """(spikes per population, synaptic events) accumulated since initialization."""
+        n = self.info().n_populations
+        sp = np.zeros(n, np.uint64)
+        ev = ctypes.c_uint64()
+        _check(self._lib.scaffold_get_totals(self._h, _ptr(sp, ctypes.c_uint64), ctypes.byref(ev)))
+        return sp, ev.value
 
     def fired(self, step, cap=None):
         cap = self.info().total_neurons if cap is None else cap

@@ -386,11 +386,10 @@ int scaffold_set_option(scaffold_env *env, const char *key, int64_t value)
         if (frozen) return frozen_err();
         if (value < 0 || value > 4096) return fail(SCAFFOLD_ERR_INVALID_ARG, "lane_max_fanin in [0,4096]");
         o.lane_max_fanin = (int)value;
-    } else if (!strcmp(key, "chunk")) {
+    } else if (!strcmp(key, "lane_synapses")) {
         if (frozen) return frozen_err();
-        if (value < 64 || value > (1 << 20) || value % 64)
-            return fail(SCAFFOLD_ERR_INVALID_ARG, "chunk must be a multiple of 64 in [64, 2^20]");
-        o.chunk = (int)value;
+        if (value < 1 || value > 4096) return fail(SCAFFOLD_ERR_INVALID_ARG, "lane_synapses in [1, 4096]");
+        o.lane_synapses = (int)value;
     } else if (!strcmp(key, "graph")) o.graph = value ? 1 : 0;
     else if (!strcmp(key, "graph_steps")) {
         if (value < 1 || value > 100000) return fail(SCAFFOLD_ERR_INVALID_ARG, "graph_steps in [1,100000]");
@@ -412,7 +411,7 @@ int scaffold_get_option(scaffold_env *env, const char *key, int64_t *value)
     else if (!strcmp(key, "weight_precision")) *value = o.weight_precision;
     else if (!strcmp(key, "index_bits")) *value = o.index_bits;
     else if (!strcmp(key, "lane_max_fanin")) *value = o.lane_max_fanin;
-    else if (!strcmp(key, "chunk")) *value = o.chunk;
+    else if (!strcmp(key, "lane_synapses")) *value = o.lane_synapses;
     else if (!strcmp(key, "graph")) *value = o.graph;
     else if (!strcmp(key, "graph_steps")) *value = o.graph_steps;
     else if (!strcmp(key, "record_fired")) *value = o.record_fired;
@@ -472,7 +471,6 @@ int scaffold_loop_n_steps(scaffold_env *env, int start_step, int n_steps, int tr
     const int R = env->plan.R;
     if (start_step != env->cur_step) {
         rc = engine_set_ctl(env->eng, start_step, env->draws);
-        if (!rc) rc = engine_clear_counts_row(env->eng, start_step % R);
         if (rc) return rc;
         env->cur_step = start_step;
     }
@@ -785,6 +783,20 @@ int scaffold_get_counts(scaffold_env *env, int step, int32_t *counts, uint64_t *
     rc = engine_get_counts(env->eng, row, counts, &ev);
     if (events) *events = ev;
     return rc;
+}
+
+int scaffold_get_totals(scaffold_env *env, uint64_t *spikes, uint64_t *events)
+{
+    clear_error();
+    int rc = check_ready(env);
+    if (rc) return rc;
+    std::vector<unsigned long long> t((size_t)env->n_types + 1, 0ull);
+    rc = engine_get_totals(env->eng, t.data());
+    if (rc) return rc;
+    if (spikes)
+        for (int p = 0; p < env->n_types; ++p) spikes[p] = t[p];
+    if (events) *events = t[env->n_types];
+    return SCAFFOLD_OK;
 }
 
 int scaffold_get_fired(scaffold_env *env, int step, int32_t *ids, int32_t cap, int32_t *n)

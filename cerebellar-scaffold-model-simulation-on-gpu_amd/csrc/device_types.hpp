@@ -6,22 +6,21 @@
 //     64-neuron group of a single population.
 //   * Neuron state is SoA: V[Ndev] f32, g[2*Ndev] f32 (exc, inh interleaved),
 //     refr[Ndev] i32.
-//   * Spike ring: R rows x Wr 64-bit words.  Word g of row r is the wavefront
-//     ballot of group g at the step s with s mod R == r.  The last word of every
-//     row stays zero; padding entries of sliced-ELL slices point into it.
-//   * Projections are gathered in one of two modes:
-//       lane mode: sliced ELL, one slice per 64 post neurons, column-major inside
-//                  the slice (lane l reads entry k of its row at slice + 64*k + l),
-//                  so every load of a wavefront is one contiguous 256-byte run;
-//       row mode:  CSR rows cut into chunks of `chunk` synapses, one wavefront per
-//                  chunk, lanes striding over the chunk (coalesced).
-//     Column entries are device bit indices of the presynaptic neuron (32-bit), or
-//     16-bit offsets from a per-slice / per-chunk base.
-//   * Row-mode chunks (and lane-mode projections of groups that also receive
-//     row-mode input) leave f64 partial sums in slots.  With multipliers that pass
-//     the exactness check every partial is the exact sum, so the order in which
-//     slots are combined cannot change the result and no floating-point atomics
-//     are used anywhere.
+//   * Spike ring: R rows x Wr 64-bit words.  Word g of row r holds the spikes of
+//     group g at the step s with s mod R == r.  The last word of every row stays
+//     zero; padding entries of sliced-ELL slices point into it.
+//   * Synapses are gathered in one of two layouts, chosen per post population:
+//       lane mode: sliced ELL per projection, one slice per 64 post neurons,
+//                  column-major inside the slice (lane l reads entry k of its row
+//                  at slice + 64*k + l), so every wavefront load is one contiguous
+//                  256-byte run; one lane per post neuron;
+//       team mode: one merged CSR row per post neuron with the synapses of all its
+//                  incoming projections (projection after projection, in index
+//                  order); a team of 1..16 wavefronts strides over the row.
+//     Column entries are device bit indices of the presynaptic neuron.
+//   * Row sums are accumulated in f64.  With multipliers that pass the exactness
+//     check every sum is exact, so the order in which lanes and wavefronts combine
+//     them cannot change the result; no floating-point atomics are used anywhere.
 #pragma once
 #include <stdint.h>
 
@@ -35,14 +34,14 @@
 #define SC_W_FP8  2
 #define SC_W_UNIT 3
 
-// projection gather modes
+// gather layout of a projection (that of its post population)
 #define SC_MODE_LANE 0
-#define SC_MODE_ROW  1
+#define SC_MODE_ROW  1   // team mode
 
 // task kinds
 #define SC_TASK_STIM 0
 #define SC_TASK_LANE 1
-#define SC_TASK_ROW  2
+#define SC_TASK_ROW  2   // one team-mode neuron
 
 struct ScPop {
     double  dtc;           // DT / Cm
@@ -63,33 +62,21 @@ struct ScPop {
     int32_t in_proj[SC_MAX_IN];
 };
 
-struct ScProj {
-    const uint32_t *col;       // 32-bit device bit indices
-    const uint16_t *col16;     // 16-bit offsets from a per-slice / per-chunk base
-    const void     *val;       // f32 / bf16 / fp8 multipliers; nullptr when unit
-    const int64_t  *row_ptr;   // row mode: CSR row pointers by post local id
-    const int64_t  *sell_off;  // lane mode: slice offset per post group
-    const int32_t  *sell_w;    // lane mode: slice width per post group
-    const int32_t  *sell_base; // lane mode, 16-bit columns: base bit index per group
-    int64_t slot;              // first partial slot of this projection (-1: none)
-    float   W;                 // |w_bar|
-    float   scale;             // FP8 multiplier scale (a power of two), else 1
+struct ScProj {            // host-side summary of one projection
+    float   W;             // |w_bar|
+    float   scale;         // FP8 multiplier scale (a power of two), else 1
     int32_t delay;
     int32_t exc;
-    int32_t mode;
+    int32_t mode;          // SC_MODE_LANE / SC_MODE_ROW, -1: not registered
     int32_t wprec;
-    int32_t idx16;
     int32_t pre_pop, post_pop;
-    int32_t chunk;
-    int32_t maxch;
-    int32_t pad_;
 };
 
 struct ScGroup {
     int32_t pop;
     int32_t first_local;   // population-local id of lane 0
     int32_t nvalid;        // valid lanes
-    int32_t self_final;    // 1: the lane task finalizes the group directly
+    int32_t self_final;
 };
 
 struct ScStim {
@@ -107,20 +94,75 @@ struct ScStim {
 struct ScCtl {
     int64_t n;      // trial-relative step of launch offset k = 0
     int64_t m;      // stimulus draws since init (3 Philox blocks per draw)
+    int32_t row;    // ring row of step n (n mod R)
+    int32_t pad_;
 };
 
-// One task is processed by one wavefront.
+// Team-mode column words: projection index (segment of the merged row) in the top
+// bits, presynaptic device bit index below.
+#define SC_Q_SHIFT  29
+#define SC_BIT_MASK 0x1FFFFFFFu
+
+// Planner form of one unit of work.
 struct ScTask {
     int32_t kind;
     int32_t inst;
     int32_t group;      // instance-wide group index
-    int32_t proj;       // row mode: projection
-    int32_t post;       // row mode: post population-local id
-    int32_t chunk_idx;  // row mode: chunk index within the row
-    int32_t len;        // row mode: synapses in the chunk
-    int32_t col_base;   // row mode, 16-bit columns: base bit index
-    int64_t e0;         // row mode: first synapse of the chunk
-    int64_t pad_;
+    int32_t proj;       // team mode: population
+    int32_t post;       // team mode: population-local neuron id
+    int32_t pad_[3];
+};
+
+// ---- lowered work descriptors: each holds absolute device pointers, so a
+// wavefront reaches its data after one descriptor load instead of a chain of
+// table lookups (every dependent global load costs on the order of a microsecond).
+#define SC_MAX_SLICES 3    // projections of a lane-mode population
+#define SC_MAX_TEAM   16   // wavefronts per team-mode neuron (one 1024-thread block)
+#define SC_BLOCK      1024 // threads per block of the step kernel
+
+struct ScSlice {           // one lane-mode projection of one 64-neuron group
+    const uint32_t *col;   // slice columns, 64 x width, column-major
+    const void *val;       // slice multipliers (nullptr: unit)
+    int32_t width, delay, wprec, exc;
+    float W, scale;
+};
+
+struct ScLaneTask {        // 64 neurons of a lane-mode population
+    int32_t inst, group, pop, n_sl;
+    int32_t nvalid, pad0, pad1, pad2;
+    ScSlice sl[SC_MAX_SLICES];
+};
+
+struct ScStimTask {        // 64 neurons of an INPUT population
+    int32_t inst, group, pop, nvalid;
+};
+
+struct ScTeamIn {          // one incoming projection of a team-mode neuron
+    int32_t end;           // end of its segment in the merged row
+    int32_t delay, exc, pad_;
+    float W, scale;
+};
+
+struct ScTeamTask {        // one neuron of a team-mode population
+    const uint32_t *col;   // merged row columns
+    const void *val;       // merged row multipliers (nullptr: unit)
+    int32_t inst, group, bit, pop;
+    int32_t n_in, wprec, len, pad_;
+    ScTeamIn in[SC_MAX_IN];
+};
+
+// Block layout of the step kernel (a kernel argument).  Blocks are grouped in
+// classes: team neurons of width 16, 8, 4, 2, 1 wavefronts (classes 0..4, with
+// 16/width neurons per block), lane groups (class 5) and stimulus groups (class 6),
+// 16 wavefronts per block.
+#define SC_CLASSES    7
+#define SC_CLASS_LANE 5
+#define SC_CLASS_STIM 6
+struct ScLayout {
+    int32_t cls_end[SC_CLASSES];    // cumulative block count at the end of each class
+    int32_t cls_ntask[SC_CLASSES];  // tasks of each class
+    int32_t cls_task0[SC_CLASSES];  // first descriptor index of each class
+    int32_t pad_;
 };
 
 struct ScInstance {
@@ -128,16 +170,15 @@ struct ScInstance {
     float    *g;
     int32_t  *refr;
     uint64_t *ring;
-    double   *partial;
     int32_t  *counts;          // R x n_pops
     unsigned long long *events; // R
+    unsigned long long *totals; // n_pops spike totals, then the event total (since initialization)
     const ScGroup *groups;
     const ScPop   *pops;
-    const ScProj  *projs;
     const ScStim  *stims;
     const int32_t *stim_s;     // per device index of INPUT groups: stimulus (-1: none)
     const int32_t *stim_e;     // entry index
     const uint32_t *outdeg;    // per device index
     ScCtl   *ctl;
-    int32_t R, Wr, n_groups, n_pops, n_projs, n_stims, Ndev, pad_;
+    int32_t R, Wr, n_groups, n_pops, n_stims, Ndev, pad0, pad1;
 };

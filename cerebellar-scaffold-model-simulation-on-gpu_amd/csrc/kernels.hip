@@ -1,0 +1,590 @@
+// kernels.hip -- gfx950 kernels of the exchange-step hot path and their launchers.
+//
+// One exchange step (1 ms = 10 Euler sub-steps of 0.1 ms) is one launch of k_step
+// (1024-thread blocks, each holding tasks of one kind):
+//   LANE  a wavefront per 64 post neurons of a lane-mode population (short rows):
+//         one lane per neuron over the sliced-ELL slices of its projections, LIF
+//         update, spike word = wavefront ballot, plain store;
+//   TEAM  1..16 wavefronts per post neuron of a team-mode population (long rows):
+//         the team strides over the neuron's merged CSR row and accumulates the
+//         exact f64 sum of every incoming projection in LDS; one lane runs the LIF
+//         update and sets the neuron's bit in the spike word with an integer
+//         atomicOr;
+//   STIM  a wavefront per 64 INPUT neurons: counter-based Philox4x32-10 draws and
+//         the stimulus rule, spike word = ballot.
+// A step reads ring rows of earlier steps only (every delay >= 1) and writes the
+// row of the current step, so blocks never wait on each other; the kernel
+// boundary orders consecutive steps.  k_stats popcounts the spike words of a run
+// of steps into per-population counts and sums the out-degrees of the spiking
+// neurons (synaptic events); k_advance moves the device step counter.  Runs of
+// steps are captured once as a hipGraph and replayed.
+//
+// Latency: the work per step is tens of MB, so the step time is set by chains of
+// dependent memory accesses and by instruction fetch.  Every task is a descriptor
+// with absolute pointers (one dependent load to reach the data); the gathers
+// issue a whole batch of column, multiplier and spike-word loads before consuming
+// any of them; and the code a block runs is kept small (loops that are serial
+// anyway -- Euler sub-steps, Philox rounds -- are not unrolled), because a block
+// executes its path once per step and a long straight-line path stalls on
+// instruction fetch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+
+#include "kernels.hpp"
+
+namespace sc {
+
+// ------------------------------------------------------------------ device math
+// Philox4x32-10 round (Salmon et al., SC'11) on counter (c0,c1,c2,c3) with key (k0,k1).
+__device__ __forceinline__ void philox_round(uint32_t &c0, uint32_t &c1, uint32_t &c2, uint32_t &c3, uint32_t k0,
+                                             uint32_t k1)
+{
+    const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+    const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+    const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+    c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+}
+
+__device__ __forceinline__ void philox10(uint32_t &c0, uint32_t &c1, uint32_t &c2, uint32_t &c3, uint32_t k0,
+                                         uint32_t k1)
+{
+#pragma unroll 1
+    for (int r = 0; r < 10; ++r) {
+        philox_round(c0, c1, c2, c3, k0, k1);
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+// u32 -> (0,1]: 2^-32 + v * 2^-32 in float.
+__device__ __forceinline__ float u01(uint32_t v)
+{
+    const float inv = 2.3283064365386963e-10f;
+    return inv + (float)v * inv;
+}
+
+// Multiplier of synapse e in storage WP (conversions to float are exact).
+template <int WP>
+__device__ __forceinline__ float wload(const void *__restrict__ val, int64_t e)
+{
+    if constexpr (WP == SC_W_F32) {
+        return static_cast<const float *>(val)[e];
+    } else if constexpr (WP == SC_W_BF16) {
+        return __uint_as_float((uint32_t) static_cast<const uint16_t *>(val)[e] << 16);
+    } else if constexpr (WP == SC_W_FP8) {
+        // OCP e4m3: (1 + m/8) * 2^(e-7), subnormal m * 2^-9
+        const uint32_t c = static_cast<const uint8_t *>(val)[e];
+        const uint32_t ex = (c >> 3) & 15u, mt = c & 7u;
+        const float v = ex ? __uint_as_float(((ex + 120u) << 23) | (mt << 20)) : (float)mt * 0x1p-9f;
+        return (c & 0x80u) ? -v : v;
+    } else {
+        return 1.f;
+    }
+}
+
+// Storage known only at run time (lane mode, where projections may differ).
+__device__ __forceinline__ float wload_rt(int wp, const void *__restrict__ val, int64_t e)
+{
+    switch (wp) {
+    case SC_W_F32: return wload<SC_W_F32>(val, e);
+    case SC_W_BF16: return wload<SC_W_BF16>(val, e);
+    case SC_W_FP8: return wload<SC_W_FP8>(val, e);
+    default: return 1.f;
+    }
+}
+
+// Ring row of the step `back` steps before the step whose row is `row` (back < R).
+__device__ __forceinline__ int row_back(int row, int back, int R)
+{
+    const int r = row - back;
+    return r < 0 ? r + R : r;
+}
+
+// One exchange step of the conductance LIF neuron, update_lif_ch
+// (Simulation_header.cu:315-377), every expression promoted as the reference
+// writes it; the library is compiled with -ffp-contract=off so no product is
+// fused into a following sum.  The sub-step loop is a serial dependence chain, so
+// it is kept rolled.
+__device__ __forceinline__ int lif_step(const ScPop &P, float &u, float &g0, float &g1, int32_t &r, float dge,
+                                        float dgi)
+{
+    const int cn = P.c_num;
+    const float inj0 = P.gw[0] * (P.exc_ch[0] ? dge : dgi);
+    const float inj1 = P.gw[1] * (P.exc_ch[1] ? dge : dgi);
+    const double dtc = P.dtc, dtt0 = P.dtt[0], dtt1 = P.dtt[1];
+    const float E0 = P.E[0], E1 = P.E[1], El = P.El, gL = P.gL, Ie = P.Ie, Vth = P.Vth, Vr = P.Vr;
+    const int ref_step = P.ref_step;
+    int spk = 0;
+#pragma unroll 1
+    for (int k = 0; k < kStepMax; ++k) {
+        float Isyn = 0.f;
+        if (cn > 0) Isyn = Isyn + g0 * (u - E0);
+        if (cn > 1) Isyn = Isyn + g1 * (u - E1);
+        const float leak = gL * (u - El);
+        const float inner = (Ie - leak) - Isyn;
+        const float du = (float)(dtc * (double)inner);
+        if (cn > 0) {
+            const double dec = (-dtt0) * (double)g0;
+            g0 = (float)((double)g0 + (k == 0 ? dec + (double)inj0 : dec));
+        }
+        if (cn > 1) {
+            const double dec = (-dtt1) * (double)g1;
+            g1 = (float)((double)g1 + (k == 0 ? dec + (double)inj1 : dec));
+        }
+        const int fired = u > Vth;
+        spk |= fired;
+        r = fired ? ref_step : (r > 0 ? r - 1 : 0);
+        u = r ? Vr : u + du;
+    }
+    return spk;
+}
+
+// Entries k >= k0 of one sliced-ELL row (slices wider than the batched part);
+// rare, so a compact rolled loop.
+__device__ __forceinline__ double slice_tail(const ScSlice &sl, const uint32_t *__restrict__ row32, int k0, int lane)
+{
+    double S = 0.0;
+#pragma unroll 1
+    for (int k = k0; k < sl.width; ++k) {
+        const uint32_t c = sl.col[k * SC_WAVE + lane];
+        if ((row32[c >> 5] >> (c & 31)) & 1u) S += (double)wload_rt(sl.wprec, sl.val, k * SC_WAVE + lane);
+    }
+    return S;
+}
+
+// One 64-neuron group of a lane-mode population: the first SC_LANE_U entries of
+// every slice are loaded in one batch (column and multiplier loads, then all
+// spike words), the neuron state is loaded alongside, then the LIF update.
+#define SC_LANE_U 6
+__device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance &I, const ScPop &P, int row_n,
+                                           int lane)
+{
+    const uint32_t zb = (uint32_t)(I.Wr - 1) * SC_WAVE;
+    const int d = t.group * SC_WAVE + lane;
+    const bool valid = lane < t.nvalid;
+    float u = 0.f;
+    float2 gg = {0.f, 0.f};
+    int32_t rf = 0;
+    if (valid) {
+        u = I.V[d];
+        gg = reinterpret_cast<const float2 *>(I.g)[d];
+        rf = I.refr[d];
+    }
+    constexpr int U = SC_LANE_U;
+    uint32_t c[SC_MAX_SLICES][U], sw[SC_MAX_SLICES][U];
+    float w[SC_MAX_SLICES][U];
+    const uint32_t *row[SC_MAX_SLICES];
+#pragma unroll
+    for (int s = 0; s < SC_MAX_SLICES; ++s) {
+        const bool on = s < t.n_sl;
+        row[s] = reinterpret_cast<const uint32_t *>(I.ring + (int64_t)(on ? row_back(row_n, t.sl[s].delay, I.R) : 0) * I.Wr);
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const bool e = on && k < t.sl[s].width;
+            c[s][k] = e ? t.sl[s].col[k * SC_WAVE + lane] : zb;
+            w[s][k] = e ? wload_rt(t.sl[s].wprec, t.sl[s].val, k * SC_WAVE + lane) : 0.f;
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < SC_MAX_SLICES; ++s)
+#pragma unroll
+        for (int k = 0; k < U; ++k) sw[s][k] = row[s][c[s][k] >> 5];
+    float dge = 0.f, dgi = 0.f;
+#pragma unroll
+    for (int s = 0; s < SC_MAX_SLICES; ++s) {
+        if (s >= t.n_sl) break;
+        double S = 0.0;
+#pragma unroll
+        for (int k = 0; k < U; ++k)
+            if ((sw[s][k] >> (c[s][k] & 31)) & 1u) S += (double)w[s][k];
+        if (t.sl[s].width > U) S += slice_tail(t.sl[s], row[s], U, lane);
+        // dg[post] += S * w_bar (Simulation_header.cu:98), projections in index order
+        const float Sf = (float)(S * (double)t.sl[s].scale);
+        const float add = Sf * t.sl[s].W;
+        if (t.sl[s].exc) dge = dge + add;
+        else dgi = dgi + add;
+    }
+    int spk = 0;
+    if (valid) {
+        spk = lif_step(P, u, gg.x, gg.y, rf, dge, dgi);
+        I.V[d] = u;
+        reinterpret_cast<float2 *>(I.g)[d] = gg;
+        I.refr[d] = rf;
+    }
+    const unsigned long long word = __ballot(spk != 0);
+    if (lane == 0) I.ring[(int64_t)row_n * I.Wr + t.group] = word;
+}
+
+// Team gather over a merged row: thread `tid` of a team of TT threads handles
+// entries tid, tid+TT, ... in batches of U (all column and multiplier loads of a
+// batch, then all spike words).  A column word carries the projection index in its
+// top bits; rowoff[q] is the offset (in 32-bit words) of that projection's ring
+// row.  Spiking entries add their multiplier to the team's f64 LDS accumulator of
+// the projection: the sums are exact, so the order of the additions cannot change
+// them.
+template <int WP, int U>
+__device__ __forceinline__ void team_gather(const ScTeamTask &T, const uint32_t *__restrict__ ring32,
+                                            const uint32_t *rowoff, uint32_t zero_col, int tid, int TT, double *acc)
+{
+    const int len = T.len;
+    const uint32_t *__restrict__ col = T.col;
+    for (int b = tid; b < len; b += TT * U) {
+        uint32_t c[U], s[U];
+        float w[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = b + u * TT;
+            c[u] = i < len ? col[i] : zero_col;
+            w[u] = i < len ? wload<WP>(T.val, i) : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t bit = c[u] & SC_BIT_MASK;
+            s[u] = ring32[rowoff[c[u] >> SC_Q_SHIFT] + (bit >> 5)];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if ((s[u] >> (c[u] & 31)) & 1u) atomicAdd(&acc[c[u] >> SC_Q_SHIFT], (double)w[u]);
+    }
+}
+
+// Whether one stimulus entry fires in exchange step n (draw index m):
+// InputStimulation (UserInputFunctions.cu:94-118) with the rule of ScStim.  The
+// three Philox blocks of the step run interleaved, round by round.
+__device__ __noinline__ int stim_fires(const ScStim &S, int entry, int64_t n, int64_t m)
+{
+    const uint64_t blk = (uint64_t)(3 * m);
+    uint32_t a0 = (uint32_t)blk, a1 = (uint32_t)(blk >> 32), a2 = (uint32_t)entry, a3 = 0u;
+    uint32_t b0 = (uint32_t)(blk + 1), b1 = (uint32_t)((blk + 1) >> 32), b2 = a2, b3 = 0u;
+    uint32_t c0 = (uint32_t)(blk + 2), c1 = (uint32_t)((blk + 2) >> 32), c2 = a2, c3 = 0u;
+    uint32_t k0 = S.key0, k1 = S.key1;
+#pragma unroll 1
+    for (int r = 0; r < 10; ++r) {
+        philox_round(a0, a1, a2, a3, k0, k1);
+        philox_round(b0, b1, b2, b3, k0, k1);
+        philox_round(c0, c1, c2, c3, k0, k1);
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    const uint32_t word[10] = {a0, a1, a2, a3, b0, b1, b2, b3, c0, c1};
+    int spk = 0;
+    const double nf = (double)(float)n;
+#pragma unroll 1
+    for (int k = 0; k < kStepMax; ++k) {
+        const float t = (float)(nf + (double)(float)k * kDT);
+        const double td = (double)t;
+        const double rk = (double)u01(word[k]);
+        int f = 0;
+        if (S.poisson_on <= td && td < S.poisson_off && rk < S.p_thr) f = 1;
+        if (S.burst_on <= td && td < S.burst_off && rk < S.b_thr) f = 1;
+        if (S.periodic_hz != 0.f && S.periodic_on <= td &&
+            (S.periodic_incl ? td <= S.periodic_off : td < S.periodic_off)) {
+            // PeriodicFiring (UserInputFunctions.cu:8-10)
+            const float dd = t - S.periodic_start;
+            const int a = (int)(((double)dd - kDT) * (double)S.periodic_hz / kSec);
+            const float fp = dd * S.periodic_hz;
+            const int b = (int)((double)fp / kSec);
+            if (a != b) f = 1;
+        }
+        spk |= f;
+    }
+    return spk;
+}
+
+// ---------------------------------------------------------------------- kernels
+// SINGLE: one instance passed by value (kernel argument); otherwise the task's
+// instance is read from the instance table.
+template <bool SINGLE>
+__device__ __forceinline__ const ScInstance &inst_of(const ScInstance &I0, const ScInstance *__restrict__ insts,
+                                                     int i)
+{
+    if constexpr (SINGLE) return I0;
+    else return insts[i];
+}
+
+template <bool SINGLE>
+__device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *__restrict__ insts, int t,
+                                        int first, int count, const ScTeamTask *__restrict__ teams, int row_single,
+                                        int k)
+{
+    __shared__ double acc[SC_BLOCK / SC_WAVE][SC_MAX_IN];
+    __shared__ uint32_t rowoff[SC_BLOCK / SC_WAVE][SC_MAX_IN];
+    const int wave = threadIdx.x / SC_WAVE, lane = threadIdx.x & (SC_WAVE - 1);
+    const int team = wave / t, tw = wave - team * t;
+    const bool active = team < count;
+    const ScTeamTask *T = active ? &teams[first + team] : nullptr;
+    const ScInstance &I = inst_of<SINGLE>(I0, insts, active ? T->inst : 0);
+    int row_n = row_single;
+    if (!SINGLE && active) row_n = (I.ctl->row + k) % I.R;
+    float u = 0.f;
+    float2 gg = {0.f, 0.f};
+    int32_t rf = 0;
+    const int d = active ? T->group * SC_WAVE + T->bit : 0;
+    if (active && tw == 0) {
+        if (lane < SC_MAX_IN) {
+            acc[team][lane] = 0.0;
+            rowoff[team][lane] =
+                lane < T->n_in ? (uint32_t)row_back(row_n, T->in[lane].delay, I.R) * (uint32_t)(2 * I.Wr) : 0u;
+        }
+        if (lane == 0) {  // state of the team's neuron, loaded while the gather runs
+            u = I.V[d];
+            gg = reinterpret_cast<const float2 *>(I.g)[d];
+            rf = I.refr[d];
+        }
+    }
+    __syncthreads();
+    if (active) {
+        const uint32_t *ring32 = reinterpret_cast<const uint32_t *>(I.ring);
+        const uint32_t zc = (uint32_t)(I.Wr - 1) * SC_WAVE;  // projection 0, bit of the zero word
+        const int tid = tw * SC_WAVE + lane, TT = t * SC_WAVE;
+        switch (T->wprec) {
+        case SC_W_F32: team_gather<SC_W_F32, 12>(*T, ring32, rowoff[team], zc, tid, TT, acc[team]); break;
+        case SC_W_BF16: team_gather<SC_W_BF16, 12>(*T, ring32, rowoff[team], zc, tid, TT, acc[team]); break;
+        case SC_W_FP8: team_gather<SC_W_FP8, 12>(*T, ring32, rowoff[team], zc, tid, TT, acc[team]); break;
+        default: team_gather<SC_W_UNIT, 12>(*T, ring32, rowoff[team], zc, tid, TT, acc[team]); break;
+        }
+    }
+    __syncthreads();
+    if (active && tw == 0 && lane == 0) {
+        const ScPop &P = I.pops[T->pop];
+        float dge = 0.f, dgi = 0.f;
+        for (int j = 0; j < T->n_in; ++j) {
+            // dg[post] += S * w_bar (Simulation_header.cu:98), projections in index order
+            const float Sf = (float)(acc[team][j] * (double)T->in[j].scale);
+            const float add = Sf * T->in[j].W;
+            if (T->in[j].exc) dge = dge + add;
+            else dgi = dgi + add;
+        }
+        const int spk = lif_step(P, u, gg.x, gg.y, rf, dge, dgi);
+        I.V[d] = u;
+        reinterpret_cast<float2 *>(I.g)[d] = gg;
+        I.refr[d] = rf;
+        if (spk) atomicOr(&I.ring[(int64_t)row_n * I.Wr + T->group], 1ull << T->bit);
+        // the word of the next step starts empty (that row is not read during this step)
+        if (T->bit == 0) {
+            const int row1 = row_n + 1 == I.R ? 0 : row_n + 1;
+            I.ring[(int64_t)row1 * I.Wr + T->group] = 0ull;
+        }
+    }
+}
+
+// The block layout is a kernel argument: block b belongs to the first class c with
+// b < cls_end[c]; classes 0..4 are team neurons of width 16, 8, 4, 2, 1
+// wavefronts, class 5 lane groups, class 6 stimulus groups.
+template <bool SINGLE>
+__global__ void __launch_bounds__(SC_BLOCK) k_step(ScInstance I0, const ScInstance *__restrict__ insts,
+                                                   ScLayout L, const ScLaneTask *__restrict__ lanes,
+                                                   const ScStimTask *__restrict__ stt,
+                                                   const ScTeamTask *__restrict__ teams, int k)
+{
+    // the step counter load is issued first; it overlaps the descriptor loads
+    int row_single = 0;
+    if (SINGLE) {
+        row_single = I0.ctl->row + k;
+        if (row_single >= I0.R) row_single -= I0.R;
+    }
+    const int b = blockIdx.x;
+    int c = 0;
+#pragma unroll
+    for (int j = 0; j < SC_CLASSES - 1; ++j) c += (b >= L.cls_end[j]) ? 1 : 0;
+    const int bi = b - (c ? L.cls_end[c - 1] : 0);
+    const int wave = threadIdx.x / SC_WAVE, lane = threadIdx.x & (SC_WAVE - 1);
+    if (c < SC_CLASS_LANE) {
+        const int t = SC_MAX_TEAM >> c, per = 1 << c;
+        do_team<SINGLE>(I0, insts, t, L.cls_task0[c] + bi * per, min(per, L.cls_ntask[c] - bi * per), teams,
+                        row_single, k);
+        return;
+    }
+    const int idx = bi * (SC_BLOCK / SC_WAVE) + wave;
+    if (idx >= L.cls_ntask[c]) return;
+    if (c == SC_CLASS_LANE) {
+        const ScLaneTask &t = lanes[idx];
+        const ScInstance &I = inst_of<SINGLE>(I0, insts, t.inst);
+        const int row_n = SINGLE ? row_single : (I.ctl->row + k) % I.R;
+        lane_group(t, I, I.pops[t.pop], row_n, lane);
+    } else {
+        const ScStimTask t = stt[idx];
+        const ScInstance &I = inst_of<SINGLE>(I0, insts, t.inst);
+        const int row_n = SINGLE ? row_single : (I.ctl->row + k) % I.R;
+        const int d = t.group * SC_WAVE + lane;
+        int spk = 0;
+        if (lane < t.nvalid) {
+            const int s = I.stim_s[d];
+            if (s >= 0) spk = stim_fires(I.stims[s], I.stim_e[d], I.ctl->n + k, I.ctl->m + k);
+        }
+        const unsigned long long word = __ballot(spk != 0);
+        if (lane == 0) I.ring[(int64_t)row_n * I.Wr + t.group] = word;
+    }
+}
+
+// Per-population spike counts and synaptic events of K consecutive steps: one
+// block per (instance, step) popcounts the spike words of that step's ring row
+// and sums the out-degrees of the set bits; the results go to the count / event
+// rows and the running totals.
+template <bool SINGLE>
+__global__ void __launch_bounds__(256) k_stats(ScInstance I0, const ScInstance *__restrict__ insts, int K)
+{
+    __shared__ unsigned long long acc[SC_MAX_POPS + 1];
+    const int inst = blockIdx.x / K, kk = blockIdx.x - inst * K;
+    const ScInstance &I = inst_of<SINGLE>(I0, insts, inst);
+    if (threadIdx.x <= SC_MAX_POPS) acc[threadIdx.x] = 0ull;
+    __syncthreads();
+    const int row = (I.ctl->row + kk) % I.R;
+    const uint64_t *__restrict__ w = I.ring + (int64_t)row * I.Wr;
+    unsigned long long ev = 0ull;
+    for (int g = threadIdx.x; g < I.n_groups; g += blockDim.x) {
+        unsigned long long word = w[g];
+        if (!word) continue;
+        atomicAdd(&acc[I.groups[g].pop], (unsigned long long)__popcll(word));
+        while (word) {
+            const int bt = __ffsll((long long)word) - 1;
+            word &= word - 1;
+            ev += I.outdeg[g * SC_WAVE + bt];
+        }
+    }
+    if (ev) atomicAdd(&acc[SC_MAX_POPS], ev);
+    __syncthreads();
+    const int p = threadIdx.x;
+    if (p < I.n_pops) {
+        I.counts[(int64_t)row * I.n_pops + p] = (int32_t)acc[p];
+        if (acc[p]) atomicAdd(&I.totals[p], acc[p]);
+    }
+    if (p == 0) {
+        I.events[row] = acc[SC_MAX_POPS];
+        if (acc[SC_MAX_POPS]) atomicAdd(&I.totals[I.n_pops], acc[SC_MAX_POPS]);
+    }
+}
+
+__global__ void k_advance(const ScInstance *__restrict__ insts, int n_inst, int K)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_inst) {
+        ScCtl *c = insts[i].ctl;
+        c->n += K;
+        c->m += K;
+        c->row = (c->row + K) % insts[i].R;
+    }
+}
+
+__global__ void k_set_ctl(ScCtl *ctl, int64_t n, int64_t m, int row)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        ctl->n = n;
+        ctl->m = m;
+        ctl->row = row;
+    }
+}
+
+// InitParams (init_params.cu:343-352): u = Vr + (Vth - Vr) * U with U the word
+// `off` of the Philox stream (seed, subsequence = reference global id); refractory
+// counters and ring row 0 cleared.
+__global__ void k_init_params(const ScInstance *__restrict__ ip, uint32_t k0, uint32_t k1, uint64_t off)
+{
+    const ScInstance &I = *ip;
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d < I.Wr) I.ring[d] = 0ull;
+    if (d >= I.Ndev) return;
+    const ScGroup &G = I.groups[d / SC_WAVE];
+    const int lane = d & (SC_WAVE - 1);
+    if (lane >= G.nvalid) return;
+    const ScPop &P = I.pops[G.pop];
+    const uint32_t ref = (uint32_t)(P.ref_base + G.first_local + lane);
+    const uint64_t blk = off >> 2;
+    uint32_t c0 = (uint32_t)blk, c1 = (uint32_t)(blk >> 32), c2 = ref, c3 = 0u;
+    philox10(c0, c1, c2, c3, k0, k1);
+    const uint32_t w = (off & 3) == 0 ? c0 : (off & 3) == 1 ? c1 : (off & 3) == 2 ? c2 : c3;
+    const float prod = P.Vr_span * u01(w);
+    I.V[d] = P.Vr + prod;
+    I.refr[d] = 0;
+}
+
+// Fired-neuron list of one ring row: the spike words are the ballots of spike
+// detection; one block popcounts them, prefix-sums the counts and writes the
+// reference global ids in ascending order.
+__global__ void __launch_bounds__(1024) k_compact(const uint64_t *__restrict__ row, int n_groups,
+                                                  const ScGroup *__restrict__ groups,
+                                                  const ScPop *__restrict__ pops, int32_t *__restrict__ out,
+                                                  int cap, int32_t *__restrict__ count)
+{
+    __shared__ int scan[1024];
+    const int t = threadIdx.x;
+    const int per = (n_groups + 1023) / 1024;
+    const int g0 = min(n_groups, t * per), g1 = min(n_groups, g0 + per);
+    int c = 0;
+    for (int g = g0; g < g1; ++g) c += __popcll(row[g]);
+    scan[t] = c;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const int v = t >= o ? scan[t - o] : 0;
+        __syncthreads();
+        scan[t] += v;
+        __syncthreads();
+    }
+    int pos = scan[t] - c;
+    if (t == 1023) *count = scan[1023];
+    for (int g = g0; g < g1; ++g) {
+        unsigned long long w = row[g];
+        const ScGroup G = groups[g];
+        const int base = pops[G.pop].ref_base + G.first_local;
+        while (w) {
+            const int b = __ffsll((long long)w) - 1;
+            w &= w - 1;
+            if (pos < cap) out[pos] = base + b;
+            ++pos;
+        }
+    }
+}
+
+__global__ void k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+// --------------------------------------------------------------------- launchers
+void launch_step(hipStream_t s, const ScInstance *single, const ScInstance *insts, const DevWork &w, int k)
+{
+    if (w.n_blocks <= 0) return;
+    if (single)
+        k_step<true><<<w.n_blocks, SC_BLOCK, 0, s>>>(*single, insts, w.layout, w.lanes, w.stims, w.teams, k);
+    else
+        k_step<false><<<w.n_blocks, SC_BLOCK, 0, s>>>(ScInstance{}, insts, w.layout, w.lanes, w.stims, w.teams, k);
+}
+
+void launch_stats(hipStream_t s, const ScInstance *single, const ScInstance *insts, int n_inst, int K)
+{
+    if (K <= 0 || n_inst <= 0) return;
+    if (single) k_stats<true><<<K, 256, 0, s>>>(*single, insts, K);
+    else k_stats<false><<<K * n_inst, 256, 0, s>>>(ScInstance{}, insts, K);
+}
+
+void launch_advance(hipStream_t s, const ScInstance *insts, int n_inst, int K)
+{
+    k_advance<<<(n_inst + 63) / 64, 64, 0, s>>>(insts, n_inst, K);
+}
+
+void launch_set_ctl(hipStream_t s, ScCtl *ctl, int64_t n, int64_t m, int row)
+{
+    k_set_ctl<<<1, 64, 0, s>>>(ctl, n, m, row);
+}
+
+void launch_init_params(hipStream_t s, const ScInstance *di, int n_threads, uint64_t seed, uint64_t off)
+{
+    k_init_params<<<(n_threads + 255) / 256, 256, 0, s>>>(di, (uint32_t)seed, (uint32_t)(seed >> 32), off);
+}
+
+void launch_compact(hipStream_t s, const uint64_t *row, int n_groups, const ScGroup *groups, const ScPop *pops,
+                    int32_t *out, int cap, int32_t *count)
+{
+    k_compact<<<1, 1024, 0, s>>>(row, n_groups, groups, pops, out, cap, count);
+}
+
+void launch_copy(hipStream_t s, const void *src, void *dst, int64_t n16, int grid)
+{
+    k_copy<<<grid, 256, 0, s>>>(static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), n16);
+}
+
+}  // namespace sc

@@ -1,0 +1,36 @@
+// kernels.hpp -- launchers of the gfx950 kernels (kernels.hip) used by the engine.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "model.hpp"
+
+namespace sc {
+
+#define HIP_TRY(x)                                                                                   \
+    do {                                                                                             \
+        hipError_t e_ = (x);                                                                         \
+        if (e_ != hipSuccess)                                                                        \
+            return fail(SCAFFOLD_ERR_HIP, "%s: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
+    } while (0)
+
+// Device work of one step: blocks and the descriptor arrays they index.
+struct DevWork {
+    ScLayout layout{};
+    int n_blocks = 0;
+    const ScLaneTask *lanes = nullptr;
+    const ScStimTask *stims = nullptr;
+    const ScTeamTask *teams = nullptr;
+};
+
+// All launchers are asynchronous on stream s.  `single` != nullptr selects the
+// kernels that take one instance by value; otherwise instances come from `insts`.
+void launch_step(hipStream_t s, const ScInstance *single, const ScInstance *insts, const DevWork &w, int k);
+void launch_stats(hipStream_t s, const ScInstance *single, const ScInstance *insts, int n_inst, int K);
+void launch_advance(hipStream_t s, const ScInstance *insts, int n_inst, int K);
+void launch_set_ctl(hipStream_t s, ScCtl *ctl, int64_t n, int64_t m, int row);
+void launch_init_params(hipStream_t s, const ScInstance *di, int n_threads, uint64_t seed, uint64_t off);
+void launch_compact(hipStream_t s, const uint64_t *row, int n_groups, const ScGroup *groups, const ScPop *pops,
+                    int32_t *out, int cap, int32_t *count);
+void launch_copy(hipStream_t s, const void *src, void *dst, int64_t n16, int grid);
+
+}  // namespace sc

@@ -56,7 +56,7 @@ struct Options {
     int weight_precision = SCAFFOLD_WEIGHTS_F32;
     int index_bits = 32;
     int lane_max_fanin = 48;
-    int chunk = 1024;
+    int lane_synapses = 16;   // team mode: target synapses per lane of a neuron's team
     int graph = 1;
     int graph_steps = 25;
     int record_fired = 0;
@@ -74,6 +74,18 @@ struct HostProjArrays {
     std::vector<int32_t> sell_base;
 };
 
+// Team-mode population: one merged CSR row per post neuron holding its synapses of
+// every incoming projection, projection after projection in index order.
+struct TeamPop {
+    int pop = -1;
+    int team_waves = 1;
+    int wprec = SC_W_UNIT;
+    std::vector<int64_t> row_ptr;   // num + 1
+    std::vector<int32_t> seg_end;   // num x n_in: end of each projection's segment, relative to the row
+    std::vector<uint32_t> col;      // device bit indices
+    std::vector<uint8_t> val;       // encoded multipliers
+};
+
 struct Plan {
     int Ndev = 0, n_groups = 0, Wr = 0, R = 0;
     int total_ref = 0;
@@ -86,9 +98,9 @@ struct Plan {
     std::vector<uint32_t> outdeg;         // per device index
     std::vector<int32_t> dev_to_ref;      // -1 for padding
     std::vector<int32_t> ref_to_dev;
-    std::vector<ScTask> tasks;            // gather kernel work list
-    std::vector<int32_t> fin_groups;      // groups finalized by the second kernel
-    int64_t n_slots = 0;
+    std::vector<TeamPop> teams;
+    std::vector<int> team_of;             // per population: index into teams or -1
+    std::vector<ScTask> tasks;            // step kernel work list (planner form)
     int64_t nnz = 0;
     int64_t algo_bytes = 0;
     int max_delay = 0;
@@ -104,7 +116,6 @@ void engine_destroy(Engine *e);
 int engine_init_params(Engine *e, uint64_t seed, int64_t call_index);
 int engine_zero_g(Engine *e);
 int engine_set_ctl(Engine *e, int64_t n, int64_t m);
-int engine_clear_counts_row(Engine *e, int row);
 int engine_upload_stims(Engine *e, const std::vector<ScStim> &stims);
 int engine_run(Engine *e, int n_steps, int use_graph, int graph_steps);
 int engine_sync(Engine *e);
@@ -113,6 +124,7 @@ int engine_set_state(Engine *e, const float *V, const float *g, const int32_t *r
 int engine_get_ring_row(Engine *e, int row, uint64_t *words);
 int engine_get_counts(Engine *e, int row, int32_t *counts, unsigned long long *events);
 int engine_get_fired(Engine *e, int row, int32_t *ids, int32_t cap, int32_t *n);
+int engine_get_totals(Engine *e, unsigned long long *totals);
 int engine_time(Engine *e, int n_steps, int use_graph, int graph_steps,
                 double *ms_total, double *ms_kernel_mean);
 int64_t engine_device_bytes(Engine *e);

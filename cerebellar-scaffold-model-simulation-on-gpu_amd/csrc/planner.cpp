@@ -1,7 +1,7 @@
 // planner.cpp -- .dat loader and the host planner that turns a network description
-// into the device layout (device_types.hpp): device order, sliced-ELL / chunked-CSR
-// projection streams, weight storage, partial-sum slots, the wavefront work list,
-// stimulus ownership and out-degrees.
+// into the device layout (device_types.hpp): device order, sliced-ELL slices and
+// merged per-neuron CSR rows, weight storage, the work list, stimulus ownership and
+// out-degrees.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -86,12 +86,12 @@ static float e4m3_to_f32(uint8_t c)
 
 static uint8_t f32_to_e4m3(float x)  // nearest, ties to even code, saturating at 448
 {
-    static float table[127];
-    static bool init = false;
-    if (!init) {
-        for (int c = 0; c < 127; ++c) table[c] = e4m3_to_f32((uint8_t)c);
-        init = true;
-    }
+    struct Table {
+        float v[127];
+        Table() { for (int c = 0; c < 127; ++c) v[c] = e4m3_to_f32((uint8_t)c); }
+    };
+    static const Table tab;  // thread-safe one-time initialization
+    const float *table = tab.v;
     uint8_t sign = x < 0 ? 0x80 : 0;
     float a = std::fabs(x);
     if (a >= 448.f) return sign | 126;
@@ -179,7 +179,8 @@ int plan_build(scaffold_env *env, Plan &P)
     P.total_ref = ref;
     P.n_output = 0;
     for (int p : P.out_pops) P.n_output += P.pops[p].num;
-    if ((int64_t)dev >= (1ll << 31) - 64) return fail(SCAFFOLD_ERR_CAPACITY, "too many neurons");
+    if ((int64_t)dev + SC_WAVE > (int64_t)SC_BIT_MASK)
+        return fail(SCAFFOLD_ERR_CAPACITY, "more than 2^29 neurons in one instance");
     const uint32_t sentinel = (uint32_t)P.n_groups * SC_WAVE;  // a bit of the always-zero word
 
     P.dev_to_ref.assign(dev, -1);
@@ -210,166 +211,192 @@ int plan_build(scaffold_env *env, Plan &P)
     P.nnz = 0;
     P.max_delay = 0;
     int64_t weight_bytes = 0;
+    // Gather mode per post population:
+    //   lane mode: one lane per post neuron over sliced-ELL slices, for populations
+    //              whose mean total fan-in is at most lane_max_fanin (and at most
+    //              SC_MAX_SLICES incoming projections);
+    //   team mode: a team of 1..16 wavefronts per post neuron over one merged CSR
+    //              row holding the neuron's synapses of every incoming projection,
+    //              projection after projection in index order.
+    std::vector<char> team_pop(npop, 0);
+    std::vector<int64_t> fanin(npop, 0);
+    std::vector<char> weighted(npop, 0);
+    for (int pi = 0; pi < nproj; ++pi) {
+        const Projection &src = env->projs[pi];
+        if (!src.set) continue;
+        fanin[src.post] += (int64_t)src.col.size();
+        if (!src.val.empty()) weighted[src.post] = 1;
+        ScPop &post = P.pops[src.post];
+        if (post.n_in >= SC_MAX_IN)
+            return fail(SCAFFOLD_ERR_CAPACITY, "population %d receives more than %d projections", src.post, SC_MAX_IN);
+        post.in_proj[post.n_in++] = pi;
+    }
+    for (int p = 0; p < npop; ++p) {
+        const ScPop &d = P.pops[p];
+        if (d.kind == SCAFFOLD_INPUT || d.num == 0) continue;
+        const double mean = (double)fanin[p] / d.num;
+        team_pop[p] = (mean > (double)opt.lane_max_fanin || d.n_in > SC_MAX_SLICES) ? 1 : 0;
+    }
+    auto storage_for = [&](bool has_val) {
+        if (!has_val) return SC_W_UNIT;
+        return opt.weight_precision == SCAFFOLD_WEIGHTS_BF16 ? SC_W_BF16
+             : opt.weight_precision == SCAFFOLD_WEIGHTS_FP8 ? SC_W_FP8 : SC_W_F32;
+    };
+    auto wbytes_of = [](int wp) { return wp == SC_W_F32 ? 4 : wp == SC_W_BF16 ? 2 : wp == SC_W_FP8 ? 1 : 0; };
+    // encoded multipliers of one projection in storage `wp` (scale: FP8 power of two)
+    std::vector<std::vector<float>> eff(nproj);
     for (int pi = 0; pi < nproj; ++pi) {
         const Projection &src = env->projs[pi];
         ScProj &sp = P.projs[pi];
-        sp.slot = -1;
         sp.scale = 1.f;
         if (!src.set) { sp.mode = -1; continue; }
         const ScPop &pre = P.pops[src.pre];
-        ScPop &post = P.pops[src.post];
         sp.delay = src.delay();
         sp.exc = src.weight > 0 ? 1 : 0;                       // GPUScaffoldSimulation.cu:732
         sp.W = src.weight > 0 ? src.weight : src.weight * (-1.f); // :733
         sp.pre_pop = src.pre;
         sp.post_pop = src.post;
-        sp.chunk = opt.chunk;
         if (sp.delay > P.R) return fail(SCAFFOLD_ERR_INVALID_ARG, "delay beyond the spike ring");
         P.max_delay = std::max(P.max_delay, sp.delay);
-        if (post.n_in >= SC_MAX_IN)
-            return fail(SCAFFOLD_ERR_CAPACITY, "population %d receives more than %d projections", src.post, SC_MAX_IN);
-        post.in_proj[post.n_in++] = pi;
-        const int rows = post.num;
         const int64_t nnz = (int64_t)src.col.size();
         P.nnz += nnz;
-        // weight storage
-        std::vector<float> eff;  // effective multipliers (empty: unit)
-        if (src.val.empty()) {
-            sp.wprec = SC_W_UNIT;
-        } else if (opt.weight_precision == SCAFFOLD_WEIGHTS_BF16) {
-            sp.wprec = SC_W_BF16;
-            eff.resize(nnz);
-            for (int64_t e = 0; e < nnz; ++e) eff[e] = bf16_to_f32(f32_to_bf16(src.val[e]));
-        } else if (opt.weight_precision == SCAFFOLD_WEIGHTS_FP8) {
-            sp.wprec = SC_W_FP8;
-            float mx = 0.f;
-            for (float x : src.val) mx = std::max(mx, std::fabs(x));
-            sp.scale = mx > 0.f ? std::ldexp(1.f, (int)std::ceil(std::log2((double)mx / 448.0))) : 1.f;
-            eff.resize(nnz);
-            for (int64_t e = 0; e < nnz; ++e) eff[e] = e4m3_to_f32(f32_to_e4m3(src.val[e] / sp.scale)) * sp.scale;
-        } else {
-            sp.wprec = SC_W_F32;
-            eff = src.val;
+        sp.mode = team_pop[src.post] ? SC_MODE_ROW : SC_MODE_LANE;
+        sp.wprec = storage_for(sp.mode == SC_MODE_ROW ? weighted[src.post] : !src.val.empty());
+        std::vector<float> &ev = eff[pi];
+        if (sp.wprec != SC_W_UNIT) {
+            ev.resize(nnz);
+            for (int64_t e = 0; e < nnz; ++e) ev[e] = src.val.empty() ? 1.f : src.val[e];
+            if (sp.wprec == SC_W_FP8) {
+                float mx = 0.f;
+                for (float x : ev) mx = std::max(mx, std::fabs(x));
+                sp.scale = mx > 0.f ? std::ldexp(1.f, (int)std::ceil(std::log2((double)mx / 448.0))) : 1.f;
+                for (auto &x : ev) x = e4m3_to_f32(f32_to_e4m3(x / sp.scale)) * sp.scale;
+            } else if (sp.wprec == SC_W_BF16) {
+                for (auto &x : ev) x = bf16_to_f32(f32_to_bf16(x));
+            }
+            if (!row_sums_exact(ev, src.row_ptr))
+                return fail(SCAFFOLD_ERR_WEIGHT_RANGE,
+                            "projection %d: multipliers span too wide a range for exact f64 row sums", pi);
         }
-        if (!eff.empty() && !row_sums_exact(eff, src.row_ptr))
-            return fail(SCAFFOLD_ERR_WEIGHT_RANGE,
-                        "projection %d: multipliers span too wide a range for exact f64 row sums", pi);
-        const int wbytes = sp.wprec == SC_W_F32 ? 4 : sp.wprec == SC_W_BF16 ? 2 : sp.wprec == SC_W_FP8 ? 1 : 0;
-        weight_bytes += nnz * wbytes;
-        auto put_w = [&](std::vector<uint8_t> &out, int64_t pos, int64_t e) {
-            if (sp.wprec == SC_W_F32) memcpy(&out[pos * 4], &src.val[e], 4);
-            else if (sp.wprec == SC_W_BF16) { uint16_t h = f32_to_bf16(src.val[e]); memcpy(&out[pos * 2], &h, 2); }
-            else if (sp.wprec == SC_W_FP8) out[pos] = f32_to_e4m3(src.val[e] / sp.scale);
-        };
+        weight_bytes += nnz * wbytes_of(sp.wprec);
         for (int64_t e = 0; e < nnz; ++e) P.outdeg[pre.dev_base + src.col[e]]++;
-        const double mean_fanin = rows > 0 ? (double)nnz / rows : 0.0;
+    }
+    // write one encoded multiplier
+    auto put_w = [&](std::vector<uint8_t> &out, int64_t pos, const ScProj &sp, float x) {
+        if (sp.wprec == SC_W_F32) memcpy(&out[pos * 4], &x, 4);
+        else if (sp.wprec == SC_W_BF16) { uint16_t h = f32_to_bf16(x); memcpy(&out[pos * 2], &h, 2); }
+        else if (sp.wprec == SC_W_FP8) out[pos] = f32_to_e4m3(x / sp.scale);
+    };
+    // lane-mode slices
+    for (int pi = 0; pi < nproj; ++pi) {
+        const Projection &src = env->projs[pi];
+        const ScProj &sp = P.projs[pi];
+        if (!src.set || sp.mode != SC_MODE_LANE) continue;
+        const ScPop &pre = P.pops[src.pre];
+        const int rows = P.pops[src.post].num;
+        const int wb = wbytes_of(sp.wprec);
         HostProjArrays &A = P.parr[pi];
-        if (mean_fanin <= (double)opt.lane_max_fanin) {
-            sp.mode = SC_MODE_LANE;
-            const int ng = (rows + SC_WAVE - 1) / SC_WAVE;
-            A.sell_off.assign(ng, 0);
-            A.sell_w.assign(ng, 0);
-            int64_t total = 0;
-            for (int gi = 0; gi < ng; ++gi) {
-                int w = 0;
-                for (int l = 0; l < SC_WAVE; ++l) {
-                    int j = gi * SC_WAVE + l;
-                    if (j < rows) w = std::max(w, (int)(src.row_ptr[j + 1] - src.row_ptr[j]));
-                }
-                A.sell_off[gi] = total;
-                A.sell_w[gi] = w;
-                total += (int64_t)w * SC_WAVE;
+        const int ng = (rows + SC_WAVE - 1) / SC_WAVE;
+        A.sell_off.assign(ng, 0);
+        A.sell_w.assign(ng, 0);
+        int64_t total = 0;
+        for (int gi = 0; gi < ng; ++gi) {
+            int w = 0;
+            for (int l = 0; l < SC_WAVE; ++l) {
+                int j = gi * SC_WAVE + l;
+                if (j < rows) w = std::max(w, (int)(src.row_ptr[j + 1] - src.row_ptr[j]));
             }
-            A.col.assign(total, sentinel);
-            if (wbytes) A.val.assign(total * wbytes, 0);
-            for (int gi = 0; gi < ng; ++gi) {
-                for (int l = 0; l < SC_WAVE; ++l) {
-                    int j = gi * SC_WAVE + l;
-                    if (j >= rows) continue;
-                    int k = 0;
-                    for (int64_t e = src.row_ptr[j]; e < src.row_ptr[j + 1]; ++e, ++k) {
-                        int64_t pos = A.sell_off[gi] + (int64_t)k * SC_WAVE + l;
-                        A.col[pos] = (uint32_t)(pre.dev_base + src.col[e]);
-                        if (wbytes) put_w(A.val, pos, e);
-                    }
-                }
-            }
-        } else {
-            sp.mode = SC_MODE_ROW;
-            A.row_ptr = src.row_ptr;
-            A.col.resize(nnz);
-            if (wbytes) A.val.assign(nnz * wbytes, 0);
-            for (int64_t e = 0; e < nnz; ++e) {
-                A.col[e] = (uint32_t)(pre.dev_base + src.col[e]);
-                if (wbytes) put_w(A.val, e, e);
-            }
-            int maxch = 0;
-            for (int j = 0; j < rows; ++j) {
-                int64_t len = src.row_ptr[j + 1] - src.row_ptr[j];
-                maxch = std::max(maxch, (int)((len + sp.chunk - 1) / sp.chunk));
-            }
-            sp.maxch = maxch;
+            A.sell_off[gi] = total;
+            A.sell_w[gi] = w;
+            total += (int64_t)w * SC_WAVE;
         }
+        A.col.assign(total, sentinel);
+        if (wb) A.val.assign(total * wb, 0);
+        for (int gi = 0; gi < ng; ++gi)
+            for (int l = 0; l < SC_WAVE; ++l) {
+                int j = gi * SC_WAVE + l;
+                if (j >= rows) continue;
+                int k = 0;
+                for (int64_t e = src.row_ptr[j]; e < src.row_ptr[j + 1]; ++e, ++k) {
+                    int64_t pos = A.sell_off[gi] + (int64_t)k * SC_WAVE + l;
+                    A.col[pos] = (uint32_t)(pre.dev_base + src.col[e]);
+                    if (wb) put_w(A.val, pos, sp, eff[pi][e]);
+                }
+            }
+    }
+    // team-mode merged rows
+    P.teams.clear();
+    P.team_of.assign(npop, -1);
+    for (int p = 0; p < npop; ++p) {
+        if (!team_pop[p]) continue;
+        const ScPop &d = P.pops[p];
+        TeamPop T;
+        T.pop = p;
+        T.wprec = storage_for(weighted[p]);
+        const int wb = wbytes_of(T.wprec);
+        T.row_ptr.assign(d.num + 1, 0);
+        T.seg_end.assign((size_t)d.num * d.n_in, 0);
+        for (int j = 0; j < d.num; ++j) {
+            int64_t len = 0;
+            for (int q = 0; q < d.n_in; ++q) {
+                const Projection &src = env->projs[d.in_proj[q]];
+                len += src.row_ptr[j + 1] - src.row_ptr[j];
+                T.seg_end[(size_t)j * d.n_in + q] = (int32_t)len;
+            }
+            T.row_ptr[j + 1] = T.row_ptr[j] + len;
+        }
+        const int64_t tot = T.row_ptr[d.num];
+        T.col.resize(tot);
+        if (wb) T.val.assign(tot * wb, 0);
+        for (int j = 0; j < d.num; ++j) {
+            int64_t pos = T.row_ptr[j];
+            for (int q = 0; q < d.n_in; ++q) {
+                const int pi = d.in_proj[q];
+                const Projection &src = env->projs[pi];
+                const ScPop &pre = P.pops[src.pre];
+                for (int64_t e = src.row_ptr[j]; e < src.row_ptr[j + 1]; ++e, ++pos) {
+                    T.col[pos] = ((uint32_t)q << SC_Q_SHIFT) | (uint32_t)(pre.dev_base + src.col[e]);
+                    if (wb) put_w(T.val, pos, P.projs[pi], eff[pi][e]);
+                }
+            }
+        }
+        // team width: about lane_synapses synapses per lane for the mean row
+        const double per_lane = (double)tot / std::max(1, d.num) / SC_WAVE / std::max(1, opt.lane_synapses);
+        int t = 1;
+        while (t < SC_MAX_TEAM && t < per_lane) t <<= 1;
+        T.team_waves = t;
+        P.team_of[p] = (int)P.teams.size();
+        P.teams.push_back(std::move(T));
     }
     P.algo_bytes = P.nnz * 4 + weight_bytes + (int64_t)ref * 32 + ref / 8;
+    for (auto &g : P.groups) g.self_final = 1;
 
-    // groups whose population has any row-mode input are finalized by the second kernel
-    std::vector<int> mixed(npop, 0);
-    for (int p = 0; p < npop; ++p) {
-        const ScPop &d = P.pops[p];
-        for (int q = 0; q < d.n_in; ++q)
-            if (P.projs[d.in_proj[q]].mode == SC_MODE_ROW) mixed[p] = 1;
-    }
-    P.n_slots = 0;
-    for (int p = 0; p < npop; ++p) {
-        if (!mixed[p]) continue;
-        const ScPop &d = P.pops[p];
-        for (int q = 0; q < d.n_in; ++q) {
-            ScProj &sp = P.projs[d.in_proj[q]];
-            sp.slot = P.n_slots;
-            P.n_slots += (int64_t)d.num * (sp.mode == SC_MODE_LANE ? 1 : std::max(sp.maxch, 1));
-        }
-    }
-    for (auto &g : P.groups) g.self_final = mixed[g.pop] ? 0 : 1;
-
-    // work list: row chunks first (largest), then lane groups, then stimulus groups
+    // work list: team neurons (widest teams first), lane groups, stimulus groups
     P.tasks.clear();
-    P.fin_groups.clear();
-    for (int pi = 0; pi < nproj; ++pi) {
-        const ScProj &sp = P.projs[pi];
-        if (sp.mode != SC_MODE_ROW) continue;
-        const ScPop &post = P.pops[sp.post_pop];
-        const Projection &src = env->projs[pi];
-        for (int j = 0; j < post.num; ++j) {
-            int64_t len = src.row_ptr[j + 1] - src.row_ptr[j];
-            int nch = (int)((len + sp.chunk - 1) / sp.chunk);
-            for (int c = 0; c < nch; ++c) {
-                ScTask t{};
-                t.kind = SC_TASK_ROW;
-                t.group = post.group_base + j / SC_WAVE;
-                t.proj = pi;
-                t.post = j;
-                t.chunk_idx = c;
-                t.e0 = src.row_ptr[j] + (int64_t)c * sp.chunk;
-                t.len = (int)std::min<int64_t>(sp.chunk, len - (int64_t)c * sp.chunk);
-                P.tasks.push_back(t);
-            }
+    std::vector<int> order;
+    for (size_t ti = 0; ti < P.teams.size(); ++ti) order.push_back((int)ti);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return P.teams[a].team_waves > P.teams[b].team_waves; });
+    for (int ti : order) {
+        const TeamPop &T = P.teams[ti];
+        const ScPop &d = P.pops[T.pop];
+        for (int j = 0; j < d.num; ++j) {
+            ScTask t{};
+            t.kind = SC_TASK_ROW;
+            t.group = d.group_base + j / SC_WAVE;
+            t.proj = T.pop;
+            t.post = j;
+            P.tasks.push_back(t);
         }
     }
     for (int gid = 0; gid < P.n_groups; ++gid) {
         const ScGroup &g = P.groups[gid];
         const ScPop &d = P.pops[g.pop];
-        if (d.kind == SCAFFOLD_INPUT) continue;
-        bool has_lane = false;
-        for (int q = 0; q < d.n_in; ++q)
-            if (P.projs[d.in_proj[q]].mode == SC_MODE_LANE) has_lane = true;
-        if (g.self_final || has_lane) {
-            ScTask t{};
-            t.kind = SC_TASK_LANE;
-            t.group = gid;
-            P.tasks.push_back(t);
-        }
-        if (!g.self_final) P.fin_groups.push_back(gid);
+        if (d.kind == SCAFFOLD_INPUT || team_pop[g.pop]) continue;
+        ScTask t{};
+        t.kind = SC_TASK_LANE;
+        t.group = gid;
+        P.tasks.push_back(t);
     }
     for (int gid = 0; gid < P.n_groups; ++gid) {
         const ScGroup &g = P.groups[gid];

@@ -383,7 +383,10 @@ int synth_describe(scaffold_env *env, const scaffold_synth_config *cfg)
                      nearest_pick(cell[GLOM], g_glom, cell[post].x[j], cell[post].z[j], 66, 132, r, o); });
             break;
         case 15: E = build_rows(counts[post], ps, J, [&](int j, Rng &r, std::vector<int32_t> &o) {
-                     nearest_pick(cell[GOC], g_goc, cell[post].x[j], cell[post].z[j], 4, 8, r, o); });
+                     // 2 or 3 Golgi cells per granule cell, mean 2.3 as in the reference's
+                     // connectome (206k synapses onto 88k granule cells)
+                     const int k = r.uni() < 0.3 ? 3 : 2;
+                     nearest_pick(cell[GOC], g_goc, cell[post].x[j], cell[post].z[j], k, 8, r, o); });
             break;
         }
         float delay = rp.delay;

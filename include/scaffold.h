@@ -156,9 +156,10 @@ int scaffold_set_input_stim_ids(scaffold_env *env, int neuron_index,
  *                       init_params.cu:343-352); default 0x5eed
  *   "weight_precision"  SCAFFOLD_WEIGHTS_*; default F32
  *   "index_bits"        32 or 16 (16-bit pre offsets within 65,536-neuron tiles)
- *   "lane_max_fanin"    projections with mean fan-in <= this are gathered one
- *                       lane per post neuron; others one wavefront per row chunk
- *   "chunk"             synapses per row chunk (multiple of 64)
+ *   "lane_max_fanin"    populations whose mean total fan-in is <= this are
+ *                       gathered one lane per post neuron (sliced ELL); others by a
+ *                       team of wavefronts per post neuron (merged CSR row)
+ *   "lane_synapses"     team mode: target synapses per lane (sets the team width)
  *   "graph"             1: replay captured hipGraphs of the step kernels
  *   "graph_steps"       exchange steps per captured graph
  *   "record_fired"      1: keep a compacted fired-neuron list per step
@@ -292,6 +293,9 @@ int scaffold_get_spike_row(scaffold_env *env, int step, uint8_t *spikes);
 /* Per-population spike counts (n_populations) and synaptic events
  * (sum over fired neurons of their out-degree) of step `step`. */
 int scaffold_get_counts(scaffold_env *env, int step, int32_t *counts, uint64_t *events);
+/* Spike totals per population (n_populations) and the synaptic-event total,
+ * accumulated on the GPU since initialization. */
+int scaffold_get_totals(scaffold_env *env, uint64_t *spikes, uint64_t *events);
 /* Compacted fired-neuron list of step `step` (global ids, ascending), built on
  * the GPU with wavefront ballot/popcount and a prefix sum. */
 int scaffold_get_fired(scaffold_env *env, int step, int32_t *ids, int32_t cap, int32_t *n);

@@ -85,3 +85,164 @@ def test_config1_bit_exact(pkg, orc):
     run_and_compare(env, ref, 0, 150, check_counts=True)
     assert_same_state(env, ref, "config1 150 steps")
     env.finalize()
+
+
+@pytest.mark.parametrize("precision", [1, 2])
+def test_quantized_weights_bit_exact(pkg, orc, precision):
+    """BF16 / FP8-e4m3 multipliers: the oracle runs on the dequantized values the
+    engine reports (FP8 scale is a power of two, so they are exact floats)."""
+    env = new_env(pkg, 0)
+    env.set_option("weight_precision", precision)
+    env.initialize(50, 25)
+    _, _, val = env.projection_csr(0)
+    assert val.dtype == np.float32 and val.size > 0
+    if precision == 2:
+        assert env.projection(0).fp8_scale == 2.0 ** np.round(np.log2(env.projection(0).fp8_scale))
+    ref = orc.OracleSim.from_env(env)
+    ref.initialize()
+    run_and_compare(env, ref, 0, 200)
+    assert_same_state(env, ref, f"precision {precision}")
+    env.finalize()
+
+
+def test_unit_weights_and_reference_delays(pkg, orc):
+    env = new_env(pkg, 0, weight_jitter=0, reference_delays=1)
+    env.initialize(50, 25)
+    assert env.info().max_delay == 5
+    ref = orc.OracleSim.from_env(env)
+    ref.initialize()
+    run_and_compare(env, ref, 0, 200)
+    env.finalize()
+
+
+def test_reference_dat_drop_in(pkg, orc, tmp_path):
+    """The reference's description plug-in over .dat files (unit multipliers,
+    reference population sizes and delays): written by the generator, read back by
+    the .dat loader, simulated and compared with the oracle."""
+    src = new_env(pkg, 1, weight_jitter=0, reference_delays=1)
+    files = {0: "parallel_fiber_to_purkinje.dat", 1: "parallel_fiber_to_golgi.dat",
+             2: "parallel_fiber_to_stellate.dat", 3: "parallel_fiber_to_basket.dat",
+             4: "ascending_axon_to_golgi.dat", 5: "ascending_axon_to_purkinje.dat", 6: "golgi_to_golgi.dat",
+             7: "stellate_to_stellate.dat", 8: "basket_to_basket.dat", 9: "basket_to_purkinje.dat",
+             10: "stellate_to_purkinje.dat", 11: "glomerulus_to_granule.dat", 12: "glomerulus_to_golgi.dat",
+             13: "golgi_to_granule.dat", 14: "purkinje_to_dcn.dat", 15: "glomerulus_to_dcn.dat"}
+    for i, f in files.items():
+        src.write_dat(i, tmp_path / f)
+    rng = np.random.default_rng(5)
+    nglom = src.population(6).num
+    bg = np.arange(nglom)
+    tone = np.sort(rng.choice(nglom, 2906, replace=False))
+    np.savetxt(tmp_path / "background_noise.dat", bg, fmt="%d")
+    np.savetxt(tmp_path / "tone_stim.dat", tone, fmt="%d")
+    env = pkg.NetworkEnvironment(1, 7, 16, 2, 0)
+    env.describe_reference(str(tmp_path), stim_seed=11)
+    env.initialize(50, 25)
+    for i in range(16):
+        a = src.projection_csr(i)
+        b = env.projection_csr(i)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(np.sort(a[1]), np.sort(b[1]))
+    ref = orc.OracleSim.from_env(env)
+    ref.initialize()
+    run_and_compare(env, ref, 0, 400)  # includes the 140 Hz tone window at 300-351 ms
+    env.finalize()
+
+
+def test_trials_reset_and_freq(pkg, orc):
+    """Trial loop of main.cu: loop_n_steps in windows of T_cpu_sim, SetInputFreq,
+    ResetNetworkEnv (potentials redrawn, conductances and stimulus streams kept)."""
+    env = new_env(pkg, 0)
+    env.initialize(50, 25)
+    ref = orc.OracleSim.from_env(env)
+    ref.initialize()
+    for trial in range(2):
+        for t in range(0, 150, 25):
+            f = [4.0 + trial + t / 50.0, 6.0]
+            env.set_input_freq(f)
+            ref.set_freq(f)
+            env.loop_n_steps(t, 25, trial, want_output=False)
+            ref.run(t, 25)
+            for s in range(t, t + 25):
+                assert np.array_equal(env.spike_row(s), ref.spike_row(s)), f"trial {trial} step {s}"
+        assert_same_state(env, ref, f"end of trial {trial}")
+        env.reset(trial)
+        ref.reset()
+        assert_same_state(env, ref, f"after reset {trial}")
+    env.finalize()
+
+
+def test_output_train_fired_and_text(pkg, orc, tmp_path):
+    env = new_env(pkg, 0)
+    env.initialize(50, 25)
+    ref = orc.OracleSim.from_env(env)
+    ref.initialize()
+    info = env.info()
+    dcn = [i for i in range(info.n_populations) if env.population(i).device_type == pkg.OUTPUT]
+    base = env.population(dcn[0]).base_id
+    num = env.population(dcn[0]).num
+    for t in range(0, 100, 25):
+        out = env.loop_n_steps(t, 25, 0)
+        ref.run(t, 25)
+        want = np.stack([ref.spike_row(s)[base:base + num] for s in range(t, t + 25)])
+        assert np.array_equal(out, want)
+    for s in range(50, 100):
+        ids, n = env.fired(s)
+        want = np.nonzero(ref.spike_row(s))[0]
+        assert n == want.size and np.array_equal(ids, want), f"fired list at step {s}"
+    path = tmp_path / "spikes.txt"
+    env.write_spike_text(75, 25, path)
+    lines = open(path).read().split("\n")[:-1]
+    nz = sum(int(ref.spike_row(s).sum()) for s in range(75, 100))
+    assert len(lines) == nz
+    env.finalize()
+
+
+def test_batch_matches_single(pkg, orc):
+    """Uneven instances advanced by the batched step kernel reproduce their own
+    oracle runs exactly."""
+    envs = []
+    for i, (scale, rate) in enumerate([(0.1, 4.0), (0.05, 20.0), (0.15, 8.0)]):
+        e = new_env(pkg, 0, scale=scale, counts=(0,) * 8, seed=100 + i, stim_seed=7 + i, mf_rate_hz=rate)
+        e.set_option("neuron_seed", 300 + i)
+        e.set_option("weight_precision", pkg.WEIGHTS_BF16)
+        e.initialize(50, 25)
+        envs.append(e)
+    refs = []
+    for e in envs:
+        r = orc.OracleSim.from_env(e)
+        r.initialize()
+        refs.append(r)
+    b = pkg.Batch(envs)
+    for t in range(0, 150, 25):
+        b.run(25)
+        for e, r in zip(envs, refs):
+            r.run(t, 25)
+            for s in range(t, t + 25):
+                assert np.array_equal(e.spike_row(s), r.spike_row(s)), f"step {s}"
+    for e, r in zip(envs, refs):
+        assert_same_state(e, r, "batch")
+        c, ev = e.counts(149)
+    b.destroy()
+    for e in envs:
+        e.finalize()
+
+
+def test_long_run_chunking_and_totals(pkg, orc):
+    """Runs longer than the spike ring (statistics per run chunk) and the running
+    totals equal the oracle's spike counts."""
+    env = new_env(pkg, 0)
+    env.set_option("graph", 0)
+    env.initialize(5, 5)   # ring of 10 rows
+    ref = orc.OracleSim.from_env(env)
+    ref.initialize()
+    env.loop_n_steps(0, 300, 0, want_output=False)
+    tot = np.zeros(env.info().n_populations, np.int64)
+    for s in range(300):
+        ref.step(s)
+        row = ref.spike_row(s)
+        for p in range(len(tot)):
+            b, n = env.population(p).base_id, env.population(p).num
+            tot[p] += int(row[b:b + n].sum())
+    sp, ev = env.totals()
+    assert np.array_equal(sp.astype(np.int64), tot)
+    assert_same_state(env, ref, "300 steps")
+    env.finalize()
