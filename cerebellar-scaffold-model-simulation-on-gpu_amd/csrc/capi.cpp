@@ -380,7 +380,8 @@ int scaffold_set_option(scaffold_env *env, const char *key, int64_t value)
         o.weight_precision = (int)value;
     } else if (!strcmp(key, "index_bits")) {
         if (frozen) return frozen_err();
-        if (value != 32 && value != 16) return fail(SCAFFOLD_ERR_INVALID_ARG, "index_bits is 32 or 16");
+        if (value == 16) return fail(SCAFFOLD_ERR_UNSUPPORTED, "16-bit column indices are not implemented");
+        if (value != 32) return fail(SCAFFOLD_ERR_INVALID_ARG, "index_bits is 32");
         o.index_bits = (int)value;
     } else if (!strcmp(key, "lane_max_fanin")) {
         if (frozen) return frozen_err();
@@ -394,8 +395,7 @@ int scaffold_set_option(scaffold_env *env, const char *key, int64_t value)
     else if (!strcmp(key, "graph_steps")) {
         if (value < 1 || value > 100000) return fail(SCAFFOLD_ERR_INVALID_ARG, "graph_steps in [1,100000]");
         o.graph_steps = (int)value;
-    } else if (!strcmp(key, "record_fired")) o.record_fired = value ? 1 : 0;
-    else if (!strcmp(key, "reset_g")) o.reset_g = value ? 1 : 0;
+    } else if (!strcmp(key, "reset_g")) o.reset_g = value ? 1 : 0;
     else return fail(SCAFFOLD_ERR_INVALID_ARG, "unknown option %s", key);
     return SCAFFOLD_OK;
 }
@@ -414,7 +414,6 @@ int scaffold_get_option(scaffold_env *env, const char *key, int64_t *value)
     else if (!strcmp(key, "lane_synapses")) *value = o.lane_synapses;
     else if (!strcmp(key, "graph")) *value = o.graph;
     else if (!strcmp(key, "graph_steps")) *value = o.graph_steps;
-    else if (!strcmp(key, "record_fired")) *value = o.record_fired;
     else if (!strcmp(key, "reset_g")) *value = o.reset_g;
     else return fail(SCAFFOLD_ERR_INVALID_ARG, "unknown option %s", key);
     return SCAFFOLD_OK;

@@ -59,7 +59,6 @@ struct Options {
     int lane_synapses = 16;   // team mode: target synapses per lane of a neuron's team
     int graph = 1;
     int graph_steps = 25;
-    int record_fired = 0;
     int reset_g = 0;
 };
 

@@ -155,14 +155,14 @@ int scaffold_set_input_stim_ids(scaffold_env *env, int neuron_index,
  *   "neuron_seed"       Philox key of the initial-potential draw (InitParams,
  *                       init_params.cu:343-352); default 0x5eed
  *   "weight_precision"  SCAFFOLD_WEIGHTS_*; default F32
- *   "index_bits"        32 or 16 (16-bit pre offsets within 65,536-neuron tiles)
+ *   "index_bits"        32 (the only width implemented; 16 returns
+ *                       SCAFFOLD_ERR_UNSUPPORTED)
  *   "lane_max_fanin"    populations whose mean total fan-in is <= this are
  *                       gathered one lane per post neuron (sliced ELL); others by a
  *                       team of wavefronts per post neuron (merged CSR row)
  *   "lane_synapses"     team mode: target synapses per lane (sets the team width)
  *   "graph"             1: replay captured hipGraphs of the step kernels
  *   "graph_steps"       exchange steps per captured graph
- *   "record_fired"      1: keep a compacted fired-neuron list per step
  *   "reset_g"           1: also zero conductances in scaffold_reset_network_env
  *                       (the reference keeps them, GPUScaffoldSimulation.cu:802-854) */
 int scaffold_set_option(scaffold_env *env, const char *key, int64_t value);

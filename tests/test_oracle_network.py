@@ -1,0 +1,351 @@
+"""The oracle's stimulus, propagation, delay and reset rules on tiny networks.
+
+Each check recomputes a rule from the reference's description with a plain
+Python brute force built on functions pinned elsewhere (Philox against the
+golden vectors, the neuron update against closed forms), so a dropped term, a
+wrong row or delay index, or a wrong draw offset shows up as a mismatch.
+
+Reference rules followed (SimulationModule/Scaffold/...):
+  InputStimulation      UserInputFunctions.cu:94-118 (3 uniform4 blocks per entry
+                        and step, r[0..9], t_k = n + k*DT, later stimuli overwrite)
+  PoissonProcess        UserInputFunctions.cu:4-6   (r < freq/1000*DT)
+  PeriodicFiring        UserInputFunctions.cu:8-10
+  tone_stim             UserInputFunctions.cu:47-56 (300 <= t < 351, 140 Hz from 292.8)
+  propagation           Simulation_header.cu:85-99, GPUScaffoldSimulation.cu:726-744
+                        (row (n - delay) mod R, dg += S * |w_bar|)
+  InitParams            init_params.cu:343-352 (u = Vr + (Vth - Vr) U; ring row 0 cleared)
+"""
+import math
+
+import numpy as np
+import pytest
+
+INF = float("inf")
+GRANULE = (3.0, -74.0, 1.5, 0.0, -84.0, -42.0, 1.5, (1., 0., 1.0, 0.5, 1., -85., 1.0, 10.0))
+PURKINJE = (620.0, -62.0, 0.8, 600.0, -72.0, -47.0, 7.0, (1., 0., 1.0, 0.5, 1., -85., 1.0, 1.6))
+
+
+def draws(orc, seed, entry, m):
+    """r[0..9] of one stimulus entry at draw index m: blocks 3m, 3m+1, 3m+2 of
+    subsequence `entry`, words (x, y, z, w, x, y, z, w, x, y)."""
+    words = []
+    for b in range(3):
+        words += orc.philox_block([(3 * m + b) & 0xFFFFFFFF, (3 * m + b) >> 32, entry, 0],
+                                  [seed & 0xFFFFFFFF, seed >> 32])
+    return [orc.uniform(w) for w in words[:10]]
+
+
+# ------------------------------------------------------------------ stimulus
+
+def test_poisson_threshold_value(orc):
+    # 1 Hz -> 1e-4 per sub-step, 1 kHz -> 0.1, compared in double
+    st = orc.make_stim(0, [0], seed=5, poisson_hz=1000.0)
+    hits = 0
+    for m in range(400):
+        r = draws(orc, 5, 0, m)
+        want = any(x < 0.1 for x in r)
+        assert orc.lib().orc_stim_entry_fires(st, 1000.0, 0, 7, m) == want
+        hits += want
+    # P(fire) = 1 - 0.9^10 = 0.651
+    assert 0.58 < hits / 400 < 0.72
+
+
+def test_draw_pattern_uses_ten_of_twelve_words(orc):
+    """Words z, w of the third block are drawn but never used: an entry whose only
+    small draws sit there must stay silent; one at word 9 (third block y) fires."""
+    thr = 0.05                        # 500 Hz
+    st = orc.make_stim(0, [0], seed=11, poisson_hz=500.0)
+    seen_unused = seen_last = 0
+    for entry in range(40):
+        for m in range(60):
+            words = []
+            for b in range(3):
+                words += orc.philox_block([3 * m + b, 0, entry, 0], [11, 0])
+            u = [orc.uniform(w) for w in words]
+            small = [i for i, x in enumerate(u) if x < thr]
+            got = orc.lib().orc_stim_entry_fires(st, 500.0, entry, 3, m)
+            if small and all(i >= 10 for i in small):
+                assert got == 0
+                seen_unused += 1
+            elif small == [9]:
+                assert got == 1
+                seen_last += 1
+            else:
+                assert got == (1 if small else 0)
+    assert seen_unused > 0 and seen_last > 0
+
+
+def test_poisson_rate_statistics(orc):
+    # 40 Hz: p = 0.004 per sub-step, P(fire per step) = 1 - 0.996^10 = 0.0393
+    st = orc.make_stim(0, [0], seed=3, poisson_hz=40.0)
+    n = 0
+    N = 20000
+    for i in range(N):
+        n += orc.lib().orc_stim_entry_fires(st, 40.0, i % 200, 0, i // 200)
+    p = 1 - 0.996 ** 10
+    assert abs(n - N * p) < 5 * math.sqrt(N * p * (1 - p))
+
+
+def test_tone_periodic_spikes(orc):
+    """140 Hz from 292.8 ms: crossings at 292.8 + 1000k/140 ms, first sub-step at
+    or after each: (300, 0), (307, 1), (314, 3), (321, 4), (328, 6), (335, 7),
+    (342, 8), (350, 0) inside the 300 <= t < 351 window; Poisson part off."""
+    st = orc.make_stim(0, [0], periodic_hz=140.0, periodic_start=292.8, periodic_on=300.0, periodic_off=351.0)
+    fired = [n for n in range(280, 370) if orc.lib().orc_stim_entry_fires(st, 0.0, 0, n, n)]
+    assert fired == [300, 307, 314, 321, 328, 335, 342, 350]
+    # independently: first sub-step time t = n + k/10 with t - 292.8 >= 1000j/140
+    want = []
+    for j in range(1, 12):
+        c = 292.8 + 1000.0 * j / 140.0
+        t = math.ceil(c * 10 - 1e-9) / 10
+        if 300 <= t < 351:
+            want.append(int(t))
+    assert fired == want
+
+
+def test_window_edges(orc):
+    # Poisson window [on, off): always-fire rate, window [10, 12)
+    st = orc.make_stim(0, [0], seed=1, poisson_hz=1e6, poisson_on=10.0, poisson_off=12.0)
+    fired = [n for n in range(5, 16) if orc.lib().orc_stim_entry_fires(st, 1e6, 0, n, n)]
+    assert fired == [10, 11]          # step 9 ends at t = 9.9; step 12 starts at t = 12
+    # periodic window with an inclusive end (t <= off) keeps a crossing that lands on it
+    st = orc.make_stim(0, [0], periodic_hz=100.0, periodic_start=0.0, periodic_on=0.0, periodic_off=20.0)
+    excl = [n for n in range(0, 30) if orc.lib().orc_stim_entry_fires(st, 0.0, 0, n, n)]
+    st.periodic_off_inclusive = 1
+    incl = [n for n in range(0, 30) if orc.lib().orc_stim_entry_fires(st, 0.0, 0, n, n)]
+    assert excl == [10] and incl == [10, 20]
+
+
+def test_later_stimulus_overwrites(orc):
+    # stimulus 0 always fires on targets 0..9, stimulus 1 never fires on 5..14:
+    # the later one writes 0 over 5..9
+    pops = [orc.make_pop(16, 1)]
+    s0 = orc.make_stim(0, np.arange(10), seed=1, poisson_hz=1e6)
+    s1 = orc.make_stim(0, np.arange(5, 15), seed=2, poisson_hz=0.0)
+    sim = orc.OracleSim(pops, [], [s0, s1], 4, 1)
+    sim.initialize()
+    sim.step(0)
+    row = sim.spike_row(0)
+    assert list(row[:5]) == [1] * 5 and not row[5:].any()
+
+
+def test_fanout_entries_share_one_draw(orc):
+    # entry i drives targets[fp[i]:fp[i+1]] with entry i's own stream
+    fp = [0, 3, 5, 9]
+    tg = [0, 1, 2, 5, 6, 9, 10, 11, 12]
+    pops = [orc.make_pop(13, 1)]
+    st = orc.make_stim(0, tg, fp, seed=9, poisson_hz=400.0)
+    sim = orc.OracleSim(pops, [], [st], 64, 1)
+    sim.initialize()
+    fired = 0
+    for n in range(60):
+        sim.step(n)
+        row = sim.spike_row(n)
+        for e in range(3):
+            want = orc.lib().orc_stim_entry_fires(st, 400.0, e, n, n)
+            assert all(row[t] == want for t in tg[fp[e]:fp[e + 1]])
+            fired += want
+        assert row[[3, 4, 7, 8]].sum() == 0
+    assert fired > 10
+
+
+# --------------------------------------------------------- propagation, delay
+
+def make_csr(rng, n_post, n_pre, max_fan, weighted):
+    rp = [0]
+    col = []
+    for _ in range(n_post):
+        k = int(rng.integers(0, max_fan + 1))
+        col += list(rng.choice(n_pre, size=k, replace=False))
+        rp.append(len(col))
+    val = rng.uniform(0.8, 1.2, len(col)).astype(np.float32) if weighted else None
+    return np.array(rp, np.int64), np.array(col, np.int32), val
+
+
+def build_net(orc, R, weighted, seed=0, input_hz=300.0):
+    """INPUT pop 0 (40 entries, Poisson `input_hz`), Purkinje pop 1 (30), granule pop 2 (20).
+    Projections in order: 1->2 exc (delay 3), 0->2 exc (delay 1), 1->2 inh (delay 2)."""
+    import ctypes
+    rng = np.random.default_rng(seed)
+    pops = [orc.make_pop(40, 1), orc.make_pop(30, 0, *PURKINJE[:7], PURKINJE[7]),
+            orc.make_pop(20, 0, *GRANULE[:7], GRANULE[7])]
+    spec = [(1, 2, 0.37, 3, 30), (0, 2, 0.25, 1, 40), (1, 2, -0.5, 2, 30)]
+    projs, keep, csr = [], [], []
+    for pre, post, w, d, npre in spec:
+        rp, col, val = make_csr(rng, 20, npre, 12, weighted)
+        o = orc.OrcProj()
+        o.pre, o.post, o.w_bar, o.delay = pre, post, w, d
+        o.row_ptr = rp.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+        o.col = col.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        o.val = None if val is None else val.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        projs.append(o)
+        keep += [rp, col, val]
+        csr.append((pre, w, d, rp, col, val))
+    st = orc.make_stim(0, np.arange(40), seed=77, poisson_hz=input_hz)
+    sim = orc.OracleSim(pops, projs, [st], R, 12345, keep=keep)
+    return sim, csr, pops
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_propagation_and_delays_brute_force(orc, weighted):
+    """Recompute every post-synaptic input from the recorded spike rows: projection
+    q delivers at step n the spikes of step n - delay_q, S_j = sum of multipliers of
+    firing presynaptic entries (exact, then float), dg = float(S_j) * |w_bar|
+    accumulated in projection order; then the pinned neuron update must reproduce
+    the oracle's post state bit for bit.  R = 8 makes the ring wrap 5 times."""
+    R = 8
+    sim, csr, pops = build_net(orc, R, weighted)
+    sim.initialize()
+    base = sim.base
+    rows = {}
+    pre_spikes = 0
+    for n in range(40):
+        V0, g0, r0 = sim.state()
+        sim.step(n)
+        rows[n] = sim.spike_row(n).copy()
+        pre_spikes += int(rows[n][:70].sum())
+        V1, g1, r1 = sim.state()
+        for j in range(20):
+            dge = np.float32(0)
+            dgi = np.float32(0)
+            for pre, w, d, rp, col, val in csr:
+                src = rows.get(n - d)
+                S = 0.0
+                if src is not None:
+                    for e in range(rp[j], rp[j + 1]):
+                        if src[base[pre] + col[e]]:
+                            S += 1.0 if val is None else float(val[e])
+                add = np.float32(np.float32(S) * np.float32(abs(w)))
+                if w > 0:
+                    dge = np.float32(dge + add)
+                else:
+                    dgi = np.float32(dgi + add)
+            i = base[2] + j
+            s, u, g, r = orc.lif_step(pops[2], V0[i], [g0[2 * i], g0[2 * i + 1]], r0[i], float(dge), float(dgi))
+            assert np.float32(u) == V1[i] and r == r1[i], (n, j)
+            assert np.float32(g[0]) == g1[2 * i] and np.float32(g[1]) == g1[2 * i + 1], (n, j)
+            assert s == rows[n][i]
+    assert pre_spikes > 100
+
+
+def test_single_spike_arrives_after_delay(orc):
+    """One Purkinje spike at step 5 reaches the granule cells of the delay-3
+    excitatory projection at step 8 and those of the delay-2 inhibitory one at
+    step 7, nowhere else."""
+    R = 16
+    sim, csr, pops = build_net(orc, R, False, seed=4, input_hz=0.0)
+    sim.initialize()
+    N = sim.total
+    base = sim.base
+    V, g, r = sim.state()
+    hold = np.zeros(N, np.int32)
+    hold[base[1]:base[2]] = 1 << 20          # Purkinje cells held refractory (silent)
+    g[:] = 0
+    sim.set_state(V, g, hold)
+    seen = {}
+    for n in range(12):
+        V, g, r = sim.state()
+        if n == 5:
+            V[base[1] + 7] = 0.0                 # above threshold
+            r[base[1] + 7] = 0
+        else:
+            r[base[1]:base[2]] = 1 << 20
+        sim.set_state(V, None, r)
+        sim.step(n)
+        _, g1, _ = sim.state()
+        seen[n] = (g1[2 * base[2]:2 * base[2] + 40:2].copy(), g1[2 * base[2] + 1:2 * base[2] + 40:2].copy())
+    rp_e, col_e = csr[0][3], csr[0][4]
+    rp_i, col_i = csr[2][3], csr[2][4]
+    tgt_e = {j for j in range(20) if 7 in col_e[rp_e[j]:rp_e[j + 1]]}
+    tgt_i = {j for j in range(20) if 7 in col_i[rp_i[j]:rp_i[j + 1]]}
+    assert tgt_e and tgt_i
+    for n in range(12):
+        ge, gi = seen[n]
+        got_e = {j for j in range(20) if ge[j] > 0}
+        got_i = {j for j in range(20) if gi[j] > 0}
+        assert got_e == (tgt_e if n >= 8 else set()), n
+        assert got_i == (tgt_i if n >= 7 else set()), n
+
+
+def test_init_draw_offsets_and_reset(orc):
+    """InitParams: u_i = Vr + (Vth - Vr) * U with U the k-th output of stream
+    (seed, subsequence = global id, offset 0), k counting InitParams calls; the
+    initialization and the trial-0 reset make two calls, so trial 0 starts from
+    output #1.  Reset clears pending input, refractory counters and ring row 0,
+    and leaves conductances and the other ring rows alone."""
+    sim, csr, pops = build_net(orc, 8, True, seed=1)
+    seed = 12345
+    sim.initialize()
+    V, g, r = sim.state()
+    for gid in range(sim.total):
+        p = 0 if gid < 40 else (1 if gid < 70 else 2)
+        if p == 0:
+            continue
+        Vr, Vth = np.float32(pops[p].Vr), np.float32(pops[p].Vth)
+        U = np.float32(orc.uniform(orc.philox_u32(seed, gid, 1)))
+        assert V[gid] == np.float32(Vr + np.float32((Vth - Vr) * U)), gid
+    for n in range(10):
+        sim.step(n)
+    V, g, r = sim.state()
+    row1 = sim.spike_row(1).copy()
+    assert g.any() and row1.any()
+    sim.reset()
+    V2, g2, r2 = sim.state()
+    assert (g2 == g).all() and not r2.any()
+    assert not sim.spike_row(0).any() and (sim.spike_row(1) == row1).all()
+    gid = 50
+    U = np.float32(orc.uniform(orc.philox_u32(seed, gid, 2)))
+    Vr, Vth = np.float32(pops[1].Vr), np.float32(pops[1].Vth)
+    assert V2[gid] == np.float32(Vr + np.float32((Vth - Vr) * U))
+
+
+def test_stimulus_draws_continue_across_trials(orc):
+    # draw index m counts steps since initialization, not trial steps
+    pops = [orc.make_pop(4, 1)]
+    st = orc.make_stim(0, np.arange(4), seed=21, poisson_hz=2000.0)
+    sim = orc.OracleSim(pops, [], [st], 64, 1)
+    sim.initialize()
+    for n in range(7):
+        sim.step(n)
+    sim.reset()
+    assert sim.steps_since_init() == 7
+    sim.step(0)
+    row = sim.spike_row(0)
+    want = [orc.lib().orc_stim_entry_fires(st, 2000.0, e, 0, 7) for e in range(4)]
+    assert list(row) == want
+
+
+def test_sum_modes_agree_for_unit_multipliers(orc):
+    # with unit multipliers every partial sum is a small integer: the literal f32
+    # loop and the exact sum give the same trajectories
+    outs = []
+    for mode in (0, 1):
+        sim, csr, pops = build_net(orc, 8, False, seed=3)
+        sim = type(sim)(pops, sim._projs[:3], [orc.make_stim(0, np.arange(40), seed=77, poisson_hz=300.0)], 8,
+                        12345, 0, mode, keep=sim._keep)
+        sim.initialize()
+        rows = []
+        for n in range(60):
+            sim.step(n)
+            rows.append(sim.spike_row(n).copy())
+        outs.append((np.array(rows), sim.state()))
+    assert (outs[0][0] == outs[1][0]).all()
+    for a, b in zip(outs[0][1], outs[1][1]):
+        assert (a == b).all()
+
+
+def test_f64_mode_tracks_f32(orc):
+    # FP64 sums and state follow the same dynamics: over 60 steps the membrane
+    # potentials of the two precisions stay within a small band unless a spike
+    # time moved by a step
+    sim32, _, pops = build_net(orc, 8, True, seed=5)
+    sim64 = type(sim32)(pops, sim32._projs[:3], [orc.make_stim(0, np.arange(40), seed=77, poisson_hz=300.0)], 8,
+                        12345, 1, 0, keep=sim32._keep)
+    sim32.initialize()
+    sim64.initialize()
+    agree = 0
+    for n in range(60):
+        sim32.step(n)
+        sim64.step(n)
+        agree += int((sim32.spike_row(n) == sim64.spike_row(n)).all())
+    assert agree >= 55
