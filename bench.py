@@ -17,6 +17,7 @@ advanced together).
 import argparse
 import json
 import os
+import re
 import sys
 import threading
 import time
@@ -152,7 +153,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
-              f"(independent instances shard whole per GPU, see DESIGN.md)", file=sys.stderr)
+              f"(independent instances shard whole per GPU; run one process per GPU)", file=sys.stderr)
         sys.exit(2)
     import __graft_entry__ as ge
     pkg = ge.load_package()
@@ -171,7 +172,6 @@ def main():
         f"{info.n_tasks} wavefront tasks, {time.perf_counter() - t0:.1f} s")
 
     advance(env, 0, args.warmup)
-    start = args.warmup
     # timed region: exactly `steps` exchange steps, synchronized before and after
     sp0, ev0 = env.totals()
     t0 = time.perf_counter()
@@ -185,11 +185,14 @@ def main():
     _, kmean = env.time_steps(args.kernel_steps, kernel_mean=True)
     algo = info.algorithmic_bytes_per_step
     achieved = algo / (kmean * 1e-3) / 1e9
+    # HBM-side bytes per k_step launch from the FETCH_SIZE / WRITE_SIZE counter
+    # passes, calibrated against a copy kernel of known size
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_scaffold96k_fp32.json")
+    tpath = os.path.join(ROOT, "profiles", "traffic_k_step_scaffold96k_fp32.md")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            m = re.search(r"^hbm_bytes_per_launch = (\d+)$", f.read(), re.M)
+        traffic = int(m.group(1)) if m else None
     rates = {}
     for i in range(info.n_populations):
         p = env.population(i)
