@@ -61,24 +61,44 @@ def outdegree(env):
     return deg
 
 
-def cpu_baseline(env, steps):
-    """The oracle (plain C restatement of the reference step, one thread) on the
-    first `steps` exchange steps of the same network."""
+def cpu_baseline(env, steps, threads):
+    """The oracle (plain C restatement of the reference step) on the first `steps`
+    exchange steps of the same network: once on one thread (the reference runs
+    its CPU path as one pthread per simulation, GPUScaffoldSimulation.cu:7-53),
+    once with OpenMP over `threads` host cores (propagation and neuron update
+    split by post neuron)."""
     import oracle
     deg = outdegree(env)
-    ref = oracle.OracleSim.from_env(env)
-    ref.initialize()
-    spent = 0.0
-    events = 0
-    for s in range(steps):
-        t0 = time.perf_counter()
-        ref.step(s)
-        spent += time.perf_counter() - t0
-        events += int(deg[ref.spike_row(s).astype(bool)].sum())
-    return {"value": events / spent, "unit": "synaptic events/s", "cores": 1, "kind": "port",
+
+    def timed(nthreads):
+        ref = oracle.OracleSim.from_env(env)
+        ref.set_threads(nthreads)
+        ref.initialize()
+        spent = 0.0
+        events = 0
+        for s in range(steps):
+            t0 = time.perf_counter()
+            ref.step(s)
+            spent += time.perf_counter() - t0
+            events += int(deg[ref.spike_row(s).astype(bool)].sum())
+        return events, spent
+
+    ev1, t1 = timed(1)
+    evn, tn = timed(threads)
+    assert ev1 == evn
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
+    return {"host_cpu": model, "host_cpus_visible": os.cpu_count(), "value": ev1 / t1, "unit": "synaptic events/s", "cores": 1, "kind": "port",
             "sample": f"first {steps} exchange steps ({steps} ms simulated) of the same 96.7k-cell network, "
-                      f"{spent:.1f} s of CPU time, single thread",
-            "real_time_factor": steps * 1e-3 / spent, "ms_per_step": spent * 1e3 / steps}
+                      f"{t1:.1f} s on one thread",
+            "real_time_factor": steps * 1e-3 / t1, "ms_per_step": t1 * 1e3 / steps,
+            "all_cores": {"value": evn / tn, "unit": "synaptic events/s", "cores": threads,
+                          "real_time_factor": steps * 1e-3 / tn, "ms_per_step": tn * 1e3 / steps,
+                          "sample": f"same {steps} steps, OpenMP over {threads} threads, {tn:.1f} s"}}
 
 
 def batched(pkg, n_inst, steps, warmup, threads):
@@ -142,7 +162,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--cpu-steps", type=int, default=400)
+    ap.add_argument("--cpu-steps", type=int, default=1000)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batch-instances", type=int, default=64)
@@ -217,7 +238,7 @@ def main():
     }
     env_for_cpu = env
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(env_for_cpu, args.cpu_steps)
+        line["cpu_baseline"] = cpu_baseline(env_for_cpu, args.cpu_steps, args.cpu_threads)
     env.finalize()
     if not args.no_batched:
         b = batched(pkg, args.batch_instances, args.batch_steps, 50, threads=min(16, os.cpu_count() or 4))

@@ -71,6 +71,8 @@ def lib():
         L.orc_step.restype = None
         L.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         L.orc_run.restype = None
+        L.orc_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_set_threads.restype = None
         L.orc_set_freq.argtypes = [ctypes.c_void_p, P(ctypes.c_float)]
         L.orc_set_freq.restype = None
         L.orc_get_state.argtypes = [ctypes.c_void_p, P(ctypes.c_float), P(ctypes.c_float), P(i32)]
@@ -234,6 +236,9 @@ class OracleSim:
 
     def run(self, n0, k):
         lib().orc_run(self._h, n0, k)
+
+    def set_threads(self, n):
+        lib().orc_set_threads(self._h, n)
 
     def set_freq(self, freqs):
         f = np.ascontiguousarray(freqs, dtype=np.float32)

@@ -103,7 +103,12 @@ struct orc_sim {
     double   *u64, *g64, *dge64, *dgi64;
     int32_t  *refr;
     uint8_t  *ring;            /* R x total bytes (GPUScaffoldSimulation.cu:361) */
+    int       threads;         /* OpenMP threads of orc_step (1: serial)  */
 };
+
+/* Threads used by orc_step for propagation and the neuron update (both are
+ * independent per post neuron, so the result does not depend on the count). */
+void orc_set_threads(orc_sim *s, int threads) { s->threads = threads > 0 ? threads : 1; }
 
 int orc_ref_step(float dt_ref)
 {
@@ -137,6 +142,7 @@ orc_sim *orc_create(int n_pops, const orc_pop *pops, int n_projs, const orc_proj
     s->neuron_seed = neuron_seed;
     s->precision = precision;
     s->sum_mode = sum_mode;
+    s->threads = 1;
     size_t N = (size_t)(b > 0 ? b : 1);
     s->refr = (int32_t *)calloc(N, sizeof(int32_t));
     s->ring = (uint8_t *)calloc(N * (size_t)ring_rows, 1); /* defined zero (reference: cudaMalloc) */
@@ -377,6 +383,7 @@ void orc_step(orc_sim *s, int n)
         const int post_num = s->pops[pr->post].num;
         const int exc = pr->w_bar > 0;
         const float W = exc ? pr->w_bar : pr->w_bar * (-1.f);
+#pragma omp parallel for schedule(static) num_threads(s->threads) if (s->threads > 1)
         for (int j = 0; j < post_num; ++j) {
             if (s->precision == 0) {
                 float Sf;
@@ -406,6 +413,7 @@ void orc_step(orc_sim *s, int n)
     for (int p = 0; p < s->n_pops; ++p) {
         const orc_pop *pp = &s->pops[p];
         if (pp->kind == ORC_INPUT) continue;
+#pragma omp parallel for schedule(static) num_threads(s->threads) if (s->threads > 1)
         for (int i = s->base[p]; i < s->base[p] + pp->num; ++i) {
             int spk;
             if (s->precision == 0) {

@@ -85,6 +85,7 @@ void orc_reset(orc_sim *s);
 void orc_step(orc_sim *s, int n);
 /* Steps n0 .. n0+k-1. */
 void orc_run(orc_sim *s, int n0, int k);
+void orc_set_threads(orc_sim *s, int threads);
 void orc_set_freq(orc_sim *s, const float *freq_list);
 
 /* State access (FP32 oracle: float arrays; FP64 oracle: the *_f64 calls). */

@@ -349,3 +349,20 @@ def test_f64_mode_tracks_f32(orc):
         sim64.step(n)
         agree += int((sim32.spike_row(n) == sim64.spike_row(n)).all())
     assert agree >= 55
+
+
+def test_threaded_oracle_matches_serial(orc):
+    # the OpenMP split is by post neuron, so any thread count gives the same result
+    outs = []
+    for threads in (1, 4):
+        sim, csr, pops = build_net(orc, 8, True, seed=6)
+        sim.set_threads(threads)
+        sim.initialize()
+        rows = []
+        for n in range(50):
+            sim.step(n)
+            rows.append(sim.spike_row(n).copy())
+        outs.append((np.array(rows), sim.state()))
+    assert (outs[0][0] == outs[1][0]).all()
+    for a, b in zip(outs[0][1], outs[1][1]):
+        assert (a.view(np.uint32) == b.view(np.uint32)).all()
