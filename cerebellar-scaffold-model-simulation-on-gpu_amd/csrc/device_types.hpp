@@ -129,7 +129,9 @@ struct ScSlice {           // one lane-mode projection of one 64-neuron group
 
 struct ScLaneTask {        // 64 neurons of a lane-mode population
     int32_t inst, group, pop, n_sl;
-    int32_t nvalid, pad0, pad1, pad2;
+    int32_t nvalid;
+    int32_t wp;            // storage of the slices that have multipliers (SC_W_UNIT: none)
+    int32_t pad1, pad2;
     ScSlice sl[SC_MAX_SLICES];
 };
 

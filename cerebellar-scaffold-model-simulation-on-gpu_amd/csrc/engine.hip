@@ -233,6 +233,7 @@ int engine_create(Engine **out, Plan &P)
             const ScPop &pp = P.pops[G.pop];
             const int gi = t.group - pp.group_base;
             ScLaneTask l{};
+            l.wp = SC_W_UNIT;
             l.group = t.group;
             l.pop = G.pop;
             l.nvalid = G.nvalid;
@@ -250,6 +251,7 @@ int engine_create(Engine **out, Plan &P)
                 s.exc = sp.exc;
                 s.W = sp.W;
                 s.scale = sp.scale;
+                if (s.val) l.wp = sp.wprec;
             }
             e->h_lanes.push_back(l);
         } else {

@@ -66,33 +66,56 @@ __device__ __forceinline__ float u01(uint32_t v)
     return inv + (float)v * inv;
 }
 
-// Multiplier of synapse e in storage WP (conversions to float are exact).
+// Pointers into device memory reached through descriptors are generic pointers
+// to the compiler; casting them to the global address space makes it emit
+// global loads (vector memory only) instead of flat loads, which would also hold
+// the LDS counter and serialise LDS waits behind memory loads.
+template <typename T>
+using gptr = const __attribute__((address_space(1))) T *;
+template <typename T>
+__device__ __forceinline__ gptr<T> gbl(const T *p)
+{
+    return (gptr<T>)p;
+}
+
+// Both conductances of device neuron d (interleaved exc, inh) in one 8-byte load.
+__device__ __forceinline__ float2 load_g(const float *g, int d)
+{
+    const uint64_t v = gbl(reinterpret_cast<const uint64_t *>(g))[d];
+    return make_float2(__uint_as_float((uint32_t)v), __uint_as_float((uint32_t)(v >> 32)));
+}
+
+// Wave-uniform value in a scalar register (descriptor indices derived from the
+// wavefront index), so descriptor fields are fetched with scalar loads and
+// branches on them are scalar branches.
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Raw bits of multiplier e in storage WP.  Loads and decoding are separate so a
+// batch of loads is issued back to back and decoded after they arrive.
 template <int WP>
-__device__ __forceinline__ float wload(const void *__restrict__ val, int64_t e)
+__device__ __forceinline__ uint32_t wraw(const void *val, int64_t e)
+{
+    if constexpr (WP == SC_W_F32) return gbl(static_cast<const uint32_t *>(val))[e];
+    else if constexpr (WP == SC_W_BF16) return gbl(static_cast<const uint16_t *>(val))[e];
+    else if constexpr (WP == SC_W_FP8) return gbl(static_cast<const uint8_t *>(val))[e];
+    else return 0u;
+}
+
+// Multiplier value of raw bits in storage WP (every conversion to float is exact).
+template <int WP>
+__device__ __forceinline__ float wdec(uint32_t c)
 {
     if constexpr (WP == SC_W_F32) {
-        return static_cast<const float *>(val)[e];
+        return __uint_as_float(c);
     } else if constexpr (WP == SC_W_BF16) {
-        return __uint_as_float((uint32_t) static_cast<const uint16_t *>(val)[e] << 16);
+        return __uint_as_float(c << 16);
     } else if constexpr (WP == SC_W_FP8) {
         // OCP e4m3: (1 + m/8) * 2^(e-7), subnormal m * 2^-9
-        const uint32_t c = static_cast<const uint8_t *>(val)[e];
         const uint32_t ex = (c >> 3) & 15u, mt = c & 7u;
         const float v = ex ? __uint_as_float(((ex + 120u) << 23) | (mt << 20)) : (float)mt * 0x1p-9f;
         return (c & 0x80u) ? -v : v;
     } else {
         return 1.f;
-    }
-}
-
-// Storage known only at run time (lane mode, where projections may differ).
-__device__ __forceinline__ float wload_rt(int wp, const void *__restrict__ val, int64_t e)
-{
-    switch (wp) {
-    case SC_W_F32: return wload<SC_W_F32>(val, e);
-    case SC_W_BF16: return wload<SC_W_BF16>(val, e);
-    case SC_W_FP8: return wload<SC_W_FP8>(val, e);
-    default: return 1.f;
     }
 }
 
@@ -144,13 +167,16 @@ __device__ __forceinline__ int lif_step(const ScPop &P, float &u, float &g0, flo
 
 // Entries k >= k0 of one sliced-ELL row (slices wider than the batched part);
 // rare, so a compact rolled loop.
-__device__ __forceinline__ double slice_tail(const ScSlice &sl, const uint32_t *__restrict__ row32, int k0, int lane)
+template <int WP>
+__device__ __forceinline__ double slice_tail(const ScSlice &sl, gptr<uint32_t> row32, int k0, int lane)
 {
     double S = 0.0;
+    gptr<uint32_t> col = gbl(sl.col);
 #pragma unroll 1
     for (int k = k0; k < sl.width; ++k) {
-        const uint32_t c = sl.col[k * SC_WAVE + lane];
-        if ((row32[c >> 5] >> (c & 31)) & 1u) S += (double)wload_rt(sl.wprec, sl.val, k * SC_WAVE + lane);
+        const uint32_t c = col[k * SC_WAVE + lane];
+        const uint32_t r = sl.val ? wraw<WP>(sl.val, k * SC_WAVE + lane) : 0u;
+        if ((row32[c >> 5] >> (c & 31)) & 1u) S += sl.val ? (double)wdec<WP>(r) : 1.0;
     }
     return S;
 }
@@ -158,7 +184,10 @@ __device__ __forceinline__ double slice_tail(const ScSlice &sl, const uint32_t *
 // One 64-neuron group of a lane-mode population: the first SC_LANE_U entries of
 // every slice are loaded in one batch (column and multiplier loads, then all
 // spike words), the neuron state is loaded alongside, then the LIF update.
+// Slice widths and pointers are wave-uniform; slices without multipliers (unit
+// weights) skip the multiplier loads.
 #define SC_LANE_U 6
+template <int WP>
 __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance &I, const ScPop &P, int row_n,
                                            int lane)
 {
@@ -169,23 +198,29 @@ __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance
     float2 gg = {0.f, 0.f};
     int32_t rf = 0;
     if (valid) {
-        u = I.V[d];
-        gg = reinterpret_cast<const float2 *>(I.g)[d];
-        rf = I.refr[d];
+        u = gbl(I.V)[d];
+        gg = load_g(I.g, d);
+        rf = gbl(I.refr)[d];
     }
     constexpr int U = SC_LANE_U;
-    uint32_t c[SC_MAX_SLICES][U], sw[SC_MAX_SLICES][U];
-    float w[SC_MAX_SLICES][U];
-    const uint32_t *row[SC_MAX_SLICES];
+    uint32_t c[SC_MAX_SLICES][U], raw[SC_MAX_SLICES][U], sw[SC_MAX_SLICES][U];
+    gptr<uint32_t> row[SC_MAX_SLICES];
+    const int n_sl = t.n_sl;
 #pragma unroll
     for (int s = 0; s < SC_MAX_SLICES; ++s) {
-        const bool on = s < t.n_sl;
-        row[s] = reinterpret_cast<const uint32_t *>(I.ring + (int64_t)(on ? row_back(row_n, t.sl[s].delay, I.R) : 0) * I.Wr);
+        const bool on = s < n_sl;
+        const int width = on ? t.sl[s].width : 0;
+        row[s] = gbl(reinterpret_cast<const uint32_t *>(I.ring + (int64_t)(on ? row_back(row_n, t.sl[s].delay, I.R) : 0) * I.Wr));
+        gptr<uint32_t> col = gbl(t.sl[s].col);
+        const void *val = t.sl[s].val;
 #pragma unroll
         for (int k = 0; k < U; ++k) {
-            const bool e = on && k < t.sl[s].width;
-            c[s][k] = e ? t.sl[s].col[k * SC_WAVE + lane] : zb;
-            w[s][k] = e ? wload_rt(t.sl[s].wprec, t.sl[s].val, k * SC_WAVE + lane) : 0.f;
+            c[s][k] = zb;
+            raw[s][k] = 0u;
+            if (k < width) {
+                c[s][k] = col[k * SC_WAVE + lane];
+                if (WP != SC_W_UNIT && val) raw[s][k] = wraw<WP>(val, k * SC_WAVE + lane);
+            }
         }
     }
 #pragma unroll
@@ -195,12 +230,13 @@ __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance
     float dge = 0.f, dgi = 0.f;
 #pragma unroll
     for (int s = 0; s < SC_MAX_SLICES; ++s) {
-        if (s >= t.n_sl) break;
+        if (s >= n_sl) break;
+        const bool unit = WP == SC_W_UNIT || !t.sl[s].val;
         double S = 0.0;
 #pragma unroll
         for (int k = 0; k < U; ++k)
-            if ((sw[s][k] >> (c[s][k] & 31)) & 1u) S += (double)w[s][k];
-        if (t.sl[s].width > U) S += slice_tail(t.sl[s], row[s], U, lane);
+            if ((sw[s][k] >> (c[s][k] & 31)) & 1u) S += unit ? 1.0 : (double)wdec<WP>(raw[s][k]);
+        if (t.sl[s].width > U) S += slice_tail<WP>(t.sl[s], row[s], U, lane);
         // dg[post] += S * w_bar (Simulation_header.cu:98), projections in index order
         const float Sf = (float)(S * (double)t.sl[s].scale);
         const float add = Sf * t.sl[s].W;
@@ -220,34 +256,35 @@ __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance
 
 // Team gather over a merged row: thread `tid` of a team of TT threads handles
 // entries tid, tid+TT, ... in batches of U (all column and multiplier loads of a
-// batch, then all spike words).  A column word carries the projection index in its
-// top bits; rowoff[q] is the offset (in 32-bit words) of that projection's ring
-// row.  Spiking entries add their multiplier to the team's f64 LDS accumulator of
-// the projection: the sums are exact, so the order of the additions cannot change
-// them.
+// batch, then all spike words).  Indices past the row end are clamped to its last
+// entry, so every load of a batch is unconditional and in bounds (the clamped
+// entries are masked out when summing).  A column word carries the projection
+// index in its top bits; rowoff[q] is the offset (in 32-bit words) of that
+// projection's ring row.  Spiking entries add their multiplier to the team's f64
+// LDS accumulator of the projection: the sums are exact, so the order of the
+// additions cannot change them.
 template <int WP, int U>
-__device__ __forceinline__ void team_gather(const ScTeamTask &T, const uint32_t *__restrict__ ring32,
-                                            const uint32_t *rowoff, uint32_t zero_col, int tid, int TT, double *acc)
+__device__ __forceinline__ void team_gather(const ScTeamTask &T, gptr<uint32_t> ring32, const uint32_t *rowoff,
+                                            int tid, int TT, double *acc)
 {
     const int len = T.len;
-    const uint32_t *__restrict__ col = T.col;
+    gptr<uint32_t> col = gbl(T.col);
+    const void *val = T.val;
     for (int b = tid; b < len; b += TT * U) {
-        uint32_t c[U], s[U];
-        float w[U];
+        uint32_t c[U], raw[U], s[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int i = b + u * TT;
-            c[u] = i < len ? col[i] : zero_col;
-            w[u] = i < len ? wload<WP>(T.val, i) : 0.f;
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const uint32_t bit = c[u] & SC_BIT_MASK;
-            s[u] = ring32[rowoff[c[u] >> SC_Q_SHIFT] + (bit >> 5)];
+            const int i = min(b + u * TT, len - 1);
+            c[u] = col[i];
+            raw[u] = wraw<WP>(val, i);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u)
-            if ((s[u] >> (c[u] & 31)) & 1u) atomicAdd(&acc[c[u] >> SC_Q_SHIFT], (double)w[u]);
+            s[u] = ring32[rowoff[c[u] >> SC_Q_SHIFT] + ((c[u] & SC_BIT_MASK) >> 5)];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (b + u * TT < len && ((s[u] >> (c[u] & 31)) & 1u))
+                atomicAdd(&acc[c[u] >> SC_Q_SHIFT], (double)wdec<WP>(raw[u]));
     }
 }
 
@@ -312,7 +349,7 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
 {
     __shared__ double acc[SC_BLOCK / SC_WAVE][SC_MAX_IN];
     __shared__ uint32_t rowoff[SC_BLOCK / SC_WAVE][SC_MAX_IN];
-    const int wave = threadIdx.x / SC_WAVE, lane = threadIdx.x & (SC_WAVE - 1);
+    const int wave = uniform(threadIdx.x / SC_WAVE), lane = threadIdx.x & (SC_WAVE - 1);
     const int team = wave / t, tw = wave - team * t;
     const bool active = team < count;
     const ScTeamTask *T = active ? &teams[first + team] : nullptr;
@@ -330,21 +367,20 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
                 lane < T->n_in ? (uint32_t)row_back(row_n, T->in[lane].delay, I.R) * (uint32_t)(2 * I.Wr) : 0u;
         }
         if (lane == 0) {  // state of the team's neuron, loaded while the gather runs
-            u = I.V[d];
-            gg = reinterpret_cast<const float2 *>(I.g)[d];
-            rf = I.refr[d];
+            u = gbl(I.V)[d];
+            gg = load_g(I.g, d);
+            rf = gbl(I.refr)[d];
         }
     }
     __syncthreads();
     if (active) {
-        const uint32_t *ring32 = reinterpret_cast<const uint32_t *>(I.ring);
-        const uint32_t zc = (uint32_t)(I.Wr - 1) * SC_WAVE;  // projection 0, bit of the zero word
+        gptr<uint32_t> ring32 = gbl(reinterpret_cast<const uint32_t *>(I.ring));
         const int tid = tw * SC_WAVE + lane, TT = t * SC_WAVE;
         switch (T->wprec) {
-        case SC_W_F32: team_gather<SC_W_F32, 12>(*T, ring32, rowoff[team], zc, tid, TT, acc[team]); break;
-        case SC_W_BF16: team_gather<SC_W_BF16, 12>(*T, ring32, rowoff[team], zc, tid, TT, acc[team]); break;
-        case SC_W_FP8: team_gather<SC_W_FP8, 12>(*T, ring32, rowoff[team], zc, tid, TT, acc[team]); break;
-        default: team_gather<SC_W_UNIT, 12>(*T, ring32, rowoff[team], zc, tid, TT, acc[team]); break;
+        case SC_W_F32: team_gather<SC_W_F32, 12>(*T, ring32, rowoff[team], tid, TT, acc[team]); break;
+        case SC_W_BF16: team_gather<SC_W_BF16, 12>(*T, ring32, rowoff[team], tid, TT, acc[team]); break;
+        case SC_W_FP8: team_gather<SC_W_FP8, 12>(*T, ring32, rowoff[team], tid, TT, acc[team]); break;
+        default: team_gather<SC_W_UNIT, 12>(*T, ring32, rowoff[team], tid, TT, acc[team]); break;
         }
     }
     __syncthreads();
@@ -391,20 +427,26 @@ __global__ void __launch_bounds__(SC_BLOCK) k_step(ScInstance I0, const ScInstan
 #pragma unroll
     for (int j = 0; j < SC_CLASSES - 1; ++j) c += (b >= L.cls_end[j]) ? 1 : 0;
     const int bi = b - (c ? L.cls_end[c - 1] : 0);
-    const int wave = threadIdx.x / SC_WAVE, lane = threadIdx.x & (SC_WAVE - 1);
+    const int wave = uniform(threadIdx.x / SC_WAVE), lane = threadIdx.x & (SC_WAVE - 1);
     if (c < SC_CLASS_LANE) {
         const int t = SC_MAX_TEAM >> c, per = 1 << c;
         do_team<SINGLE>(I0, insts, t, L.cls_task0[c] + bi * per, min(per, L.cls_ntask[c] - bi * per), teams,
                         row_single, k);
         return;
     }
-    const int idx = bi * (SC_BLOCK / SC_WAVE) + wave;
+    const int idx = uniform(bi * (SC_BLOCK / SC_WAVE) + wave);
     if (idx >= L.cls_ntask[c]) return;
     if (c == SC_CLASS_LANE) {
         const ScLaneTask &t = lanes[idx];
         const ScInstance &I = inst_of<SINGLE>(I0, insts, t.inst);
         const int row_n = SINGLE ? row_single : (I.ctl->row + k) % I.R;
-        lane_group(t, I, I.pops[t.pop], row_n, lane);
+        const ScPop &P = I.pops[t.pop];
+        switch (t.wp) {
+        case SC_W_F32: lane_group<SC_W_F32>(t, I, P, row_n, lane); break;
+        case SC_W_BF16: lane_group<SC_W_BF16>(t, I, P, row_n, lane); break;
+        case SC_W_FP8: lane_group<SC_W_FP8>(t, I, P, row_n, lane); break;
+        default: lane_group<SC_W_UNIT>(t, I, P, row_n, lane); break;
+        }
     } else {
         const ScStimTask t = stt[idx];
         const ScInstance &I = inst_of<SINGLE>(I0, insts, t.inst);
