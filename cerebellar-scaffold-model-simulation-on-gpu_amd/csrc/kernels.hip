@@ -165,18 +165,30 @@ __device__ __forceinline__ int lif_step(const ScPop &P, float &u, float &g0, flo
     return spk;
 }
 
-// Entries k >= k0 of one sliced-ELL row (slices wider than the batched part);
-// rare, so a compact rolled loop.
-template <int WP>
+// Entries k >= k0 of one sliced-ELL row (slices wider than the batched part), in
+// batches of U like the head of the row: entry indices past the slice width are
+// clamped to its last column and masked out of the sum.
+template <int WP, int U>
 __device__ __forceinline__ double slice_tail(const ScSlice &sl, gptr<uint32_t> row32, int k0, int lane)
 {
     double S = 0.0;
     gptr<uint32_t> col = gbl(sl.col);
+    const int width = sl.width;
+    const bool unit = WP == SC_W_UNIT || !sl.val;
 #pragma unroll 1
-    for (int k = k0; k < sl.width; ++k) {
-        const uint32_t c = col[k * SC_WAVE + lane];
-        const uint32_t r = sl.val ? wraw<WP>(sl.val, k * SC_WAVE + lane) : 0u;
-        if ((row32[c >> 5] >> (c & 31)) & 1u) S += sl.val ? (double)wdec<WP>(r) : 1.0;
+    for (int k = k0; k < width; k += U) {
+        uint32_t c[U], raw[U], w[U];
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+            const int e = min(k + j, width - 1) * SC_WAVE + lane;
+            c[j] = col[e];
+            raw[j] = unit ? 0u : wraw<WP>(sl.val, e);
+        }
+#pragma unroll
+        for (int j = 0; j < U; ++j) w[j] = row32[c[j] >> 5];
+#pragma unroll
+        for (int j = 0; j < U; ++j)
+            if (k + j < width && ((w[j] >> (c[j] & 31)) & 1u)) S += unit ? 1.0 : (double)wdec<WP>(raw[j]);
     }
     return S;
 }
@@ -186,7 +198,7 @@ __device__ __forceinline__ double slice_tail(const ScSlice &sl, gptr<uint32_t> r
 // spike words), the neuron state is loaded alongside, then the LIF update.
 // Slice widths and pointers are wave-uniform; slices without multipliers (unit
 // weights) skip the multiplier loads.
-#define SC_LANE_U 6
+#define SC_LANE_U 4
 template <int WP>
 __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance &I, const ScPop &P, int row_n,
                                            int lane)
@@ -236,7 +248,7 @@ __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance
 #pragma unroll
         for (int k = 0; k < U; ++k)
             if ((sw[s][k] >> (c[s][k] & 31)) & 1u) S += unit ? 1.0 : (double)wdec<WP>(raw[s][k]);
-        if (t.sl[s].width > U) S += slice_tail<WP>(t.sl[s], row[s], U, lane);
+        if (t.sl[s].width > U) S += slice_tail<WP, U>(t.sl[s], row[s], U, lane);
         // dg[post] += S * w_bar (Simulation_header.cu:98), projections in index order
         const float Sf = (float)(S * (double)t.sl[s].scale);
         const float add = Sf * t.sl[s].W;
@@ -377,10 +389,10 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
         gptr<uint32_t> ring32 = gbl(reinterpret_cast<const uint32_t *>(I.ring));
         const int tid = tw * SC_WAVE + lane, TT = t * SC_WAVE;
         switch (T->wprec) {
-        case SC_W_F32: team_gather<SC_W_F32, 12>(*T, ring32, rowoff[team], tid, TT, acc[team]); break;
-        case SC_W_BF16: team_gather<SC_W_BF16, 12>(*T, ring32, rowoff[team], tid, TT, acc[team]); break;
-        case SC_W_FP8: team_gather<SC_W_FP8, 12>(*T, ring32, rowoff[team], tid, TT, acc[team]); break;
-        default: team_gather<SC_W_UNIT, 12>(*T, ring32, rowoff[team], tid, TT, acc[team]); break;
+        case SC_W_F32: team_gather<SC_W_F32, 8>(*T, ring32, rowoff[team], tid, TT, acc[team]); break;
+        case SC_W_BF16: team_gather<SC_W_BF16, 8>(*T, ring32, rowoff[team], tid, TT, acc[team]); break;
+        case SC_W_FP8: team_gather<SC_W_FP8, 8>(*T, ring32, rowoff[team], tid, TT, acc[team]); break;
+        default: team_gather<SC_W_UNIT, 8>(*T, ring32, rowoff[team], tid, TT, acc[team]); break;
         }
     }
     __syncthreads();
@@ -411,10 +423,9 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
 // b < cls_end[c]; classes 0..4 are team neurons of width 16, 8, 4, 2, 1
 // wavefronts, class 5 lane groups, class 6 stimulus groups.
 template <bool SINGLE>
-__global__ void __launch_bounds__(SC_BLOCK) k_step(ScInstance I0, const ScInstance *__restrict__ insts,
-                                                   ScLayout L, const ScLaneTask *__restrict__ lanes,
-                                                   const ScStimTask *__restrict__ stt,
-                                                   const ScTeamTask *__restrict__ teams, int k)
+__device__ __forceinline__ void step_body(const ScInstance &I0, const ScInstance *__restrict__ insts, const ScLayout &L,
+                                          const ScLaneTask *__restrict__ lanes, const ScStimTask *__restrict__ stt,
+                                          const ScTeamTask *__restrict__ teams, int k)
 {
     // the step counter load is issued first; it overlaps the descriptor loads
     int row_single = 0;
@@ -460,6 +471,27 @@ __global__ void __launch_bounds__(SC_BLOCK) k_step(ScInstance I0, const ScInstan
         const unsigned long long word = __ballot(spk != 0);
         if (lane == 0) I.ring[(int64_t)row_n * I.Wr + t.group] = word;
     }
+}
+
+// One network: 64 VGPRs at most, so two 1024-thread blocks share a CU and one
+// block's serial phases (descriptor loads, barriers, the neuron update) overlap
+// the other's gathers.
+__global__ void __launch_bounds__(SC_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8)))
+k_step(ScInstance I0, ScLayout L, const ScLaneTask *__restrict__ lanes, const ScStimTask *__restrict__ stt,
+       const ScTeamTask *__restrict__ teams, int k)
+{
+    step_body<true>(I0, nullptr, L, lanes, stt, teams, k);
+}
+
+// Many instances: every task names its instance in the instance table.  Held to
+// 64 VGPRs as well (a few spill to scratch): the doubled occupancy outweighs
+// the spill traffic.
+__global__ void __launch_bounds__(SC_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) k_step_batch(const ScInstance *__restrict__ insts, ScLayout L,
+                                                         const ScLaneTask *__restrict__ lanes,
+                                                         const ScStimTask *__restrict__ stt,
+                                                         const ScTeamTask *__restrict__ teams, int k)
+{
+    step_body<false>(ScInstance{}, insts, L, lanes, stt, teams, k);
 }
 
 // Per-population spike counts and synaptic events of K consecutive steps: one
@@ -591,9 +623,9 @@ void launch_step(hipStream_t s, const ScInstance *single, const ScInstance *inst
 {
     if (w.n_blocks <= 0) return;
     if (single)
-        k_step<true><<<w.n_blocks, SC_BLOCK, 0, s>>>(*single, insts, w.layout, w.lanes, w.stims, w.teams, k);
+        k_step<<<w.n_blocks, SC_BLOCK, 0, s>>>(*single, w.layout, w.lanes, w.stims, w.teams, k);
     else
-        k_step<false><<<w.n_blocks, SC_BLOCK, 0, s>>>(ScInstance{}, insts, w.layout, w.lanes, w.stims, w.teams, k);
+        k_step_batch<<<w.n_blocks, SC_BLOCK, 0, s>>>(insts, w.layout, w.lanes, w.stims, w.teams, k);
 }
 
 void launch_stats(hipStream_t s, const ScInstance *single, const ScInstance *insts, int n_inst, int K)
