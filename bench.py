@@ -157,6 +157,34 @@ def batched(pkg, n_inst, steps, warmup, threads):
             "aggregate_real_time_factor": len(scales) * steps * 1e-3 / wall}
 
 
+def scaffold16x(pkg, steps):
+    """The scaffold tiled 16x in area (1.55M cells, 50.7M synapses) with FP32, BF16
+    and FP8-e4m3 multipliers: step time, real-time factor, synaptic events/s and
+    the step kernel's algorithmic bandwidth."""
+    out = {}
+    for name, prec in (("fp32", pkg.WEIGHTS_F32), ("bf16", pkg.WEIGHTS_BF16), ("fp8", pkg.WEIGHTS_FP8)):
+        env = make_env(pkg, 2)
+        env.set_option("weight_precision", prec)
+        env.initialize(50, 25)
+        info = env.info()
+        advance(env, 0, 50)
+        _, ev0 = env.totals()
+        t0 = time.perf_counter()
+        env.time_steps(steps)
+        wall = time.perf_counter() - t0
+        _, ev1 = env.totals()
+        _, kmean = env.time_steps(50, kernel_mean=True)
+        algo = info.algorithmic_bytes_per_step
+        out[name] = {"ms_per_step": wall * 1e3 / steps, "real_time_factor": steps * 1e-3 / wall,
+                     "synaptic_events_per_s": (ev1 - ev0) / wall, "kernel_mean_ms": kmean,
+                     "algorithmic_bytes_per_step": algo, "achieved_GBs": algo / (kmean * 1e-3) / 1e9,
+                     "frac": algo / (kmean * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        out["neurons"], out["synapses"] = info.total_neurons, info.total_synapses
+        env.finalize()
+    out["steps"] = steps
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -166,6 +194,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-16x", action="store_true")
+    ap.add_argument("--steps-16x", type=int, default=200)
     ap.add_argument("--batch-instances", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--kernel-steps", type=int, default=100)
@@ -245,6 +275,8 @@ def main():
         b["vs_plain"] = b["value"] / value
         b["vs_plain_synapses_scanned"] = b["synapses_scanned_per_s"] / line["synapses_scanned_per_s"]
         line["batched_bf16"] = b
+    if not args.no_16x:
+        line["scaffold16x"] = scaffold16x(pkg, args.steps_16x)
     print(json.dumps(line))
 
 
