@@ -90,7 +90,7 @@ static ScLayout make_layout(const std::vector<int> &team_t, int n_lanes, int n_s
         i = j;
     }
     L.cls_ntask[SC_CLASS_LANE] = n_lanes;
-    blocks += (n_lanes + wpb - 1) / wpb;
+    blocks += (n_lanes + SC_LANE_PER_BLOCK - 1) / SC_LANE_PER_BLOCK;
     L.cls_end[SC_CLASS_LANE] = blocks;
     L.cls_ntask[SC_CLASS_STIM] = n_stims;
     blocks += (n_stims + wpb - 1) / wpb;

@@ -131,7 +131,12 @@ __device__ __forceinline__ int row_back(int row, int back, int R)
 // writes it; the library is compiled with -ffp-contract=off so no product is
 // fused into a following sum.  The sub-step loop is a serial dependence chain, so
 // it is kept rolled.
-__device__ __forceinline__ int lif_step(const ScPop &P, float &u, float &g0, float &g1, int32_t &r, float dge,
+// The leading words of ScPop, up to and including ref_step: all the neuron update
+// reads.  Team blocks copy them into LDS when a task starts.
+#define SC_POP_WORDS ((int)(offsetof(ScPop, ref_step) / 4 + 1))
+
+template <typename PopT>
+__device__ __forceinline__ int lif_step(const PopT &P, float &u, float &g0, float &g1, int32_t &r, float dge,
                                         float dgi)
 {
     const int cn = P.c_num;
@@ -354,16 +359,28 @@ __device__ __forceinline__ const ScInstance &inst_of(const ScInstance &I0, const
     else return insts[i];
 }
 
+// Team-mode neurons.  Teams of one wavefront come 16 to a block, wider teams
+// fewer.  After the gather, each team's first lane turns its row sums into dg
+// and parks dg, the neuron state and the population parameters in LDS; then the
+// block's first wavefront runs the neuron updates of all its teams side by side
+// (one lane per team) instead of one lane in each of up to 16 wavefronts, which
+// would make the SIMDs issue the long update 16 times.
 template <bool SINGLE>
 __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *__restrict__ insts, int t,
                                         int first, int count, const ScTeamTask *__restrict__ teams, int row_single,
                                         int k)
 {
-    __shared__ double acc[SC_BLOCK / SC_WAVE][SC_MAX_IN];
-    __shared__ uint32_t rowoff[SC_BLOCK / SC_WAVE][SC_MAX_IN];
+    constexpr int NT = SC_BLOCK / SC_WAVE;
+    __shared__ double acc[NT][SC_MAX_IN];
+    __shared__ uint32_t rowoff[NT][SC_MAX_IN];
+    constexpr int PW = (SC_POP_WORDS + 1) / 2 * 2;
+    __shared__ alignas(8) uint32_t par[NT][PW];    // population parameters (8-byte aligned rows)
+    __shared__ float sv[NT][8];                    // u, g_exc, g_inh, dge, dgi; refr, device index (as bits)
     const int wave = uniform(threadIdx.x / SC_WAVE), lane = threadIdx.x & (SC_WAVE - 1);
     const int team = wave / t, tw = wave - team * t;
     const bool active = team < count;
+    // one network only: teams of a batch block may belong to different instances
+    const bool gather_updates = SINGLE && count > 1;
     const ScTeamTask *T = active ? &teams[first + team] : nullptr;
     const ScInstance &I = inst_of<SINGLE>(I0, insts, active ? T->inst : 0);
     int row_n = row_single;
@@ -378,6 +395,8 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
             rowoff[team][lane] =
                 lane < T->n_in ? (uint32_t)row_back(row_n, T->in[lane].delay, I.R) * (uint32_t)(2 * I.Wr) : 0u;
         }
+        if (gather_updates && lane < SC_POP_WORDS)
+            par[team][lane] = gbl(reinterpret_cast<const uint32_t *>(&I.pops[T->pop]))[lane];
         if (lane == 0) {  // state of the team's neuron, loaded while the gather runs
             u = gbl(I.V)[d];
             gg = load_g(I.g, d);
@@ -396,9 +415,8 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
         }
     }
     __syncthreads();
+    float dge = 0.f, dgi = 0.f;
     if (active && tw == 0 && lane == 0) {
-        const ScPop &P = I.pops[T->pop];
-        float dge = 0.f, dgi = 0.f;
         for (int j = 0; j < T->n_in; ++j) {
             // dg[post] += S * w_bar (Simulation_header.cu:98), projections in index order
             const float Sf = (float)(acc[team][j] * (double)T->in[j].scale);
@@ -406,15 +424,45 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
             if (T->in[j].exc) dge = dge + add;
             else dgi = dgi + add;
         }
-        const int spk = lif_step(P, u, gg.x, gg.y, rf, dge, dgi);
-        I.V[d] = u;
-        reinterpret_cast<float2 *>(I.g)[d] = gg;
-        I.refr[d] = rf;
-        if (spk) atomicOr(&I.ring[(int64_t)row_n * I.Wr + T->group], 1ull << T->bit);
+        if (gather_updates) {
+            sv[team][0] = u;
+            sv[team][1] = gg.x;
+            sv[team][2] = gg.y;
+            sv[team][3] = dge;
+            sv[team][4] = dgi;
+            sv[team][5] = __int_as_float(rf);
+            sv[team][6] = __int_as_float(d);
+        }
+    }
+    int j = team;  // the team whose update this lane runs
+    bool upd = active && tw == 0 && lane == 0;
+    int dj = d;
+    if (gather_updates) {
+        __syncthreads();
+        j = lane;
+        upd = wave == 0 && lane < count;
+        if (upd) {
+            u = sv[j][0];
+            gg = make_float2(sv[j][1], sv[j][2]);
+            dge = sv[j][3];
+            dgi = sv[j][4];
+            rf = __float_as_int(sv[j][5]);
+            dj = __float_as_int(sv[j][6]);
+        }
+    }
+    if (upd) {
+        int spk;
+        if (gather_updates) spk = lif_step(*reinterpret_cast<const ScPop *>(par[j]), u, gg.x, gg.y, rf, dge, dgi);
+        else spk = lif_step(I.pops[T->pop], u, gg.x, gg.y, rf, dge, dgi);
+        const int group = dj / SC_WAVE, bit = dj & (SC_WAVE - 1);
+        I.V[dj] = u;
+        reinterpret_cast<float2 *>(I.g)[dj] = gg;
+        I.refr[dj] = rf;
+        if (spk) atomicOr(&I.ring[(int64_t)row_n * I.Wr + group], 1ull << bit);
         // the word of the next step starts empty (that row is not read during this step)
-        if (T->bit == 0) {
+        if (bit == 0) {
             const int row1 = row_n + 1 == I.R ? 0 : row_n + 1;
-            I.ring[(int64_t)row1 * I.Wr + T->group] = 0ull;
+            I.ring[(int64_t)row1 * I.Wr + group] = 0ull;
         }
     }
 }
@@ -445,8 +493,9 @@ __device__ __forceinline__ void step_body(const ScInstance &I0, const ScInstance
                         row_single, k);
         return;
     }
-    const int idx = uniform(bi * (SC_BLOCK / SC_WAVE) + wave);
-    if (idx >= L.cls_ntask[c]) return;
+    const int per_block = c == SC_CLASS_LANE ? SC_LANE_PER_BLOCK : SC_BLOCK / SC_WAVE;
+    const int idx = uniform(bi * per_block + wave);
+    if (wave >= per_block || idx >= L.cls_ntask[c]) return;
     if (c == SC_CLASS_LANE) {
         const ScLaneTask &t = lanes[idx];
         const ScInstance &I = inst_of<SINGLE>(I0, insts, t.inst);
