@@ -375,12 +375,12 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
     __shared__ uint32_t rowoff[NT][SC_MAX_IN];
     constexpr int PW = (SC_POP_WORDS + 1) / 2 * 2;
     __shared__ alignas(8) uint32_t par[NT][PW];    // population parameters (8-byte aligned rows)
-    __shared__ float sv[NT][8];                    // u, g_exc, g_inh, dge, dgi; refr, device index (as bits)
+    __shared__ float sv[NT][10];  // u, g_exc, g_inh, dge, dgi; refr, device index, ring row, Wr, R (as bits)
+    __shared__ void *ptr[NT][4];  // instance arrays V, g, refr, ring of each team's network
     const int wave = uniform(threadIdx.x / SC_WAVE), lane = threadIdx.x & (SC_WAVE - 1);
     const int team = wave / t, tw = wave - team * t;
     const bool active = team < count;
-    // one network only: teams of a batch block may belong to different instances
-    const bool gather_updates = SINGLE && count > 1;
+    const bool gather_updates = count > 1;
     const ScTeamTask *T = active ? &teams[first + team] : nullptr;
     const ScInstance &I = inst_of<SINGLE>(I0, insts, active ? T->inst : 0);
     int row_n = row_single;
@@ -432,11 +432,24 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
             sv[team][4] = dgi;
             sv[team][5] = __int_as_float(rf);
             sv[team][6] = __int_as_float(d);
+            if (!SINGLE) {  // teams of a batch block may belong to different instances
+                sv[team][7] = __int_as_float(row_n);
+                sv[team][8] = __int_as_float(I.Wr);
+                sv[team][9] = __int_as_float(I.R);
+                ptr[team][0] = I.V;
+                ptr[team][1] = I.g;
+                ptr[team][2] = I.refr;
+                ptr[team][3] = I.ring;
+            }
         }
     }
     int j = team;  // the team whose update this lane runs
     bool upd = active && tw == 0 && lane == 0;
     int dj = d;
+    float *V = I.V, *G = I.g;
+    int32_t *RF = I.refr;
+    uint64_t *ring = I.ring;
+    int Wr = I.Wr, R = I.R, rown = row_n;
     if (gather_updates) {
         __syncthreads();
         j = lane;
@@ -448,6 +461,15 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
             dgi = sv[j][4];
             rf = __float_as_int(sv[j][5]);
             dj = __float_as_int(sv[j][6]);
+            if (!SINGLE) {
+                rown = __float_as_int(sv[j][7]);
+                Wr = __float_as_int(sv[j][8]);
+                R = __float_as_int(sv[j][9]);
+                V = static_cast<float *>(ptr[j][0]);
+                G = static_cast<float *>(ptr[j][1]);
+                RF = static_cast<int32_t *>(ptr[j][2]);
+                ring = static_cast<uint64_t *>(ptr[j][3]);
+            }
         }
     }
     if (upd) {
@@ -455,14 +477,14 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
         if (gather_updates) spk = lif_step(*reinterpret_cast<const ScPop *>(par[j]), u, gg.x, gg.y, rf, dge, dgi);
         else spk = lif_step(I.pops[T->pop], u, gg.x, gg.y, rf, dge, dgi);
         const int group = dj / SC_WAVE, bit = dj & (SC_WAVE - 1);
-        I.V[dj] = u;
-        reinterpret_cast<float2 *>(I.g)[dj] = gg;
-        I.refr[dj] = rf;
-        if (spk) atomicOr(&I.ring[(int64_t)row_n * I.Wr + group], 1ull << bit);
+        V[dj] = u;
+        reinterpret_cast<float2 *>(G)[dj] = gg;
+        RF[dj] = rf;
+        if (spk) atomicOr(&ring[(int64_t)rown * Wr + group], 1ull << bit);
         // the word of the next step starts empty (that row is not read during this step)
         if (bit == 0) {
-            const int row1 = row_n + 1 == I.R ? 0 : row_n + 1;
-            I.ring[(int64_t)row1 * I.Wr + group] = 0ull;
+            const int row1 = rown + 1 == R ? 0 : rown + 1;
+            ring[(int64_t)row1 * Wr + group] = 0ull;
         }
     }
 }
