@@ -135,6 +135,46 @@ __device__ __forceinline__ int row_back(int row, int back, int R)
 // reads.  Team blocks copy them into LDS when a task starts.
 #define SC_POP_WORDS ((int)(offsetof(ScPop, ref_step) / 4 + 1))
 
+// One Euler sub-step with CN channels; INJ adds the step's synaptic input (the
+// first sub-step only).
+template <int CN, bool INJ>
+__device__ __forceinline__ int lif_substep(float &u, float &g0, float &g1, int32_t &r, double dtc, double dtt0,
+                                           double dtt1, float inj0, float inj1, float E0, float E1, float El, float gL,
+                                           float Ie, float Vth, float Vr, int ref_step)
+{
+    float Isyn = 0.f;
+    if (CN > 0) Isyn = Isyn + g0 * (u - E0);
+    if (CN > 1) Isyn = Isyn + g1 * (u - E1);
+    const float leak = gL * (u - El);
+    const float inner = (Ie - leak) - Isyn;
+    const float du = (float)(dtc * (double)inner);
+    if (CN > 0) {
+        const double dec = (-dtt0) * (double)g0;
+        g0 = (float)((double)g0 + (INJ ? dec + (double)inj0 : dec));
+    }
+    if (CN > 1) {
+        const double dec = (-dtt1) * (double)g1;
+        g1 = (float)((double)g1 + (INJ ? dec + (double)inj1 : dec));
+    }
+    const int fired = u > Vth;
+    r = fired ? ref_step : (r > 0 ? r - 1 : 0);
+    u = r ? Vr : u + du;
+    return fired;
+}
+
+template <int CN>
+__device__ __forceinline__ int lif_steps(float &u, float &g0, float &g1, int32_t &r, double dtc, double dtt0,
+                                         double dtt1, float inj0, float inj1, float E0, float E1, float El, float gL,
+                                         float Ie, float Vth, float Vr, int ref_step)
+{
+    int spk = lif_substep<CN, true>(u, g0, g1, r, dtc, dtt0, dtt1, inj0, inj1, E0, E1, El, gL, Ie, Vth, Vr, ref_step);
+#pragma unroll 1
+    for (int k = 1; k < kStepMax; ++k)
+        spk |= lif_substep<CN, false>(u, g0, g1, r, dtc, dtt0, dtt1, inj0, inj1, E0, E1, El, gL, Ie, Vth, Vr,
+                                      ref_step);
+    return spk;
+}
+
 template <typename PopT>
 __device__ __forceinline__ int lif_step(const PopT &P, float &u, float &g0, float &g1, int32_t &r, float dge,
                                         float dgi)
@@ -145,29 +185,9 @@ __device__ __forceinline__ int lif_step(const PopT &P, float &u, float &g0, floa
     const double dtc = P.dtc, dtt0 = P.dtt[0], dtt1 = P.dtt[1];
     const float E0 = P.E[0], E1 = P.E[1], El = P.El, gL = P.gL, Ie = P.Ie, Vth = P.Vth, Vr = P.Vr;
     const int ref_step = P.ref_step;
-    int spk = 0;
-#pragma unroll 1
-    for (int k = 0; k < kStepMax; ++k) {
-        float Isyn = 0.f;
-        if (cn > 0) Isyn = Isyn + g0 * (u - E0);
-        if (cn > 1) Isyn = Isyn + g1 * (u - E1);
-        const float leak = gL * (u - El);
-        const float inner = (Ie - leak) - Isyn;
-        const float du = (float)(dtc * (double)inner);
-        if (cn > 0) {
-            const double dec = (-dtt0) * (double)g0;
-            g0 = (float)((double)g0 + (k == 0 ? dec + (double)inj0 : dec));
-        }
-        if (cn > 1) {
-            const double dec = (-dtt1) * (double)g1;
-            g1 = (float)((double)g1 + (k == 0 ? dec + (double)inj1 : dec));
-        }
-        const int fired = u > Vth;
-        spk |= fired;
-        r = fired ? ref_step : (r > 0 ? r - 1 : 0);
-        u = r ? Vr : u + du;
-    }
-    return spk;
+    if (cn == 2) return lif_steps<2>(u, g0, g1, r, dtc, dtt0, dtt1, inj0, inj1, E0, E1, El, gL, Ie, Vth, Vr, ref_step);
+    if (cn == 1) return lif_steps<1>(u, g0, g1, r, dtc, dtt0, dtt1, inj0, inj1, E0, E1, El, gL, Ie, Vth, Vr, ref_step);
+    return lif_steps<0>(u, g0, g1, r, dtc, dtt0, dtt1, inj0, inj1, E0, E1, El, gL, Ie, Vth, Vr, ref_step);
 }
 
 // Entries k >= k0 of one sliced-ELL row (slices wider than the batched part), in
