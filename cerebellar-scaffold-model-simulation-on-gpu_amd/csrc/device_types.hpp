@@ -120,6 +120,7 @@ struct ScTask {
 #define SC_MAX_TEAM   16   // wavefronts per team-mode neuron (one 1024-thread block)
 #define SC_BLOCK      1024 // threads per block of the step kernel
 #define SC_LANE_PER_BLOCK 16  // lane-mode groups per block (the other wavefronts idle)
+#define SC_STIM_PER_BLOCK 4   // stimulus groups per block (the other wavefronts idle)
 
 struct ScSlice {           // one lane-mode projection of one 64-neuron group
     const uint32_t *col;   // slice columns, 64 x width, column-major

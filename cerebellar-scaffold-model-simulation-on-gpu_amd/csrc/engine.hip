@@ -76,7 +76,6 @@ static int wbytes_of(int wprec) { return wprec == SC_W_F32 ? 4 : wprec == SC_W_B
 static ScLayout make_layout(const std::vector<int> &team_t, int n_lanes, int n_stims, int *n_blocks)
 {
     ScLayout L{};
-    const int wpb = SC_BLOCK / SC_WAVE;
     int blocks = 0;
     size_t i = 0;
     for (int c = 0; c < SC_CLASS_LANE; ++c) {
@@ -93,7 +92,7 @@ static ScLayout make_layout(const std::vector<int> &team_t, int n_lanes, int n_s
     blocks += (n_lanes + SC_LANE_PER_BLOCK - 1) / SC_LANE_PER_BLOCK;
     L.cls_end[SC_CLASS_LANE] = blocks;
     L.cls_ntask[SC_CLASS_STIM] = n_stims;
-    blocks += (n_stims + wpb - 1) / wpb;
+    blocks += (n_stims + SC_STIM_PER_BLOCK - 1) / SC_STIM_PER_BLOCK;
     L.cls_end[SC_CLASS_STIM] = blocks;
     *n_blocks = blocks;
     return L;

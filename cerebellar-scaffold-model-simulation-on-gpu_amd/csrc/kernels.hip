@@ -535,7 +535,7 @@ __device__ __forceinline__ void step_body(const ScInstance &I0, const ScInstance
                         row_single, k);
         return;
     }
-    const int per_block = c == SC_CLASS_LANE ? SC_LANE_PER_BLOCK : SC_BLOCK / SC_WAVE;
+    const int per_block = c == SC_CLASS_LANE ? SC_LANE_PER_BLOCK : SC_STIM_PER_BLOCK;
     const int idx = uniform(bi * per_block + wave);
     if (wave >= per_block || idx >= L.cls_ntask[c]) return;
     if (c == SC_CLASS_LANE) {
