@@ -1,6 +1,7 @@
 // synth.cpp -- network descriptions: the reference's hard-coded scaffold tables and
 // a seeded, rule-based synthetic scaffold of any area.
 #include <algorithm>
+#include <thread>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -262,23 +263,51 @@ struct Edges {
     std::vector<float> val;
 };
 
+// Rows are independent (each draws from its own stream, seeded by the row index),
+// so they are built in parallel over contiguous row ranges and concatenated in
+// order: the result does not depend on the thread count.
 template <typename F>
 Edges build_rows(int post_n, uint64_t seed, int jitter, F rule)
 {
+    const int nt = std::max(1, std::min<int>(16, (int)std::thread::hardware_concurrency()));
+    const int chunk = (post_n + nt - 1) / std::max(1, nt);
+    std::vector<Edges> part(nt);
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t) {
+        th.emplace_back([&, t]() {
+            Edges &E = part[t];
+            const int j0 = std::min(post_n, t * chunk), j1 = std::min(post_n, j0 + chunk);
+            E.row_ptr.assign(j1 - j0 + 1, 0);
+            std::vector<int32_t> tmp;
+            for (int j = j0; j < j1; ++j) {
+                Rng r(hash3(seed, 0x51, (uint64_t)j));
+                tmp.clear();
+                rule(j, r, tmp);
+                std::sort(tmp.begin(), tmp.end());
+                tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+                for (int32_t c : tmp) {
+                    E.col.push_back(c);
+                    if (jitter) E.val.push_back(0.8f + 0.4f * (float)r.uni());  // U(0.8, 1.2)
+                }
+                E.row_ptr[j - j0 + 1] = (int64_t)E.col.size();
+            }
+        });
+    }
+    for (auto &x : th) x.join();
     Edges E;
     E.row_ptr.assign(post_n + 1, 0);
-    std::vector<int32_t> tmp;
-    for (int j = 0; j < post_n; ++j) {
-        Rng r(hash3(seed, 0x51, (uint64_t)j));
-        tmp.clear();
-        rule(j, r, tmp);
-        std::sort(tmp.begin(), tmp.end());
-        tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
-        for (int32_t c : tmp) {
-            E.col.push_back(c);
-            if (jitter) E.val.push_back(0.8f + 0.4f * (float)r.uni());  // U(0.8, 1.2)
-        }
-        E.row_ptr[j + 1] = (int64_t)E.col.size();
+    size_t nnz = 0;
+    for (auto &P : part) nnz += P.col.size();
+    E.col.reserve(nnz);
+    if (jitter) E.val.reserve(nnz);
+    int j = 0;
+    for (auto &P : part) {
+        const int64_t base = (int64_t)E.col.size();
+        for (size_t r = 1; r < P.row_ptr.size(); ++r) E.row_ptr[++j] = base + P.row_ptr[r];
+        E.col.insert(E.col.end(), P.col.begin(), P.col.end());
+        E.val.insert(E.val.end(), P.val.begin(), P.val.end());
+        std::vector<int32_t>().swap(P.col);
+        std::vector<float>().swap(P.val);
     }
     return E;
 }
