@@ -124,10 +124,22 @@ def batched(pkg, n_inst, steps, warmup, threads):
                 e.describe_synthetic(cfg)
                 e.set_option("weight_precision", pkg.WEIGHTS_BF16)
                 e.set_option("neuron_seed", 77 + i)
+                for k, v in layout.items():
+                    e.set_option(k, v)
                 e.initialize(50, 25)
                 envs[i] = e
         except Exception as ex:  # surfaced below
             errs.append(ex)
+
+    # autotune the layout on one instance of the reference size and use it for all
+    probe = pkg.NetworkEnvironment(1, 7, 16, 2, 0)
+    cfg = pkg.synth_preset(1)
+    cfg.mf_rate_hz = float(np.median(rates))
+    probe.describe_synthetic(cfg)
+    probe.set_option("weight_precision", pkg.WEIGHTS_BF16)
+    tuned = probe.autotune(200)
+    probe.finalize()
+    layout = {k: tuned[k] for k in ("lane_max_fanin", "lane_synapses", "graph_steps")}
 
     order = sorted(range(len(scales)), key=lambda i: -scales[i])
     parts = [order[k::threads] for k in range(threads)]
@@ -138,7 +150,7 @@ def batched(pkg, n_inst, steps, warmup, threads):
         t.join()
     if errs:
         raise errs[0]
-    b = pkg.Batch(envs)
+    b = pkg.Batch(envs)  # replays graphs of the first instance's graph_steps
     b.run(warmup)
     before = sum(e.totals()[1] for e in envs)
     t0 = time.perf_counter()
@@ -154,16 +166,19 @@ def batched(pkg, n_inst, steps, warmup, threads):
             "synapses_scanned_per_s": syn * steps / wall,
             "neurons": int(neurons), "synapses": int(syn), "steps": steps, "ms_per_step": wall * 1e3 / steps,
             "gpu_ms_per_step": ms / steps, "dtype": "bf16 multipliers, f32 state",
-            "aggregate_real_time_factor": len(scales) * steps * 1e-3 / wall}
+            "aggregate_real_time_factor": len(scales) * steps * 1e-3 / wall, "autotuned": tuned}
 
 
-def scaffold16x(pkg, steps):
-    """The scaffold tiled 16x in area (1.55M cells, 50.7M synapses) with FP32, BF16
-    and FP8-e4m3 multipliers: step time, real-time factor, synaptic events/s and
-    the step kernel's algorithmic bandwidth."""
+def scaled(pkg, preset, steps, precisions):
+    """A tiled scaffold (preset 2: 16x in area, 1.55M cells, 50.7M synapses;
+    preset 4: 100x, 9.67M cells, 326M synapses) with the given multiplier storage:
+    step time, real-time factor, synaptic events/s and the step kernel's
+    algorithmic bandwidth."""
     out = {}
-    for name, prec in (("fp32", pkg.WEIGHTS_F32), ("bf16", pkg.WEIGHTS_BF16), ("fp8", pkg.WEIGHTS_FP8)):
-        env = make_env(pkg, 2)
+    names = {pkg.WEIGHTS_F32: "fp32", pkg.WEIGHTS_BF16: "bf16", pkg.WEIGHTS_FP8: "fp8"}
+    for prec in precisions:
+        name = names[prec]
+        env = make_env(pkg, preset)
         env.set_option("weight_precision", prec)
         env.initialize(50, 25)
         info = env.info()
@@ -196,6 +211,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-16x", action="store_true")
     ap.add_argument("--steps-16x", type=int, default=200)
+    ap.add_argument("--no-100x", action="store_true")
+    ap.add_argument("--steps-100x", type=int, default=100)
     ap.add_argument("--batch-instances", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--kernel-steps", type=int, default=100)
@@ -276,7 +293,9 @@ def main():
         b["vs_plain_synapses_scanned"] = b["synapses_scanned_per_s"] / line["synapses_scanned_per_s"]
         line["batched_bf16"] = b
     if not args.no_16x:
-        line["scaffold16x"] = scaffold16x(pkg, args.steps_16x)
+        line["scaffold16x"] = scaled(pkg, 2, args.steps_16x, (pkg.WEIGHTS_F32, pkg.WEIGHTS_BF16, pkg.WEIGHTS_FP8))
+    if not args.no_100x:
+        line["scaffold100x"] = scaled(pkg, 4, args.steps_100x, (pkg.WEIGHTS_FP8, pkg.WEIGHTS_F32))
     print(json.dumps(line))
 
 

@@ -79,6 +79,12 @@ class ProjectionInfo(ctypes.Structure):
                 ("max_conv", ctypes.c_int32), ("lane_mode", ctypes.c_int32), ("fp8_scale", ctypes.c_float)]
 
 
+class TuneResult(ctypes.Structure):
+    _fields_ = [("lane_max_fanin", ctypes.c_int32), ("lane_synapses", ctypes.c_int32),
+                ("graph_steps", ctypes.c_int32), ("n_tried", ctypes.c_int32), ("ms_per_step", ctypes.c_double),
+                ("ms_default", ctypes.c_double)]
+
+
 class StimInfo(ctypes.Structure):
     _fields_ = [("pop", ctypes.c_int32), ("n_entries", ctypes.c_int32), ("n_targets", ctypes.c_int32),
                 ("func", StimFunc)]
@@ -139,6 +145,7 @@ def load_library(path=None):
         "scaffold_get_totals": (ctypes.c_int, [vp, P(u64), P(u64)]),
         "scaffold_write_spike_text": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, cs]),
         "scaffold_time_steps": (ctypes.c_int, [vp, ctypes.c_int, P(f64), P(f64)]),
+        "scaffold_autotune": (ctypes.c_int, [vp, ctypes.c_int, vp]),
         "scaffold_batch_create": (ctypes.c_int, [P(vp)]),
         "scaffold_batch_add": (ctypes.c_int, [vp, vp]),
         "scaffold_batch_build": (ctypes.c_int, [vp]),
@@ -266,6 +273,13 @@ class NetworkEnvironment:
         v = ctypes.c_int64()
         _check(self._lib.scaffold_get_option(self._h, key.encode(), ctypes.byref(v)))
         return v.value
+
+    def autotune(self, n_steps=200):
+        """Time candidate layouts on private trial engines and set the fastest
+        options (before initialize).  Returns the result as a dict."""
+        r = TuneResult()
+        _check(self._lib.scaffold_autotune(self._h, n_steps, ctypes.byref(r)))
+        return {f: getattr(r, f) for f, _ in TuneResult._fields_}
 
     def write_dat(self, index, filename):
         _check(self._lib.scaffold_write_dat(self._h, index, str(filename).encode()))

@@ -852,6 +852,90 @@ int scaffold_time_steps(scaffold_env *env, int n_steps, double *ms_total, double
     return SCAFFOLD_OK;
 }
 
+// One trial engine of the network under the options in env->opt: built,
+// initialized like InitializeNetworkEnv, warmed up, timed, destroyed.
+static int trial_time(scaffold_env *env, int T_print, int n_steps, double *ms)
+{
+    Plan P;
+    P.R = 2 * T_print;
+    int rc = plan_build(env, P);
+    if (rc) return rc;
+    Engine *e = nullptr;
+    rc = engine_create(&e, P);
+    if (rc) return rc;
+    rc = engine_init_params(e, env->opt.neuron_seed, 0);
+    if (!rc) rc = engine_init_params(e, env->opt.neuron_seed, 1);
+    if (!rc) rc = engine_set_ctl(e, 0, 0);
+    if (!rc) rc = engine_run(e, 50, env->opt.graph, env->opt.graph_steps);
+    double total = 0.0;
+    if (!rc) rc = engine_time(e, n_steps, env->opt.graph, env->opt.graph_steps, &total, nullptr);
+    engine_destroy(e);
+    if (!rc) *ms = total / n_steps;
+    return rc;
+}
+
+int scaffold_autotune(scaffold_env *env, int n_steps, scaffold_tune_result *out)
+{
+    clear_error();
+    int rc = check_described(env);
+    if (rc) return rc;
+    if (!out || n_steps <= 0) return fail(SCAFFOLD_ERR_INVALID_ARG, "bad arguments");
+    float max_delay = 0.f;
+    for (auto &p : env->projs)
+        if (p.set && p.delay_ms > max_delay) max_delay = p.delay_ms;
+    const int T_print = std::max(50, (int)max_delay);
+    const Options saved = env->opt;
+    scaffold_tune_result r{};
+    r.ms_per_step = 1e300;
+    r.ms_default = -1.0;
+    const int fanins[] = {16, 48, 128}, synapses[] = {8, 16, 32}, graphs[] = {25, 50};
+    for (int lf : fanins)
+        for (int ls : synapses) {
+            env->opt = saved;
+            env->opt.lane_max_fanin = lf;
+            env->opt.lane_synapses = ls;
+            env->opt.graph_steps = 25;
+            double ms = 0.0;
+            rc = trial_time(env, T_print, n_steps, &ms);
+            if (rc) {
+                env->opt = saved;
+                return rc;
+            }
+            ++r.n_tried;
+            if (lf == Options().lane_max_fanin && ls == Options().lane_synapses) r.ms_default = ms;
+            if (ms < r.ms_per_step) {
+                r.ms_per_step = ms;
+                r.lane_max_fanin = lf;
+                r.lane_synapses = ls;
+                r.graph_steps = 25;
+            }
+        }
+    for (int gs : graphs) {
+        if (gs == 25) continue;
+        env->opt = saved;
+        env->opt.lane_max_fanin = r.lane_max_fanin;
+        env->opt.lane_synapses = r.lane_synapses;
+        env->opt.graph_steps = gs;
+        double ms = 0.0;
+        rc = trial_time(env, T_print, n_steps, &ms);
+        if (rc) {
+            env->opt = saved;
+            return rc;
+        }
+        ++r.n_tried;
+        if (ms < r.ms_per_step) {
+            r.ms_per_step = ms;
+            r.graph_steps = gs;
+        }
+    }
+    env->opt = saved;
+    env->opt.lane_max_fanin = r.lane_max_fanin;
+    env->opt.lane_synapses = r.lane_synapses;
+    env->opt.graph_steps = r.graph_steps;
+    *out = r;
+    return SCAFFOLD_OK;
+}
+
 int scaffold_copy_probe(int64_t bytes, int iters, double *ms)
 {
     clear_error();

@@ -310,6 +310,22 @@ int scaffold_write_spike_text(scaffold_env *env, int step0, int n, const char *f
 int scaffold_time_steps(scaffold_env *env, int n_steps, double *ms_total,
                         double *ms_step_kernel_mean);
 
+/* Autotuner (no counterpart in the reference, whose layout is fixed at compile
+ * time).  Before scaffold_initialize_network_env: builds private trial engines of
+ * this network under candidate layouts -- lane_max_fanin (which populations use
+ * one lane per neuron over sliced-ELL slices, i.e. the ELL width) in {16, 48, 128}
+ * and lane_synapses (synapses per lane, i.e. wavefronts per team-mode neuron) in
+ * {8, 16, 32} -- times n_steps step kernels of each after 50 warm-up steps, then
+ * times graph_steps in {25, 50} for the fastest layout, and sets the winning
+ * options on env.  The network's own state, seeds and counters are untouched. */
+typedef struct {
+    int32_t lane_max_fanin, lane_synapses, graph_steps;
+    int32_t n_tried;
+    double ms_per_step;     /* of the winner, graph replays included */
+    double ms_default;      /* of the default options, for comparison */
+} scaffold_tune_result;
+int scaffold_autotune(scaffold_env *env, int n_steps, scaffold_tune_result *out);
+
 /* --------------------------------------------------------------- batching */
 /* Many initialized environments of uneven size advanced together: every step
  * kernel covers all instances, and runs of steps are replayed as hipGraphs. */

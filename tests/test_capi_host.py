@@ -278,3 +278,16 @@ def test_write_dat_round_trip(pkg, tmp_path):
     r0, c0, _ = src.projection_csr(q)
     r1, c1, _ = env.projection_csr(idx)
     assert (r0 == r1).all() and (c0 == c1).all()
+
+
+def test_autotune_argument_checks(pkg):
+    env = new_env(pkg, 0)
+    with pytest.raises(pkg.ScaffoldError) as e:
+        env.autotune(0)
+    assert e.value.code == pkg.ERR_INVALID_ARG
+    if not HAS_GPU_NODE:
+        with pytest.raises(pkg.ScaffoldError) as e:
+            env.autotune(10)
+        assert e.value.code in (pkg.ERR_NO_DEVICE, pkg.ERR_HIP)
+        # a failed tuning leaves the options as they were
+        assert env.get_option("lane_synapses") == 16 and env.get_option("lane_max_fanin") == 48

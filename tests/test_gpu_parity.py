@@ -342,3 +342,22 @@ def test_long_run_chunking_and_totals(pkg, orc):
     assert np.array_equal(sp.astype(np.int64), tot)
     assert_same_state(env, ref, "300 steps")
     env.finalize()
+
+
+def test_autotune_then_bit_exact(pkg, orc):
+    """The autotuner times private trial engines and only changes layout options:
+    the tuned network still matches the oracle bit for bit, and the tuned options
+    are among the candidates."""
+    env = new_env(pkg, 0)
+    r = env.autotune(50)
+    assert r["n_tried"] == 10 and r["ms_per_step"] > 0 and r["ms_default"] > 0
+    assert r["ms_per_step"] <= r["ms_default"]
+    assert env.get_option("lane_max_fanin") == r["lane_max_fanin"] in (16, 48, 128)
+    assert env.get_option("lane_synapses") == r["lane_synapses"] in (8, 16, 32)
+    assert env.get_option("graph_steps") == r["graph_steps"] in (25, 50)
+    env.initialize(50, 25)
+    ref = orc.OracleSim.from_env(env)
+    ref.initialize()
+    run_and_compare(env, ref, 0, 100)
+    assert_same_state(env, ref, "after autotune")
+    env.finalize()
