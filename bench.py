@@ -216,6 +216,8 @@ def main():
     ap.add_argument("--batch-instances", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--kernel-steps", type=int, default=100)
+    ap.add_argument("--only-scaled", type=int, default=0, choices=(0, 2, 4),
+                    help="only run the tiled scaffold of this preset (2: 16x, 4: 100x) with FP32 and exit")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run the calibration copy kernel on this many MiB and exit")
     args = ap.parse_args()
@@ -225,6 +227,9 @@ def main():
         sys.exit(2)
     import __graft_entry__ as ge
     pkg = ge.load_package()
+    if args.only_scaled:
+        print(json.dumps(scaled(pkg, args.only_scaled, args.steps, (pkg.WEIGHTS_F32,))))
+        return
     if args.copy_probe_mb:
         ms = pkg.copy_probe(args.copy_probe_mb << 20, 20)
         print(json.dumps({"copy_probe_bytes": args.copy_probe_mb << 20, "ms": ms}))
