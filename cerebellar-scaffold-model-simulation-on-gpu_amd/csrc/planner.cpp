@@ -256,7 +256,7 @@ int plan_build(scaffold_env *env, Plan &P)
         sp.W = src.weight > 0 ? src.weight : src.weight * (-1.f); // :733
         sp.pre_pop = src.pre;
         sp.post_pop = src.post;
-        if (sp.delay > P.R) return fail(SCAFFOLD_ERR_INVALID_ARG, "delay beyond the spike ring");
+        if (sp.delay >= P.R) return fail(SCAFFOLD_ERR_INVALID_ARG, "delay beyond the spike ring");
         P.max_delay = std::max(P.max_delay, sp.delay);
         const int64_t nnz = (int64_t)src.col.size();
         P.nnz += nnz;
