@@ -380,8 +380,7 @@ int scaffold_set_option(scaffold_env *env, const char *key, int64_t value)
         o.weight_precision = (int)value;
     } else if (!strcmp(key, "index_bits")) {
         if (frozen) return frozen_err();
-        if (value == 16) return fail(SCAFFOLD_ERR_UNSUPPORTED, "16-bit column indices are not implemented");
-        if (value != 32) return fail(SCAFFOLD_ERR_INVALID_ARG, "index_bits is 32");
+        if (value != 16 && value != 32) return fail(SCAFFOLD_ERR_INVALID_ARG, "index_bits is 16 or 32");
         o.index_bits = (int)value;
     } else if (!strcmp(key, "lane_max_fanin")) {
         if (frozen) return frozen_err();
@@ -624,6 +623,7 @@ int scaffold_get_info(scaffold_env *env, scaffold_info *out)
         i.algorithmic_bytes_per_step = env->plan.algo_bytes;
         i.n_tasks = (int)env->plan.tasks.size();
         i.n_groups = env->plan.n_groups;
+        i.synapses_16bit = env->plan.syn16;
     }
     *out = i;
     return SCAFFOLD_OK;

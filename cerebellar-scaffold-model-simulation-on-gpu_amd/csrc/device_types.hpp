@@ -122,8 +122,13 @@ struct ScTask {
 #define SC_LANE_PER_BLOCK 16  // lane-mode groups per block (the other wavefronts idle)
 #define SC_STIM_PER_BLOCK 4   // stimulus groups per block (the other wavefronts idle)
 
+// 16-bit column form: columns are stored as 16-bit offsets from a 32-bit header
+// per group of 64 entries (one wavefront load); offset 0xFFFF marks padding.
+#define SC_PAD16 0xFFFFu
+
 struct ScSlice {           // one lane-mode projection of one 64-neuron group
-    const uint32_t *col;   // slice columns, 64 x width, column-major
+    const void *col;       // slice columns, 64 x width, column-major (32-bit bits or 16-bit offsets)
+    const uint32_t *hdr;   // 16-bit form: base device bit of each column k (nullptr: 32-bit)
     const void *val;       // slice multipliers (nullptr: unit)
     int32_t width, delay, wprec, exc;
     float W, scale;
@@ -133,7 +138,8 @@ struct ScLaneTask {        // 64 neurons of a lane-mode population
     int32_t inst, group, pop, n_sl;
     int32_t nvalid;
     int32_t wp;            // storage of the slices that have multipliers (SC_W_UNIT: none)
-    int32_t pad1, pad2;
+    int32_t idx16;         // slices in the 16-bit column form
+    int32_t pad2;
     ScSlice sl[SC_MAX_SLICES];
 };
 
@@ -148,7 +154,8 @@ struct ScTeamIn {          // one incoming projection of a team-mode neuron
 };
 
 struct ScTeamTask {        // one neuron of a team-mode population
-    const uint32_t *col;   // merged row columns
+    const void *col;       // merged row columns (32-bit words, or 16-bit offsets)
+    const uint32_t *hdr;   // 16-bit form: group headers of the row (nullptr: 32-bit)
     const void *val;       // merged row multipliers (nullptr: unit)
     int32_t inst, group, bit, pop;
     int32_t n_in, wprec, len, pad_;

@@ -115,6 +115,9 @@ static int upload_work(std::vector<void *> &allocs, int64_t *bytes, DevWork &w, 
     w.lanes = l;
     w.stims = s;
     w.teams = t;
+    w.idx16 = false;
+    for (const ScLaneTask &x : lanes) w.idx16 = w.idx16 || x.idx16;
+    for (const ScTeamTask &x : teams) w.idx16 = w.idx16 || x.hdr;
     return SCAFFOLD_OK;
 }
 
@@ -169,36 +172,60 @@ int engine_create(Engine **out, Plan &P)
     I.ctl = ctl;
 
     // synapse streams: sliced-ELL slices of lane-mode projections
-    std::vector<const uint32_t *> pcol(P.projs.size(), nullptr);
+    std::vector<const uint8_t *> pcol(P.projs.size(), nullptr);  // 32-bit words or 16-bit offsets
+    std::vector<const uint32_t *> phdr(P.projs.size(), nullptr);
     std::vector<const uint8_t *> pval(P.projs.size(), nullptr);
     for (size_t pi = 0; pi < P.projs.size(); ++pi) {
         if (P.projs[pi].mode != SC_MODE_LANE) continue;
         HostProjArrays &H = P.parr[pi];
-        uint32_t *col;
-        if ((rc = dupload(A, B, &col, H.col))) return bail(rc);
-        pcol[pi] = col;
+        if (H.idx16) {
+            uint16_t *c16;
+            uint32_t *h;
+            if ((rc = dupload(A, B, &c16, H.col16))) return bail(rc);
+            if ((rc = dupload(A, B, &h, H.hdr))) return bail(rc);
+            pcol[pi] = reinterpret_cast<const uint8_t *>(c16);
+            phdr[pi] = h;
+        } else {
+            uint32_t *col;
+            if ((rc = dupload(A, B, &col, H.col))) return bail(rc);
+            pcol[pi] = reinterpret_cast<const uint8_t *>(col);
+        }
         if (!H.val.empty()) {
             uint8_t *val;
             if ((rc = dupload(A, B, &val, H.val))) return bail(rc);
             pval[pi] = val;
         }
         std::vector<uint32_t>().swap(H.col);  // now on the device only
+        std::vector<uint16_t>().swap(H.col16);
+        std::vector<uint32_t>().swap(H.hdr);
         std::vector<uint8_t>().swap(H.val);
     }
     // merged rows of team-mode populations
-    std::vector<const uint32_t *> tcol(P.teams.size(), nullptr);
+    std::vector<const uint8_t *> tcol(P.teams.size(), nullptr);
+    std::vector<const uint32_t *> thdr(P.teams.size(), nullptr);
     std::vector<const uint8_t *> tval(P.teams.size(), nullptr);
     for (size_t ti = 0; ti < P.teams.size(); ++ti) {
         TeamPop &T = P.teams[ti];
-        uint32_t *col;
-        if ((rc = dupload(A, B, &col, T.col))) return bail(rc);
-        tcol[ti] = col;
+        if (T.idx16) {
+            uint16_t *c16;
+            uint32_t *h;
+            if ((rc = dupload(A, B, &c16, T.col16))) return bail(rc);
+            if ((rc = dupload(A, B, &h, T.hdr))) return bail(rc);
+            tcol[ti] = reinterpret_cast<const uint8_t *>(c16);
+            thdr[ti] = h;
+        } else {
+            uint32_t *col;
+            if ((rc = dupload(A, B, &col, T.col))) return bail(rc);
+            tcol[ti] = reinterpret_cast<const uint8_t *>(col);
+        }
         if (!T.val.empty()) {
             uint8_t *val;
             if ((rc = dupload(A, B, &val, T.val))) return bail(rc);
             tval[ti] = val;
         }
         std::vector<uint32_t>().swap(T.col);
+        std::vector<uint16_t>().swap(T.col16);
+        std::vector<uint32_t>().swap(T.hdr);
         std::vector<uint8_t>().swap(T.val);
     }
 
@@ -209,7 +236,9 @@ int engine_create(Engine **out, Plan &P)
             const TeamPop &T = P.teams[P.team_of[pop]];
             const ScPop &pp = P.pops[pop];
             ScTeamTask d{};
-            d.col = tcol[P.team_of[pop]] + T.row_ptr[j];
+            const int ti = P.team_of[pop];
+            d.col = tcol[ti] + T.row_ptr[j] * (T.idx16 ? 2 : 4);
+            d.hdr = thdr[ti] ? thdr[ti] + T.row_ptr[j] / SC_WAVE : nullptr;
             d.val = tval[P.team_of[pop]] ? tval[P.team_of[pop]] + T.row_ptr[j] * wbytes_of(T.wprec) : nullptr;
             d.group = t.group;
             d.bit = j % SC_WAVE;
@@ -242,7 +271,9 @@ int engine_create(Engine **out, Plan &P)
                 const HostProjArrays &H = P.parr[pi];
                 ScSlice &s = l.sl[l.n_sl++];
                 const int64_t off = H.sell_off[gi];
-                s.col = pcol[pi] + off;
+                s.col = H.idx16 ? pcol[pi] + H.sell_off16[gi] * 4 : pcol[pi] + off * 4;
+                s.hdr = H.idx16 ? phdr[pi] + H.sell_off16[gi] / SC_WAVE : nullptr;
+                l.idx16 = H.idx16 ? 1 : 0;
                 s.val = pval[pi] ? pval[pi] + off * wbytes_of(sp.wprec) : nullptr;
                 s.width = H.sell_w[gi];
                 s.delay = sp.delay;

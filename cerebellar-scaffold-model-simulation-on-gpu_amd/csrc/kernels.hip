@@ -190,24 +190,50 @@ __device__ __forceinline__ int lif_step(const PopT &P, float &u, float &g0, floa
     return lif_steps<0>(u, g0, g1, r, dtc, dtt0, dtt1, inj0, inj1, E0, E1, El, gL, Ie, Vth, Vr, ref_step);
 }
 
+// 16-bit slices: lane `lane`'s offsets of columns 2p and 2p+1 share the 32-bit
+// word p*64 + lane (low half: column 2p), relative to the pair's header (a
+// wave-uniform base device bit); SC_PAD16 marks padding, which reads bit zb.
+__device__ __forceinline__ uint32_t pair_bit(uint32_t word, int hi, uint32_t base, uint32_t zb)
+{
+    const uint32_t o = hi ? word >> 16 : word & 0xFFFFu;
+    return o == SC_PAD16 ? zb : base + o;
+}
+
 // Entries k >= k0 of one sliced-ELL row (slices wider than the batched part), in
 // batches of U like the head of the row: entry indices past the slice width are
-// clamped to its last column and masked out of the sum.
-template <int WP, int U>
-__device__ __forceinline__ double slice_tail(const ScSlice &sl, gptr<uint32_t> row32, int k0, int lane)
+// clamped to its last column and masked out of the sum.  k0 and U are even.
+template <int WP, int U, bool IDX16>
+__device__ __forceinline__ double slice_tail(const ScSlice &sl, gptr<uint32_t> row32, uint32_t zb, int k0, int lane)
 {
     double S = 0.0;
-    gptr<uint32_t> col = gbl(sl.col);
     const int width = sl.width;
     const bool unit = WP == SC_W_UNIT || !sl.val;
+    gptr<uint32_t> col = gbl(static_cast<const uint32_t *>(sl.col));
 #pragma unroll 1
     for (int k = k0; k < width; k += U) {
         uint32_t c[U], raw[U], w[U];
+        if constexpr (IDX16) {
+            uint32_t pw[U / 2], hb[U / 2];
+            gptr<uint32_t> hdr = gbl(sl.hdr);
+            const int last = (width - 1) >> 1;
 #pragma unroll
-        for (int j = 0; j < U; ++j) {
-            const int e = min(k + j, width - 1) * SC_WAVE + lane;
-            c[j] = col[e];
-            raw[j] = unit ? 0u : wraw<WP>(sl.val, e);
+            for (int j = 0; j < U / 2; ++j) {
+                const int pp = min((k >> 1) + j, last);
+                pw[j] = col[pp * SC_WAVE + lane];
+                hb[j] = hdr[pp];
+            }
+#pragma unroll
+            for (int j = 0; j < U; ++j)
+                raw[j] = unit ? 0u : wraw<WP>(sl.val, min(k + j, width - 1) * SC_WAVE + lane);
+#pragma unroll
+            for (int j = 0; j < U; ++j) c[j] = pair_bit(pw[j >> 1], j & 1, hb[j >> 1], zb);
+        } else {
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+                const int kk = min(k + j, width - 1);
+                c[j] = col[kk * SC_WAVE + lane];
+                raw[j] = unit ? 0u : wraw<WP>(sl.val, kk * SC_WAVE + lane);
+            }
         }
 #pragma unroll
         for (int j = 0; j < U; ++j) w[j] = row32[c[j] >> 5];
@@ -224,7 +250,7 @@ __device__ __forceinline__ double slice_tail(const ScSlice &sl, gptr<uint32_t> r
 // Slice widths and pointers are wave-uniform; slices without multipliers (unit
 // weights) skip the multiplier loads.
 #define SC_LANE_U 4
-template <int WP>
+template <int WP, bool IDX16>
 __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance &I, const ScPop &P, int row_n,
                                            int lane)
 {
@@ -241,6 +267,7 @@ __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance
     }
     constexpr int U = SC_LANE_U;
     uint32_t c[SC_MAX_SLICES][U], raw[SC_MAX_SLICES][U], sw[SC_MAX_SLICES][U];
+    uint32_t pw[SC_MAX_SLICES][U / 2], hb[SC_MAX_SLICES][U / 2];  // 16-bit form
     gptr<uint32_t> row[SC_MAX_SLICES];
     const int n_sl = t.n_sl;
 #pragma unroll
@@ -248,14 +275,25 @@ __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance
         const bool on = s < n_sl;
         const int width = on ? t.sl[s].width : 0;
         row[s] = gbl(reinterpret_cast<const uint32_t *>(I.ring + (int64_t)(on ? row_back(row_n, t.sl[s].delay, I.R) : 0) * I.Wr));
-        gptr<uint32_t> col = gbl(t.sl[s].col);
         const void *val = t.sl[s].val;
+        gptr<uint32_t> col = gbl(static_cast<const uint32_t *>(t.sl[s].col));
+        if constexpr (IDX16) {
+#pragma unroll
+            for (int p = 0; p < U / 2; ++p) {
+                pw[s][p] = 0xFFFFFFFFu;
+                hb[s][p] = 0u;
+                if (2 * p < width) {
+                    pw[s][p] = col[p * SC_WAVE + lane];
+                    hb[s][p] = gbl(t.sl[s].hdr)[p];
+                }
+            }
+        }
 #pragma unroll
         for (int k = 0; k < U; ++k) {
             c[s][k] = zb;
             raw[s][k] = 0u;
             if (k < width) {
-                c[s][k] = col[k * SC_WAVE + lane];
+                if constexpr (!IDX16) c[s][k] = col[k * SC_WAVE + lane];
                 if (WP != SC_W_UNIT && val) raw[s][k] = wraw<WP>(val, k * SC_WAVE + lane);
             }
         }
@@ -263,7 +301,10 @@ __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance
 #pragma unroll
     for (int s = 0; s < SC_MAX_SLICES; ++s)
 #pragma unroll
-        for (int k = 0; k < U; ++k) sw[s][k] = row[s][c[s][k] >> 5];
+        for (int k = 0; k < U; ++k) {
+            if constexpr (IDX16) c[s][k] = pair_bit(pw[s][k >> 1], k & 1, hb[s][k >> 1], zb);
+            sw[s][k] = row[s][c[s][k] >> 5];
+        }
     float dge = 0.f, dgi = 0.f;
 #pragma unroll
     for (int s = 0; s < SC_MAX_SLICES; ++s) {
@@ -273,7 +314,7 @@ __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance
 #pragma unroll
         for (int k = 0; k < U; ++k)
             if ((sw[s][k] >> (c[s][k] & 31)) & 1u) S += unit ? 1.0 : (double)wdec<WP>(raw[s][k]);
-        if (t.sl[s].width > U) S += slice_tail<WP, U>(t.sl[s], row[s], U, lane);
+        if (t.sl[s].width > U) S += slice_tail<WP, U, IDX16>(t.sl[s], row[s], zb, U, lane);
         // dg[post] += S * w_bar (Simulation_header.cu:98), projections in index order
         const float Sf = (float)(S * (double)t.sl[s].scale);
         const float add = Sf * t.sl[s].W;
@@ -300,28 +341,61 @@ __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance
 // projection's ring row.  Spiking entries add their multiplier to the team's f64
 // LDS accumulator of the projection: the sums are exact, so the order of the
 // additions cannot change them.
-template <int WP, int U>
+//
+// In the 16-bit form the row is a sequence of 64-entry groups (one wavefront
+// load each; every projection segment padded to whole groups), so a group's
+// header -- projection slot and base bit -- is wave-uniform, and its entries are
+// 16-bit offsets from the base (SC_PAD16: padding, masked out).
+template <int WP, int U, bool IDX16>
 __device__ __forceinline__ void team_gather(const ScTeamTask &T, gptr<uint32_t> ring32, const uint32_t *rowoff,
                                             int tid, int TT, double *acc)
 {
     const int len = T.len;
-    gptr<uint32_t> col = gbl(T.col);
     const void *val = T.val;
-    for (int b = tid; b < len; b += TT * U) {
-        uint32_t c[U], raw[U], s[U];
+    if constexpr (IDX16) {
+        gptr<uint16_t> col = gbl(static_cast<const uint16_t *>(T.col));
+        gptr<uint32_t> hdr = gbl(T.hdr);
+        const int lane = tid & (SC_WAVE - 1), ng = len / SC_WAVE, gstep = TT / SC_WAVE;
+        for (int b = tid; b < len; b += TT * U) {
+            const int g0 = uniform((b - lane) / SC_WAVE);  // group of this wavefront's first entry
+            uint32_t o[U], h[U], raw[U], s[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int i = min(b + u * TT, len - 1);
-            c[u] = col[i];
-            raw[u] = wraw<WP>(val, i);
+            for (int u = 0; u < U; ++u) {
+                const int g = min(g0 + u * gstep, ng - 1);
+                h[u] = hdr[g];
+                o[u] = col[g * SC_WAVE + lane];
+                raw[u] = wraw<WP>(val, g * SC_WAVE + lane);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t bit = (h[u] & SC_BIT_MASK) + (o[u] == SC_PAD16 ? 0u : o[u]);
+                s[u] = ring32[rowoff[h[u] >> SC_Q_SHIFT] + (bit >> 5)];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t bit = (h[u] & SC_BIT_MASK) + o[u];
+                if (g0 + u * gstep < ng && o[u] != SC_PAD16 && ((s[u] >> (bit & 31)) & 1u))
+                    atomicAdd(&acc[h[u] >> SC_Q_SHIFT], (double)wdec<WP>(raw[u]));
+            }
         }
+    } else {
+        gptr<uint32_t> col = gbl(static_cast<const uint32_t *>(T.col));
+        for (int b = tid; b < len; b += TT * U) {
+            uint32_t c[U], raw[U], s[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-            s[u] = ring32[rowoff[c[u] >> SC_Q_SHIFT] + ((c[u] & SC_BIT_MASK) >> 5)];
+            for (int u = 0; u < U; ++u) {
+                const int i = min(b + u * TT, len - 1);
+                c[u] = col[i];
+                raw[u] = wraw<WP>(val, i);
+            }
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (b + u * TT < len && ((s[u] >> (c[u] & 31)) & 1u))
-                atomicAdd(&acc[c[u] >> SC_Q_SHIFT], (double)wdec<WP>(raw[u]));
+            for (int u = 0; u < U; ++u)
+                s[u] = ring32[rowoff[c[u] >> SC_Q_SHIFT] + ((c[u] & SC_BIT_MASK) >> 5)];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (b + u * TT < len && ((s[u] >> (c[u] & 31)) & 1u))
+                    atomicAdd(&acc[c[u] >> SC_Q_SHIFT], (double)wdec<WP>(raw[u]));
+        }
     }
 }
 
@@ -385,7 +459,7 @@ __device__ __forceinline__ const ScInstance &inst_of(const ScInstance &I0, const
 // block's first wavefront runs the neuron updates of all its teams side by side
 // (one lane per team) instead of one lane in each of up to 16 wavefronts, which
 // would make the SIMDs issue the long update 16 times.
-template <bool SINGLE>
+template <bool SINGLE, bool IDX16>
 __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *__restrict__ insts, int t,
                                         int first, int count, const ScTeamTask *__restrict__ teams, int row_single,
                                         int k)
@@ -427,11 +501,20 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
     if (active) {
         gptr<uint32_t> ring32 = gbl(reinterpret_cast<const uint32_t *>(I.ring));
         const int tid = tw * SC_WAVE + lane, TT = t * SC_WAVE;
-        switch (T->wprec) {
-        case SC_W_F32: team_gather<SC_W_F32, 8>(*T, ring32, rowoff[team], tid, TT, acc[team]); break;
-        case SC_W_BF16: team_gather<SC_W_BF16, 8>(*T, ring32, rowoff[team], tid, TT, acc[team]); break;
-        case SC_W_FP8: team_gather<SC_W_FP8, 8>(*T, ring32, rowoff[team], tid, TT, acc[team]); break;
-        default: team_gather<SC_W_UNIT, 8>(*T, ring32, rowoff[team], tid, TT, acc[team]); break;
+        if (IDX16 && T->hdr) {
+            switch (T->wprec) {
+            case SC_W_F32: team_gather<SC_W_F32, 8, true>(*T, ring32, rowoff[team], tid, TT, acc[team]); break;
+            case SC_W_BF16: team_gather<SC_W_BF16, 8, true>(*T, ring32, rowoff[team], tid, TT, acc[team]); break;
+            case SC_W_FP8: team_gather<SC_W_FP8, 8, true>(*T, ring32, rowoff[team], tid, TT, acc[team]); break;
+            default: team_gather<SC_W_UNIT, 8, true>(*T, ring32, rowoff[team], tid, TT, acc[team]); break;
+            }
+        } else {
+            switch (T->wprec) {
+            case SC_W_F32: team_gather<SC_W_F32, 8, false>(*T, ring32, rowoff[team], tid, TT, acc[team]); break;
+            case SC_W_BF16: team_gather<SC_W_BF16, 8, false>(*T, ring32, rowoff[team], tid, TT, acc[team]); break;
+            case SC_W_FP8: team_gather<SC_W_FP8, 8, false>(*T, ring32, rowoff[team], tid, TT, acc[team]); break;
+            default: team_gather<SC_W_UNIT, 8, false>(*T, ring32, rowoff[team], tid, TT, acc[team]); break;
+            }
         }
     }
     __syncthreads();
@@ -512,7 +595,7 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
 // The block layout is a kernel argument: block b belongs to the first class c with
 // b < cls_end[c]; classes 0..4 are team neurons of width 16, 8, 4, 2, 1
 // wavefronts, class 5 lane groups, class 6 stimulus groups.
-template <bool SINGLE>
+template <bool SINGLE, bool IDX16>
 __device__ __forceinline__ void step_body(const ScInstance &I0, const ScInstance *__restrict__ insts, const ScLayout &L,
                                           const ScLaneTask *__restrict__ lanes, const ScStimTask *__restrict__ stt,
                                           const ScTeamTask *__restrict__ teams, int k)
@@ -531,7 +614,7 @@ __device__ __forceinline__ void step_body(const ScInstance &I0, const ScInstance
     const int wave = uniform(threadIdx.x / SC_WAVE), lane = threadIdx.x & (SC_WAVE - 1);
     if (c < SC_CLASS_LANE) {
         const int t = SC_MAX_TEAM >> c, per = 1 << c;
-        do_team<SINGLE>(I0, insts, t, L.cls_task0[c] + bi * per, min(per, L.cls_ntask[c] - bi * per), teams,
+        do_team<SINGLE, IDX16>(I0, insts, t, L.cls_task0[c] + bi * per, min(per, L.cls_ntask[c] - bi * per), teams,
                         row_single, k);
         return;
     }
@@ -543,11 +626,20 @@ __device__ __forceinline__ void step_body(const ScInstance &I0, const ScInstance
         const ScInstance &I = inst_of<SINGLE>(I0, insts, t.inst);
         const int row_n = SINGLE ? row_single : (I.ctl->row + k) % I.R;
         const ScPop &P = I.pops[t.pop];
-        switch (t.wp) {
-        case SC_W_F32: lane_group<SC_W_F32>(t, I, P, row_n, lane); break;
-        case SC_W_BF16: lane_group<SC_W_BF16>(t, I, P, row_n, lane); break;
-        case SC_W_FP8: lane_group<SC_W_FP8>(t, I, P, row_n, lane); break;
-        default: lane_group<SC_W_UNIT>(t, I, P, row_n, lane); break;
+        if (IDX16 && t.idx16) {
+            switch (t.wp) {
+            case SC_W_F32: lane_group<SC_W_F32, true>(t, I, P, row_n, lane); break;
+            case SC_W_BF16: lane_group<SC_W_BF16, true>(t, I, P, row_n, lane); break;
+            case SC_W_FP8: lane_group<SC_W_FP8, true>(t, I, P, row_n, lane); break;
+            default: lane_group<SC_W_UNIT, true>(t, I, P, row_n, lane); break;
+            }
+        } else {
+            switch (t.wp) {
+            case SC_W_F32: lane_group<SC_W_F32, false>(t, I, P, row_n, lane); break;
+            case SC_W_BF16: lane_group<SC_W_BF16, false>(t, I, P, row_n, lane); break;
+            case SC_W_FP8: lane_group<SC_W_FP8, false>(t, I, P, row_n, lane); break;
+            default: lane_group<SC_W_UNIT, false>(t, I, P, row_n, lane); break;
+            }
         }
     } else {
         const ScStimTask t = stt[idx];
@@ -566,23 +658,26 @@ __device__ __forceinline__ void step_body(const ScInstance &I0, const ScInstance
 
 // One network: 64 VGPRs at most, so two 1024-thread blocks share a CU and one
 // block's serial phases (descriptor loads, barriers, the neuron update) overlap
-// the other's gathers.
+// the other's gathers.  IDX16: the network holds 16-bit column offsets (the
+// instantiation without them carries none of their code or registers).
+template <bool IDX16>
 __global__ void __launch_bounds__(SC_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_step(ScInstance I0, ScLayout L, const ScLaneTask *__restrict__ lanes, const ScStimTask *__restrict__ stt,
        const ScTeamTask *__restrict__ teams, int k)
 {
-    step_body<true>(I0, nullptr, L, lanes, stt, teams, k);
+    step_body<true, IDX16>(I0, nullptr, L, lanes, stt, teams, k);
 }
 
 // Many instances: every task names its instance in the instance table.  Held to
 // 64 VGPRs as well (a few spill to scratch): the doubled occupancy outweighs
 // the spill traffic.
+template <bool IDX16>
 __global__ void __launch_bounds__(SC_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) k_step_batch(const ScInstance *__restrict__ insts, ScLayout L,
                                                          const ScLaneTask *__restrict__ lanes,
                                                          const ScStimTask *__restrict__ stt,
                                                          const ScTeamTask *__restrict__ teams, int k)
 {
-    step_body<false>(ScInstance{}, insts, L, lanes, stt, teams, k);
+    step_body<false, IDX16>(ScInstance{}, insts, L, lanes, stt, teams, k);
 }
 
 // Per-population spike counts and synaptic events of K consecutive steps: one
@@ -713,10 +808,14 @@ __global__ void k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, i
 void launch_step(hipStream_t s, const ScInstance *single, const ScInstance *insts, const DevWork &w, int k)
 {
     if (w.n_blocks <= 0) return;
-    if (single)
-        k_step<<<w.n_blocks, SC_BLOCK, 0, s>>>(*single, w.layout, w.lanes, w.stims, w.teams, k);
+    if (single && w.idx16)
+        k_step<true><<<w.n_blocks, SC_BLOCK, 0, s>>>(*single, w.layout, w.lanes, w.stims, w.teams, k);
+    else if (single)
+        k_step<false><<<w.n_blocks, SC_BLOCK, 0, s>>>(*single, w.layout, w.lanes, w.stims, w.teams, k);
+    else if (w.idx16)
+        k_step_batch<true><<<w.n_blocks, SC_BLOCK, 0, s>>>(insts, w.layout, w.lanes, w.stims, w.teams, k);
     else
-        k_step_batch<<<w.n_blocks, SC_BLOCK, 0, s>>>(insts, w.layout, w.lanes, w.stims, w.teams, k);
+        k_step_batch<false><<<w.n_blocks, SC_BLOCK, 0, s>>>(insts, w.layout, w.lanes, w.stims, w.teams, k);
 }
 
 void launch_stats(hipStream_t s, const ScInstance *single, const ScInstance *insts, int n_inst, int K)

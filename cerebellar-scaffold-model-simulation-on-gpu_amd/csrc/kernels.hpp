@@ -20,6 +20,7 @@ struct DevWork {
     const ScLaneTask *lanes = nullptr;
     const ScStimTask *stims = nullptr;
     const ScTeamTask *teams = nullptr;
+    bool idx16 = false;  // some descriptor holds 16-bit column offsets
 };
 
 // All launchers are asynchronous on stream s.  `single` != nullptr selects the

@@ -65,12 +65,18 @@ struct Options {
 // Everything the engine uploads, built once by plan_build().
 struct HostProjArrays {
     std::vector<uint32_t> col;
+    // 16-bit form of lane slices: the offsets of columns 2p and 2p+1 of a lane
+    // share one 32-bit word (low half: column 2p) at word p*64 + lane of the
+    // slice, both relative to the pair header hdr (base device bit); 0xFFFF marks
+    // padding.  sell_off16: start of each slice in 32-bit words.
     std::vector<uint16_t> col16;
+    std::vector<uint32_t> hdr;
+    std::vector<int64_t> sell_off16;
+    bool idx16 = false;
     std::vector<uint8_t> val;       // raw bytes of f32 / bf16 / fp8 multipliers
     std::vector<int64_t> row_ptr;
     std::vector<int64_t> sell_off;
     std::vector<int32_t> sell_w;
-    std::vector<int32_t> sell_base;
 };
 
 // Team-mode population: one merged CSR row per post neuron holding its synapses of
@@ -83,6 +89,12 @@ struct TeamPop {
     std::vector<int32_t> seg_end;   // num x n_in: end of each projection's segment, relative to the row
     std::vector<uint32_t> col;      // device bit indices
     std::vector<uint8_t> val;       // encoded multipliers
+    // 16-bit form: every projection segment of a row padded to whole groups of 64
+    // entries; a group's header holds its projection slot (top 3 bits) and base
+    // device bit, its entries 16-bit offsets from the base (0xFFFF: padding)
+    bool idx16 = false;
+    std::vector<uint16_t> col16;
+    std::vector<uint32_t> hdr;
 };
 
 struct Plan {
@@ -102,6 +114,7 @@ struct Plan {
     std::vector<ScTask> tasks;            // step kernel work list (planner form)
     int64_t nnz = 0;
     int64_t algo_bytes = 0;
+    int64_t syn16 = 0;  // synapses stored as 16-bit column offsets
     int max_delay = 0;
     std::vector<int> out_pops;            // OUTPUT populations
     int n_output = 0;

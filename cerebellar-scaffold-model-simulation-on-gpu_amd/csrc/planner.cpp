@@ -144,8 +144,7 @@ static bool row_sums_exact(const std::vector<float> &v, const std::vector<int64_
 int plan_build(scaffold_env *env, Plan &P)
 {
     const Options &opt = env->opt;
-    if (opt.index_bits != 32)
-        return fail(SCAFFOLD_ERR_UNSUPPORTED, "16-bit column offsets are not available in this build");
+    const bool want16 = opt.index_bits == 16;
     const int npop = env->n_types;
     P.pops.assign(npop, ScPop{});
     int dev = 0, ref = 0;
@@ -324,6 +323,67 @@ int plan_build(scaffold_env *env, Plan &P)
                 }
             }
     }
+    // 16-bit form of lane-mode slices, per post population: columns are taken in
+    // pairs (2p, 2p+1), whose 128 entries must span fewer than 65,535 device bits
+    if (want16) {
+        struct Conv {
+            std::vector<uint16_t> c16;
+            std::vector<uint32_t> h;
+            std::vector<int64_t> off16;
+        };
+        for (int p = 0; p < npop; ++p) {
+            const ScPop &d = P.pops[p];
+            if (team_pop[p] || d.kind == SCAFFOLD_INPUT || d.num == 0) continue;
+            std::vector<Conv> conv;
+            bool ok = true;
+            for (int q = 0; q < d.n_in && ok; ++q) {
+                const HostProjArrays &A = P.parr[d.in_proj[q]];
+                const size_t ng = A.sell_off.size();
+                Conv C;
+                C.off16.assign(ng, 0);
+                int64_t words = 0;
+                for (size_t gi = 0; gi < ng; ++gi) {
+                    C.off16[gi] = words;
+                    words += (int64_t)(A.sell_w[gi] + 1) / 2 * SC_WAVE;
+                }
+                C.c16.assign(words * 2, (uint16_t)SC_PAD16);
+                C.h.assign(words / SC_WAVE, 0);
+                for (size_t gi = 0; gi < ng && ok; ++gi) {
+                    const int w = A.sell_w[gi];
+                    for (int pp = 0; 2 * pp < w && ok; ++pp) {
+                        uint32_t lo = UINT32_MAX, hi = 0;
+                        for (int k = 2 * pp; k < std::min(w, 2 * pp + 2); ++k)
+                            for (int l = 0; l < SC_WAVE; ++l) {
+                                const uint32_t b = A.col[A.sell_off[gi] + (int64_t)k * SC_WAVE + l];
+                                if (b == sentinel) continue;
+                                lo = std::min(lo, b);
+                                hi = std::max(hi, b);
+                            }
+                        if (lo == UINT32_MAX) continue;
+                        if (hi - lo >= SC_PAD16) { ok = false; break; }
+                        const int64_t wbase = C.off16[gi] + (int64_t)pp * SC_WAVE;
+                        C.h[wbase / SC_WAVE] = lo;
+                        for (int k = 2 * pp; k < std::min(w, 2 * pp + 2); ++k)
+                            for (int l = 0; l < SC_WAVE; ++l) {
+                                const uint32_t b = A.col[A.sell_off[gi] + (int64_t)k * SC_WAVE + l];
+                                if (b != sentinel) C.c16[(wbase + l) * 2 + (k & 1)] = (uint16_t)(b - lo);
+                            }
+                    }
+                }
+                conv.push_back(std::move(C));
+            }
+            if (!ok) continue;  // this population keeps 32-bit columns
+            for (int q = 0; q < d.n_in; ++q) {
+                HostProjArrays &A = P.parr[d.in_proj[q]];
+                A.col16 = std::move(conv[q].c16);
+                A.hdr = std::move(conv[q].h);
+                A.sell_off16 = std::move(conv[q].off16);
+                A.idx16 = true;
+                std::vector<uint32_t>().swap(A.col);
+            }
+        }
+    }
+
     // team-mode merged rows
     P.teams.clear();
     P.team_of.assign(npop, -1);
@@ -360,6 +420,57 @@ int plan_build(scaffold_env *env, Plan &P)
                 }
             }
         }
+        if (want16) {
+            // 16-bit form: each projection segment padded to whole groups of 64
+            // entries; a group must span fewer than 65,535 device bits
+            std::vector<int64_t> rp(d.num + 1, 0);
+            std::vector<int32_t> se((size_t)d.num * d.n_in, 0);
+            for (int j = 0; j < d.num; ++j) {
+                int64_t len = 0, prev = 0;
+                for (int q = 0; q < d.n_in; ++q) {
+                    const int64_t end = T.seg_end[(size_t)j * d.n_in + q];
+                    len += (end - prev + SC_WAVE - 1) / SC_WAVE * SC_WAVE;
+                    prev = end;
+                    se[(size_t)j * d.n_in + q] = (int32_t)len;
+                }
+                rp[j + 1] = rp[j] + len;
+            }
+            const int64_t tot16 = rp[d.num];
+            std::vector<uint16_t> c16(tot16, (uint16_t)SC_PAD16);
+            std::vector<uint32_t> h(tot16 / SC_WAVE, 0);
+            std::vector<uint8_t> v16(wb ? tot16 * wb : 0, 0);
+            bool ok = true;
+            for (int j = 0; j < d.num && ok; ++j)
+                for (int q = 0; q < d.n_in && ok; ++q) {
+                    const int64_t s0 = T.row_ptr[j] + (q ? T.seg_end[(size_t)j * d.n_in + q - 1] : 0);
+                    const int64_t s1 = T.row_ptr[j] + T.seg_end[(size_t)j * d.n_in + q];
+                    const int64_t d0 = rp[j] + (q ? se[(size_t)j * d.n_in + q - 1] : 0);
+                    for (int64_t g = 0; g * SC_WAVE < s1 - s0; ++g) {
+                        const int64_t a = s0 + g * SC_WAVE, b = std::min(s1, a + SC_WAVE);
+                        uint32_t lo = UINT32_MAX, hi = 0;
+                        for (int64_t e = a; e < b; ++e) {
+                            lo = std::min(lo, T.col[e] & SC_BIT_MASK);
+                            hi = std::max(hi, T.col[e] & SC_BIT_MASK);
+                        }
+                        if (hi - lo >= SC_PAD16) { ok = false; break; }
+                        const int64_t o = d0 + g * SC_WAVE;
+                        h[o / SC_WAVE] = ((uint32_t)q << SC_Q_SHIFT) | lo;
+                        for (int64_t e = a; e < b; ++e) {
+                            c16[o + (e - a)] = (uint16_t)((T.col[e] & SC_BIT_MASK) - lo);
+                            if (wb) memcpy(&v16[(o + (e - a)) * wb], &T.val[e * wb], wb);
+                        }
+                    }
+                }
+            if (ok) {
+                T.idx16 = true;
+                T.row_ptr = std::move(rp);
+                T.seg_end = std::move(se);
+                T.col16 = std::move(c16);
+                T.hdr = std::move(h);
+                T.val = std::move(v16);
+                std::vector<uint32_t>().swap(T.col);
+            }
+        }
         // team width: about lane_synapses synapses per lane for the mean row
         const double per_lane = (double)tot / std::max(1, d.num) / SC_WAVE / std::max(1, opt.lane_synapses);
         int t = 1;
@@ -368,7 +479,17 @@ int plan_build(scaffold_env *env, Plan &P)
         P.team_of[p] = (int)P.teams.size();
         P.teams.push_back(std::move(T));
     }
-    P.algo_bytes = P.nnz * 4 + weight_bytes + (int64_t)ref * 32 + ref / 8;
+    int64_t index_bytes = 0;  // 2 B per synapse in the 16-bit form, else 4 B
+    P.syn16 = 0;
+    for (int pi = 0; pi < nproj; ++pi) {
+        const Projection &src = env->projs[pi];
+        if (!src.set) continue;
+        const ScProj &sp = P.projs[pi];
+        const bool is16 = sp.mode == SC_MODE_LANE ? P.parr[pi].idx16 : P.teams[P.team_of[sp.post_pop]].idx16;
+        index_bytes += (int64_t)src.col.size() * (is16 ? 2 : 4);
+        if (is16) P.syn16 += (int64_t)src.col.size();
+    }
+    P.algo_bytes = index_bytes + weight_bytes + (int64_t)ref * 32 + ref / 8;
     for (auto &g : P.groups) g.self_final = 1;
 
     // work list: team neurons (widest teams first), lane groups, stimulus groups

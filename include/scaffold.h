@@ -155,8 +155,11 @@ int scaffold_set_input_stim_ids(scaffold_env *env, int neuron_index,
  *   "neuron_seed"       Philox key of the initial-potential draw (InitParams,
  *                       init_params.cu:343-352); default 0x5eed
  *   "weight_precision"  SCAFFOLD_WEIGHTS_*; default F32
- *   "index_bits"        32 (the only width implemented; 16 returns
- *                       SCAFFOLD_ERR_UNSUPPORTED)
+ *   "index_bits"        32 (default) or 16: column indices stored as 16-bit
+ *                       offsets from a base device index held once per 64- or
+ *                       128-entry group; a population whose groups span 65,535
+ *                       or more presynaptic neurons keeps 32-bit indices
+ *                       (scaffold_info.synapses_16bit counts the converted ones)
  *   "lane_max_fanin"    populations whose mean total fan-in is <= this are
  *                       gathered one lane per post neuron (sliced ELL); others by a
  *                       team of wavefronts per post neuron (merged CSR row)
@@ -248,6 +251,7 @@ typedef struct {
     int64_t  device_bytes;         /* device memory held by this environment   */
     int64_t  algorithmic_bytes_per_step; /* nnz*(idx+w bytes) + N*32 + N/8     */
     int32_t  n_tasks, n_groups;
+    int64_t  synapses_16bit;       /* synapses stored as 16-bit column offsets */
 } scaffold_info;
 int scaffold_get_info(scaffold_env *env, scaffold_info *out);
 

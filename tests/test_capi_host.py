@@ -178,8 +178,10 @@ def test_options(pkg):
     assert env.get_option("weight_precision") == pkg.WEIGHTS_BF16
     env.set_option("neuron_seed", 99)
     assert env.get_option("neuron_seed") == 99
+    env.set_option("index_bits", 16)
+    assert env.get_option("index_bits") == 16
     for k, v, code in [("no_such_option", 1, "ERR_INVALID_ARG"), ("weight_precision", 7, "ERR_INVALID_ARG"),
-                       ("index_bits", 16, "ERR_UNSUPPORTED"), ("graph_steps", 0, "ERR_INVALID_ARG")]:
+                       ("index_bits", 8, "ERR_INVALID_ARG"), ("graph_steps", 0, "ERR_INVALID_ARG")]:
         with pytest.raises(pkg.ScaffoldError) as e:
             env.set_option(k, v)
         assert e.value.code == getattr(pkg, code), k

@@ -201,6 +201,27 @@ def test_quantized_weights_bit_exact(pkg, orc, precision):
     env.finalize()
 
 
+@pytest.mark.parametrize("preset,precision,fanin", [(0, 0, 48), (0, 2, 48), (0, 1, 4096), (0, 0, 0), (1, 0, 48),
+                                                   (1, 2, 48)])
+def test_index16_bit_exact(pkg, orc, preset, precision, fanin):
+    """16-bit column offsets: lane-mode slices (column pairs with a shared base)
+    and team-mode merged rows (64-entry groups with a shared base) give the
+    oracle's rasters and state bit for bit.  lane_max_fanin 4096 puts every
+    population in lane mode (wide slices), 0 every one in team mode."""
+    env = new_env(pkg, preset)
+    env.set_option("index_bits", 16)
+    env.set_option("weight_precision", precision)
+    env.set_option("lane_max_fanin", fanin)
+    env.initialize(50, 25)
+    info = env.info()
+    assert 0 < info.synapses_16bit <= info.total_synapses
+    ref = orc.OracleSim.from_env(env)
+    ref.initialize()
+    run_and_compare(env, ref, 0, 200 if preset == 0 else 100)
+    assert_same_state(env, ref, f"index_bits 16, precision {precision}, lane_max_fanin {fanin}")
+    env.finalize()
+
+
 def test_unit_weights_and_reference_delays(pkg, orc):
     env = new_env(pkg, 0, weight_jitter=0, reference_delays=1)
     env.initialize(50, 25)
@@ -292,14 +313,17 @@ def test_output_train_fired_and_text(pkg, orc, tmp_path):
     env.finalize()
 
 
-def test_batch_matches_single(pkg, orc):
+@pytest.mark.parametrize("index_bits", [32, 16])
+def test_batch_matches_single(pkg, orc, index_bits):
     """Uneven instances advanced by the batched step kernel reproduce their own
-    oracle runs exactly."""
+    oracle runs exactly (with index_bits 16 the instances mix 16-bit and 32-bit
+    column indices)."""
     envs = []
     for i, (scale, rate) in enumerate([(0.1, 4.0), (0.05, 20.0), (0.15, 8.0)]):
         e = new_env(pkg, 0, scale=scale, counts=(0,) * 8, seed=100 + i, stim_seed=7 + i, mf_rate_hz=rate)
         e.set_option("neuron_seed", 300 + i)
         e.set_option("weight_precision", pkg.WEIGHTS_BF16)
+        e.set_option("index_bits", index_bits if i != 1 else 32)
         e.initialize(50, 25)
         envs.append(e)
     refs = []
