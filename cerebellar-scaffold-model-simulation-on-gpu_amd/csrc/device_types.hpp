@@ -122,9 +122,12 @@ struct ScTask {
 #define SC_LANE_PER_BLOCK 16  // lane-mode groups per block (the other wavefronts idle)
 #define SC_STIM_PER_BLOCK 4   // stimulus groups per block (the other wavefronts idle)
 
-// 16-bit column form: columns are stored as 16-bit offsets from a 32-bit header
-// per group of 64 entries (one wavefront load); offset 0xFFFF marks padding.
+// 16-bit column form: columns are stored as 16-bit offsets from base device bits
+// held in headers (lane slices: one per column pair; team rows: two per 64-entry
+// group, the second selected by bit 15 of the entry, offsets below 0x7FFF);
+// 0xFFFF marks padding.
 #define SC_PAD16 0xFFFFu
+#define SC_OFF15_PAD 0x7FFFu
 
 struct ScSlice {           // one lane-mode projection of one 64-neuron group
     const void *col;       // slice columns, 64 x width, column-major (32-bit bits or 16-bit offsets)

@@ -238,7 +238,7 @@ int engine_create(Engine **out, Plan &P)
             ScTeamTask d{};
             const int ti = P.team_of[pop];
             d.col = tcol[ti] + T.row_ptr[j] * (T.idx16 ? 2 : 4);
-            d.hdr = thdr[ti] ? thdr[ti] + T.row_ptr[j] / SC_WAVE : nullptr;
+            d.hdr = thdr[ti] ? thdr[ti] + T.row_ptr[j] / SC_WAVE * 2 : nullptr;  // two header words per group
             d.val = tval[P.team_of[pop]] ? tval[P.team_of[pop]] + T.row_ptr[j] * wbytes_of(T.wprec) : nullptr;
             d.group = t.group;
             d.bit = j % SC_WAVE;

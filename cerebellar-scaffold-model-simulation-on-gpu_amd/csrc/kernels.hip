@@ -213,20 +213,17 @@ __device__ __forceinline__ double slice_tail(const ScSlice &sl, gptr<uint32_t> r
     for (int k = k0; k < width; k += U) {
         uint32_t c[U], raw[U], w[U];
         if constexpr (IDX16) {
-            uint32_t pw[U / 2], hb[U / 2];
-            gptr<uint32_t> hdr = gbl(sl.hdr);
+            uint32_t pw[U / 2];
             const int last = (width - 1) >> 1;
+            // the batch's pair headers in one load (lane j: pair j), read back with readlane
+            const uint32_t hv = gbl(sl.hdr)[min((k >> 1) + min(lane, U / 2 - 1), last)];
 #pragma unroll
-            for (int j = 0; j < U / 2; ++j) {
-                const int pp = min((k >> 1) + j, last);
-                pw[j] = col[pp * SC_WAVE + lane];
-                hb[j] = hdr[pp];
-            }
+            for (int j = 0; j < U / 2; ++j) pw[j] = col[min((k >> 1) + j, last) * SC_WAVE + lane];
 #pragma unroll
             for (int j = 0; j < U; ++j)
                 raw[j] = unit ? 0u : wraw<WP>(sl.val, min(k + j, width - 1) * SC_WAVE + lane);
 #pragma unroll
-            for (int j = 0; j < U; ++j) c[j] = pair_bit(pw[j >> 1], j & 1, hb[j >> 1], zb);
+            for (int j = 0; j < U; ++j) c[j] = pair_bit(pw[j >> 1], j & 1, __builtin_amdgcn_readlane(hv, j >> 1), zb);
         } else {
 #pragma unroll
             for (int j = 0; j < U; ++j) {
@@ -267,9 +264,22 @@ __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance
     }
     constexpr int U = SC_LANE_U;
     uint32_t c[SC_MAX_SLICES][U], raw[SC_MAX_SLICES][U], sw[SC_MAX_SLICES][U];
-    uint32_t pw[SC_MAX_SLICES][U / 2], hb[SC_MAX_SLICES][U / 2];  // 16-bit form
+    uint32_t pw[SC_MAX_SLICES][U / 2];  // 16-bit form: offset pairs
     gptr<uint32_t> row[SC_MAX_SLICES];
     const int n_sl = t.n_sl;
+    uint32_t hv = 0u;  // 16-bit form: lane 2s+p holds the header of pair p of slice s
+    if constexpr (IDX16) {
+        const int hs = lane >> 1, hp = lane & 1;
+        const uint32_t *h = nullptr;
+        int w = 0;
+#pragma unroll
+        for (int q = 0; q < SC_MAX_SLICES; ++q)
+            if (hs == q && q < n_sl) {
+                h = t.sl[q].hdr;
+                w = t.sl[q].width;
+            }
+        if (2 * hp < w) hv = gbl(h)[hp];
+    }
 #pragma unroll
     for (int s = 0; s < SC_MAX_SLICES; ++s) {
         const bool on = s < n_sl;
@@ -281,11 +291,7 @@ __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance
 #pragma unroll
             for (int p = 0; p < U / 2; ++p) {
                 pw[s][p] = 0xFFFFFFFFu;
-                hb[s][p] = 0u;
-                if (2 * p < width) {
-                    pw[s][p] = col[p * SC_WAVE + lane];
-                    hb[s][p] = gbl(t.sl[s].hdr)[p];
-                }
+                if (2 * p < width) pw[s][p] = col[p * SC_WAVE + lane];
             }
         }
 #pragma unroll
@@ -302,7 +308,8 @@ __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance
     for (int s = 0; s < SC_MAX_SLICES; ++s)
 #pragma unroll
         for (int k = 0; k < U; ++k) {
-            if constexpr (IDX16) c[s][k] = pair_bit(pw[s][k >> 1], k & 1, hb[s][k >> 1], zb);
+            if constexpr (IDX16)
+                c[s][k] = pair_bit(pw[s][k >> 1], k & 1, __builtin_amdgcn_readlane(hv, 2 * s + (k >> 1)), zb);
             sw[s][k] = row[s][c[s][k] >> 5];
         }
     float dge = 0.f, dgi = 0.f;
@@ -343,12 +350,12 @@ __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance
 // additions cannot change them.
 //
 // In the 16-bit form the row is a sequence of 64-entry groups (one wavefront
-// load each; every projection segment padded to whole groups), so a group's
-// header -- projection slot and base bit -- is wave-uniform, and its entries are
-// 16-bit offsets from the base (SC_PAD16: padding, masked out).
+// load each) with two wave-uniform header words (projection slot and base bit of
+// the group's two segments); an entry is a 15-bit offset whose bit 15 selects the
+// segment.  Padding entries (SC_PAD16) read the ring row's zero word (bit zb).
 template <int WP, int U, bool IDX16>
 __device__ __forceinline__ void team_gather(const ScTeamTask &T, gptr<uint32_t> ring32, const uint32_t *rowoff,
-                                            int tid, int TT, double *acc)
+                                            uint32_t zb, int tid, int TT, double *acc)
 {
     const int len = T.len;
     const void *val = T.val;
@@ -358,25 +365,30 @@ __device__ __forceinline__ void team_gather(const ScTeamTask &T, gptr<uint32_t> 
         const int lane = tid & (SC_WAVE - 1), ng = len / SC_WAVE, gstep = TT / SC_WAVE;
         for (int b = tid; b < len; b += TT * U) {
             const int g0 = uniform((b - lane) / SC_WAVE);  // group of this wavefront's first entry
-            uint32_t o[U], h[U], raw[U], s[U];
+            uint32_t c[U], raw[U], s[U];
+            // the 2U header words of the batch in one load: lane 2u+w holds word w
+            // of group u (lanes past 2U repeat the last group), read back per group
+            // with readlane
+            const uint32_t hv = hdr[2 * min(g0 + min(lane >> 1, U - 1) * gstep, ng - 1) + (lane & 1)];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int g = min(g0 + u * gstep, ng - 1);
-                h[u] = hdr[g];
-                o[u] = col[g * SC_WAVE + lane];
+                c[u] = col[g * SC_WAVE + lane];
                 raw[u] = wraw<WP>(val, g * SC_WAVE + lane);
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const uint32_t bit = (h[u] & SC_BIT_MASK) + (o[u] == SC_PAD16 ? 0u : o[u]);
-                s[u] = ring32[rowoff[h[u] >> SC_Q_SHIFT] + (bit >> 5)];
+                const uint32_t h0 = __builtin_amdgcn_readlane(hv, 2 * u), h1 = __builtin_amdgcn_readlane(hv, 2 * u + 1);
+                const uint32_t o = c[u];
+                const uint32_t hs = (o & 0x8000u) ? h1 : h0;
+                const uint32_t pad = 0u - (uint32_t)(o == SC_PAD16);  // all ones for padding
+                c[u] = ((hs + (o & 0x7FFFu)) & ~pad) | (((h0 & ~SC_BIT_MASK) | zb) & pad);
+                s[u] = ring32[rowoff[c[u] >> SC_Q_SHIFT] + ((c[u] & SC_BIT_MASK) >> 5)];
             }
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const uint32_t bit = (h[u] & SC_BIT_MASK) + o[u];
-                if (g0 + u * gstep < ng && o[u] != SC_PAD16 && ((s[u] >> (bit & 31)) & 1u))
-                    atomicAdd(&acc[h[u] >> SC_Q_SHIFT], (double)wdec<WP>(raw[u]));
-            }
+            for (int u = 0; u < U; ++u)
+                if (g0 + u * gstep < ng && ((s[u] >> (c[u] & 31)) & 1u))
+                    atomicAdd(&acc[c[u] >> SC_Q_SHIFT], (double)wdec<WP>(raw[u]));
         }
     } else {
         gptr<uint32_t> col = gbl(static_cast<const uint32_t *>(T.col));
@@ -501,19 +513,20 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
     if (active) {
         gptr<uint32_t> ring32 = gbl(reinterpret_cast<const uint32_t *>(I.ring));
         const int tid = tw * SC_WAVE + lane, TT = t * SC_WAVE;
+        const uint32_t zb = (uint32_t)(I.Wr - 1) * SC_WAVE;
         if (IDX16 && T->hdr) {
             switch (T->wprec) {
-            case SC_W_F32: team_gather<SC_W_F32, 8, true>(*T, ring32, rowoff[team], tid, TT, acc[team]); break;
-            case SC_W_BF16: team_gather<SC_W_BF16, 8, true>(*T, ring32, rowoff[team], tid, TT, acc[team]); break;
-            case SC_W_FP8: team_gather<SC_W_FP8, 8, true>(*T, ring32, rowoff[team], tid, TT, acc[team]); break;
-            default: team_gather<SC_W_UNIT, 8, true>(*T, ring32, rowoff[team], tid, TT, acc[team]); break;
+            case SC_W_F32: team_gather<SC_W_F32, 8, true>(*T, ring32, rowoff[team], zb, tid, TT, acc[team]); break;
+            case SC_W_BF16: team_gather<SC_W_BF16, 8, true>(*T, ring32, rowoff[team], zb, tid, TT, acc[team]); break;
+            case SC_W_FP8: team_gather<SC_W_FP8, 8, true>(*T, ring32, rowoff[team], zb, tid, TT, acc[team]); break;
+            default: team_gather<SC_W_UNIT, 8, true>(*T, ring32, rowoff[team], zb, tid, TT, acc[team]); break;
             }
         } else {
             switch (T->wprec) {
-            case SC_W_F32: team_gather<SC_W_F32, 8, false>(*T, ring32, rowoff[team], tid, TT, acc[team]); break;
-            case SC_W_BF16: team_gather<SC_W_BF16, 8, false>(*T, ring32, rowoff[team], tid, TT, acc[team]); break;
-            case SC_W_FP8: team_gather<SC_W_FP8, 8, false>(*T, ring32, rowoff[team], tid, TT, acc[team]); break;
-            default: team_gather<SC_W_UNIT, 8, false>(*T, ring32, rowoff[team], tid, TT, acc[team]); break;
+            case SC_W_F32: team_gather<SC_W_F32, 8, false>(*T, ring32, rowoff[team], zb, tid, TT, acc[team]); break;
+            case SC_W_BF16: team_gather<SC_W_BF16, 8, false>(*T, ring32, rowoff[team], zb, tid, TT, acc[team]); break;
+            case SC_W_FP8: team_gather<SC_W_FP8, 8, false>(*T, ring32, rowoff[team], zb, tid, TT, acc[team]); break;
+            default: team_gather<SC_W_UNIT, 8, false>(*T, ring32, rowoff[team], zb, tid, TT, acc[team]); break;
             }
         }
     }

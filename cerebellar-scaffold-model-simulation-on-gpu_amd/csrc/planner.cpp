@@ -421,55 +421,55 @@ int plan_build(scaffold_env *env, Plan &P)
             }
         }
         if (want16) {
-            // 16-bit form: each projection segment padded to whole groups of 64
-            // entries; a group must span fewer than 65,535 device bits
+            // 16-bit form: each row is a sequence of 64-entry groups (one wavefront
+            // load each).  A group holds at most two segments -- runs of entries of
+            // one projection, ascending, spanning fewer than 32,767 device bits --
+            // and two header words, one per segment: q << 29 | base bit.  An entry
+            // is its 15-bit offset from its segment's base, bit 15 set in the second
+            // segment; 0xFFFF is padding (multiplier 0).
             std::vector<int64_t> rp(d.num + 1, 0);
-            std::vector<int32_t> se((size_t)d.num * d.n_in, 0);
+            std::vector<uint16_t> c16;
+            std::vector<uint32_t> h;
+            std::vector<uint8_t> v16;
+            c16.reserve(tot + tot / 8);
+            std::vector<int64_t> perm;
             for (int j = 0; j < d.num; ++j) {
-                int64_t len = 0, prev = 0;
-                for (int q = 0; q < d.n_in; ++q) {
-                    const int64_t end = T.seg_end[(size_t)j * d.n_in + q];
-                    len += (end - prev + SC_WAVE - 1) / SC_WAVE * SC_WAVE;
-                    prev = end;
-                    se[(size_t)j * d.n_in + q] = (int32_t)len;
-                }
-                rp[j + 1] = rp[j] + len;
-            }
-            const int64_t tot16 = rp[d.num];
-            std::vector<uint16_t> c16(tot16, (uint16_t)SC_PAD16);
-            std::vector<uint32_t> h(tot16 / SC_WAVE, 0);
-            std::vector<uint8_t> v16(wb ? tot16 * wb : 0, 0);
-            bool ok = true;
-            for (int j = 0; j < d.num && ok; ++j)
-                for (int q = 0; q < d.n_in && ok; ++q) {
-                    const int64_t s0 = T.row_ptr[j] + (q ? T.seg_end[(size_t)j * d.n_in + q - 1] : 0);
-                    const int64_t s1 = T.row_ptr[j] + T.seg_end[(size_t)j * d.n_in + q];
-                    const int64_t d0 = rp[j] + (q ? se[(size_t)j * d.n_in + q - 1] : 0);
-                    for (int64_t g = 0; g * SC_WAVE < s1 - s0; ++g) {
-                        const int64_t a = s0 + g * SC_WAVE, b = std::min(s1, a + SC_WAVE);
-                        uint32_t lo = UINT32_MAX, hi = 0;
-                        for (int64_t e = a; e < b; ++e) {
-                            lo = std::min(lo, T.col[e] & SC_BIT_MASK);
-                            hi = std::max(hi, T.col[e] & SC_BIT_MASK);
-                        }
-                        if (hi - lo >= SC_PAD16) { ok = false; break; }
-                        const int64_t o = d0 + g * SC_WAVE;
-                        h[o / SC_WAVE] = ((uint32_t)q << SC_Q_SHIFT) | lo;
-                        for (int64_t e = a; e < b; ++e) {
-                            c16[o + (e - a)] = (uint16_t)((T.col[e] & SC_BIT_MASK) - lo);
-                            if (wb) memcpy(&v16[(o + (e - a)) * wb], &T.val[e * wb], wb);
+                // entries of each projection segment in ascending bit order (the
+                // row sums are exact, so the order cannot change them)
+                perm.resize(T.row_ptr[j + 1] - T.row_ptr[j]);
+                for (size_t i = 0; i < perm.size(); ++i) perm[i] = T.row_ptr[j] + (int64_t)i;
+                std::stable_sort(perm.begin(), perm.end(), [&](int64_t x, int64_t y) { return T.col[x] < T.col[y]; });
+                size_t e = 0;
+                while (e < perm.size()) {
+                    const int64_t g0 = (int64_t)c16.size();
+                    c16.resize(g0 + SC_WAVE, (uint16_t)SC_PAD16);
+                    if (wb) v16.resize((g0 + SC_WAVE) * wb, 0);
+                    h.resize(h.size() + 2, 0);
+                    uint32_t *hg = &h[h.size() - 2];
+                    int n = 0;
+                    for (int seg = 0; seg < 2 && e < perm.size() && n < SC_WAVE; ++seg) {
+                        const uint32_t first = T.col[perm[e]];
+                        const uint32_t q = first >> SC_Q_SHIFT, lo = first & SC_BIT_MASK;
+                        hg[seg] = first;
+                        if (seg == 0) hg[1] = first;
+                        while (e < perm.size() && n < SC_WAVE) {
+                            const uint32_t c = T.col[perm[e]];
+                            if ((c >> SC_Q_SHIFT) != q || (c & SC_BIT_MASK) - lo >= SC_OFF15_PAD) break;
+                            c16[g0 + n] = (uint16_t)(((c & SC_BIT_MASK) - lo) | (seg ? 0x8000u : 0u));
+                            if (wb) memcpy(&v16[(g0 + n) * wb], &T.val[perm[e] * wb], wb);
+                            ++n;
+                            ++e;
                         }
                     }
                 }
-            if (ok) {
-                T.idx16 = true;
-                T.row_ptr = std::move(rp);
-                T.seg_end = std::move(se);
-                T.col16 = std::move(c16);
-                T.hdr = std::move(h);
-                T.val = std::move(v16);
-                std::vector<uint32_t>().swap(T.col);
+                rp[j + 1] = (int64_t)c16.size();
             }
+            T.idx16 = true;
+            T.row_ptr = std::move(rp);
+            T.col16 = std::move(c16);
+            T.hdr = std::move(h);
+            T.val = std::move(v16);
+            std::vector<uint32_t>().swap(T.col);
         }
         // team width: about lane_synapses synapses per lane for the mean row
         const double per_lane = (double)tot / std::max(1, d.num) / SC_WAVE / std::max(1, opt.lane_synapses);
