@@ -171,15 +171,17 @@ def batched(pkg, n_inst, steps, warmup, threads):
 
 def scaled(pkg, preset, steps, precisions):
     """A tiled scaffold (preset 2: 16x in area, 1.55M cells, 50.7M synapses;
-    preset 4: 100x, 9.67M cells, 326M synapses) with the given multiplier storage:
-    step time, real-time factor, synaptic events/s and the step kernel's
-    algorithmic bandwidth."""
+    preset 4: 100x, 9.67M cells, 326M synapses) with the given multiplier storage
+    (an entry is a precision, or (precision, index bits)): step time, real-time
+    factor, synaptic events/s and the step kernel's algorithmic bandwidth."""
     out = {}
     names = {pkg.WEIGHTS_F32: "fp32", pkg.WEIGHTS_BF16: "bf16", pkg.WEIGHTS_FP8: "fp8"}
-    for prec in precisions:
-        name = names[prec]
+    for entry in precisions:
+        prec, bits = entry if isinstance(entry, tuple) else (entry, 32)
+        name = names[prec] + ("_idx16" if bits == 16 else "")
         env = make_env(pkg, preset)
         env.set_option("weight_precision", prec)
+        env.set_option("index_bits", bits)
         env.initialize(50, 25)
         info = env.info()
         advance(env, 0, 50)
@@ -193,7 +195,8 @@ def scaled(pkg, preset, steps, precisions):
         out[name] = {"ms_per_step": wall * 1e3 / steps, "real_time_factor": steps * 1e-3 / wall,
                      "synaptic_events_per_s": (ev1 - ev0) / wall, "kernel_mean_ms": kmean,
                      "algorithmic_bytes_per_step": algo, "achieved_GBs": algo / (kmean * 1e-3) / 1e9,
-                     "frac": algo / (kmean * 1e-3) / 1e9 / HBM_PEAK_GBS}
+                     "frac": algo / (kmean * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "synapses_16bit": info.synapses_16bit}
         out["neurons"], out["synapses"] = info.total_neurons, info.total_synapses
         env.finalize()
     out["steps"] = steps
@@ -300,7 +303,8 @@ def main():
     if not args.no_16x:
         line["scaffold16x"] = scaled(pkg, 2, args.steps_16x, (pkg.WEIGHTS_F32, pkg.WEIGHTS_BF16, pkg.WEIGHTS_FP8))
     if not args.no_100x:
-        line["scaffold100x"] = scaled(pkg, 4, args.steps_100x, (pkg.WEIGHTS_FP8, pkg.WEIGHTS_F32))
+        line["scaffold100x"] = scaled(pkg, 4, args.steps_100x,
+                                      ((pkg.WEIGHTS_FP8, 16), pkg.WEIGHTS_FP8, (pkg.WEIGHTS_F32, 16), pkg.WEIGHTS_F32))
     print(json.dumps(line))
 
 

@@ -134,6 +134,7 @@ def load_library(path=None):
         "scaffold_write_dat": (ctypes.c_int, [vp, ctypes.c_int, cs]),
         "scaffold_get_info": (ctypes.c_int, [vp, P(Info)]),
         "scaffold_get_population": (ctypes.c_int, [vp, ctypes.c_int, P(PopulationInfo)]),
+        "scaffold_get_positions": (ctypes.c_int, [vp, ctypes.c_int, vp, vp]),
         "scaffold_get_projection": (ctypes.c_int, [vp, ctypes.c_int, P(ProjectionInfo)]),
         "scaffold_get_projection_csr": (ctypes.c_int, [vp, ctypes.c_int, P(i64), P(i32), P(f32)]),
         "scaffold_get_stimulus": (ctypes.c_int, [vp, ctypes.c_int, P(StimInfo), P(i64), P(ctypes.c_uint32)]),
@@ -330,6 +331,13 @@ class NetworkEnvironment:
         p = PopulationInfo()
         _check(self._lib.scaffold_get_population(self._h, index, ctypes.byref(p)))
         return p
+
+    def positions(self, index):
+        """Cell positions (x, z) in um of a population of a synthetic network."""
+        n = self.population(index).num
+        x, z = np.zeros(n, np.float32), np.zeros(n, np.float32)
+        _check(self._lib.scaffold_get_positions(self._h, index, x.ctypes.data, z.ctypes.data))
+        return x, z
 
     def projection(self, index):
         p = ProjectionInfo()

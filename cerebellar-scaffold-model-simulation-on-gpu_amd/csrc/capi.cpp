@@ -652,6 +652,19 @@ int scaffold_get_population(scaffold_env *env, int index, scaffold_population_in
     return SCAFFOLD_OK;
 }
 
+int scaffold_get_positions(scaffold_env *env, int index, float *x, float *z)
+{
+    clear_error();
+    int rc = check_env(env);
+    if (rc) return rc;
+    if (index < 0 || index >= env->n_types) return fail(SCAFFOLD_ERR_INVALID_ARG, "bad index");
+    const Population &p = env->pops[index];
+    if (!p.set || (int)p.pos_x.size() != p.num) return fail(SCAFFOLD_ERR_STATE, "population %d has no positions", index);
+    if (x) memcpy(x, p.pos_x.data(), sizeof(float) * p.pos_x.size());
+    if (z) memcpy(z, p.pos_z.data(), sizeof(float) * p.pos_z.size());
+    return SCAFFOLD_OK;
+}
+
 int scaffold_get_projection(scaffold_env *env, int index, scaffold_projection_info *out)
 {
     clear_error();

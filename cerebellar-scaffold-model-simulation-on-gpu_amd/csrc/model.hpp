@@ -24,6 +24,7 @@ struct Population {
     int type = -1, duplicate = 0, num = 0, kind = SCAFFOLD_NORMAL, c_num = 0;
     float Cm = 0, El = 0, dt_ref = 0, Ie = 0, Vr = 0, Vth = 0, gL = 0;
     float w[2] = {0, 0}, E[2] = {0, 0}, g_bar[2] = {0, 0}, tau[2] = {1, 1};
+    std::vector<float> pos_x, pos_z;  // synthetic networks: cell positions (um)
 };
 
 struct Projection {

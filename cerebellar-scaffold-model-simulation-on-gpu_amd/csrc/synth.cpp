@@ -332,6 +332,11 @@ int synth_describe(scaffold_env *env, const scaffold_synth_config *cfg)
     const uint64_t S = cfg->seed;
     Cells cell[8];
     for (int i = 0; i < 8; ++i) cell[i] = place(counts[i], X, Z, hash3(S, 0xCE11, (uint64_t)i));
+    for (int i = 0; i < 7; ++i) {
+        Population &P = env->pops[idx[i]];
+        P.pos_x.assign(cell[i].x.begin(), cell[i].x.end());
+        P.pos_z.assign(cell[i].z.begin(), cell[i].z.end());
+    }
     const double dens_grc = counts[GRC] / (X * Z);
     Grid g_grc(cell[GRC], 50.0, X, Z);
     auto grid_for = [&](int p, double per_cell) {

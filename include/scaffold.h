@@ -263,6 +263,11 @@ typedef struct {
     int32_t  ref_step;
 } scaffold_population_info;
 int scaffold_get_population(scaffold_env *env, int index, scaffold_population_info *out);
+/* Cell positions (um) of population `index` of a synthetic network (x, z: num
+ * floats each; either may be NULL).  SCAFFOLD_ERR_STATE for a population without
+ * positions (networks described from .dat files).  No counterpart in the
+ * reference's simulator: its cell positions stay in the scaffold's HDF5 file. */
+int scaffold_get_positions(scaffold_env *env, int index, float *x, float *z);
 
 typedef struct {
     int32_t type, pre, post, pre_num, post_num, delay;

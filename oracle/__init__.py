@@ -73,6 +73,10 @@ def lib():
         L.orc_run.restype = None
         L.orc_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_set_threads.restype = None
+        L.orc_set_active.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_set_active.restype = None
+        L.orc_step_driven.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        L.orc_step_driven.restype = None
         L.orc_set_freq.argtypes = [ctypes.c_void_p, P(ctypes.c_float)]
         L.orc_set_freq.restype = None
         L.orc_get_state.argtypes = [ctypes.c_void_p, P(ctypes.c_float), P(ctypes.c_float), P(i32)]
@@ -239,6 +243,19 @@ class OracleSim:
 
     def set_threads(self, n):
         lib().orc_set_threads(self._h, n)
+
+    def set_active(self, active):
+        """Driven mode: simulate only the neurons with active[id] != 0 (None: all)."""
+        self._active = None if active is None else np.ascontiguousarray(active, np.uint8)
+        if self._active is not None:
+            assert self._active.shape == (self.total,)
+        lib().orc_set_active(self._h, None if self._active is None else self._active.ctypes.data)
+
+    def step_driven(self, n, ext):
+        """One step in driven mode; ext: spikes of step n of every neuron (bytes)."""
+        e = np.ascontiguousarray(ext, np.uint8)
+        assert e.shape == (self.total,)
+        lib().orc_step_driven(self._h, n, e.ctypes.data)
 
     def set_freq(self, freqs):
         f = np.ascontiguousarray(freqs, dtype=np.float32)

@@ -104,6 +104,7 @@ struct orc_sim {
     int32_t  *refr;
     uint8_t  *ring;            /* R x total bytes (GPUScaffoldSimulation.cu:361) */
     int       threads;         /* OpenMP threads of orc_step (1: serial)  */
+    uint8_t  *active;          /* driven mode: 1 = simulated here; NULL = all */
 };
 
 /* Threads used by orc_step for propagation and the neuron update (both are
@@ -167,6 +168,7 @@ void orc_destroy(orc_sim *s)
     free(s->base); free(s->type_of); free(s->refr); free(s->ring);
     free(s->u); free(s->g); free(s->dge); free(s->dgi);
     free(s->u64); free(s->g64); free(s->dge64); free(s->dgi64);
+    free(s->active);
     free(s);
 }
 
@@ -355,8 +357,28 @@ int orc_lif_step_f64(const orc_pop *p, double *u, double *g, int32_t *refr,
  *   3. LIF of every non-INPUT population, writing ring row n mod R and zeroing
  *      dg (:746-751; OUTPUT populations run the same rule on the host,
  *      GPUScaffoldSimulation.cu:41-46, Simulation_header.cu:385-465). */
-void orc_step(orc_sim *s, int n)
+static void step_impl(orc_sim *s, int n, const uint8_t *ext);
+
+void orc_step(orc_sim *s, int n) { step_impl(s, n, NULL); }
+
+/* Driven mode (a test device, not a reference function): only the neurons with
+ * active[id] = 1 are simulated; the spikes of every other neuron at step n are
+ * taken from ext[id] (orc_step_driven).  Propagation and the neuron update skip
+ * the driven neurons, whose state stays as it is. */
+void orc_set_active(orc_sim *s, const uint8_t *active)
 {
+    free(s->active);
+    s->active = NULL;
+    if (!active) return;
+    s->active = (uint8_t *)malloc((size_t)(s->total > 0 ? s->total : 1));
+    memcpy(s->active, active, (size_t)s->total);
+}
+
+void orc_step_driven(orc_sim *s, int n, const uint8_t *ext) { step_impl(s, n, ext); }
+
+static void step_impl(orc_sim *s, int n, const uint8_t *ext)
+{
+    const uint8_t *act = ext ? s->active : NULL;
     const int row = orc_row(s, n);
     uint8_t *out = s->ring + (size_t)row * (size_t)s->total;
     /* 1. stimulus */
@@ -385,6 +407,7 @@ void orc_step(orc_sim *s, int n)
         const float W = exc ? pr->w_bar : pr->w_bar * (-1.f);
 #pragma omp parallel for schedule(static) num_threads(s->threads) if (s->threads > 1)
         for (int j = 0; j < post_num; ++j) {
+            if (act && !act[post_base + j]) continue;
             if (s->precision == 0) {
                 float Sf;
                 if (s->sum_mode == 0) {
@@ -416,6 +439,7 @@ void orc_step(orc_sim *s, int n)
 #pragma omp parallel for schedule(static) num_threads(s->threads) if (s->threads > 1)
         for (int i = s->base[p]; i < s->base[p] + pp->num; ++i) {
             int spk;
+            if (act && !act[i]) continue;
             if (s->precision == 0) {
                 spk = orc_lif_step_f32(pp, &s->u[i], &s->g[2 * (size_t)i], &s->refr[i],
                                        s->dge[i], s->dgi[i]);
@@ -427,6 +451,12 @@ void orc_step(orc_sim *s, int n)
             }
             out[i] = (uint8_t)(spk ? 1 : 0);
         }
+    }
+    /* driven neurons (input populations included) take the supplied spikes */
+    if (ext) {
+#pragma omp parallel for schedule(static) num_threads(s->threads) if (s->threads > 1)
+        for (int i = 0; i < s->total; ++i)
+            if (!act || !act[i]) out[i] = ext[i];
     }
 }
 

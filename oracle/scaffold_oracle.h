@@ -86,6 +86,13 @@ void orc_step(orc_sim *s, int n);
 /* Steps n0 .. n0+k-1. */
 void orc_run(orc_sim *s, int n0, int k);
 void orc_set_threads(orc_sim *s, int threads);
+/* Driven mode (a test device with no reference counterpart): only neurons with
+ * active[id] = 1 (total_neurons bytes, copied; NULL: all) are simulated;
+ * orc_step_driven takes the spikes of step n of every other neuron from ext
+ * (total_neurons bytes).  Used to run a sub-network in FP64 inside a larger
+ * network simulated elsewhere. */
+void orc_set_active(orc_sim *s, const uint8_t *active);
+void orc_step_driven(orc_sim *s, int n, const uint8_t *ext);
 void orc_set_freq(orc_sim *s, const float *freq_list);
 
 /* State access (FP32 oracle: float arrays; FP64 oracle: the *_f64 calls). */

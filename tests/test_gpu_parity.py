@@ -183,6 +183,75 @@ def test_rates_vs_fp64_oracle(pkg, orc, preset, precision, name):
         assert diverged is None, report
 
 
+def test_subtile_rates_vs_fp64_oracle_100x(pkg, orc):
+    """The scaffold tiled 100x in area (9.67M cells, 326M synapses) on the GPU with
+    FP8-e4m3 multipliers and 16-bit column offsets, 1 s.  A 1x sub-tile -- the
+    400 x 400 um square at the centre of the sheet, inside the burst patch -- is
+    re-simulated in the FP64 oracle with the unquantized multipliers, driven by the
+    GPU's spikes of every neuron outside it.  Per-population firing rates of the
+    sub-tile's neurons agree within 1 % (populations with at least 1000 spikes in
+    the sub-tile); the first step at which the sub-tile's raster differs is
+    reported."""
+    import json
+    import os
+    env = new_env(pkg, 4)
+    env.set_option("weight_precision", pkg.WEIGHTS_FP8)
+    env.set_option("index_bits", 16)
+    env.initialize(50, 25)
+    info = env.info()
+    assert info.synapses_16bit == info.total_synapses
+    pops = [env.population(i) for i in range(info.n_populations)]
+    desc = new_env(pkg, 4)  # same network with the generated (unquantized) multipliers
+    ref = orc.OracleSim.from_env(desc, precision=1, ring_rows=100)
+    desc.finalize()
+    ref.set_threads(min(16, os.cpu_count() or 1))
+    ref.initialize()
+    side = 400.0 * np.sqrt(100.0)
+    active = np.zeros(info.total_neurons, np.uint8)
+    ids = {}
+    for i, p in enumerate(pops):
+        x, z = env.positions(i)
+        inside = (np.abs(x - side / 2) < 200.0) & (np.abs(z - side / 2) < 200.0)
+        ids[i] = p.base_id + np.nonzero(inside)[0]
+        if p.device_type != pkg.INPUT:
+            active[ids[i]] = 1
+    ref.set_active(active)
+    on = active.astype(bool)
+    gpu_n = np.zeros(len(pops), np.int64)
+    orc_n = np.zeros(len(pops), np.int64)
+    diverged = None
+    for s0 in range(0, FP64_STEPS, 25):
+        env.loop_n_steps(s0, 25, 0, want_output=False)
+        for s in range(s0, s0 + 25):
+            row = env.spike_row(s)
+            ref.step_driven(s, row)
+            mine = ref.spike_row(s)
+            for i in ids:
+                gpu_n[i] += int(row[ids[i]].sum())
+                orc_n[i] += int(mine[ids[i]].sum())
+            if diverged is None and not np.array_equal(row[on], mine[on]):
+                diverged = s
+    env.finalize()
+    report = {"network": "scaffold 100x, fp8, 16-bit offsets", "subtile_neurons": {pops[i].name.decode(): int(ids[i].size)
+                                                                                 for i in ids},
+              "steps": FP64_STEPS, "first_divergent_step": diverged, "rates_hz": {}, "rate_err_pct": {}}
+    worst = 0.0
+    for i in ids:
+        nm, n = pops[i].name.decode(), max(int(ids[i].size), 1)
+        report["rates_hz"][nm] = [gpu_n[i] / n, orc_n[i] / n]
+        err = abs(int(gpu_n[i]) - int(orc_n[i])) / max(int(orc_n[i]), 1) * 100
+        report["rate_err_pct"][nm] = err
+        if orc_n[i] >= 1000 and pops[i].device_type != pkg.INPUT:
+            worst = max(worst, err)
+    out = os.environ.get("SCAFFOLD_REPORT_DIR")
+    if out:
+        with open(os.path.join(out, "fp64_subtile_p4_fp8_idx16.json"), "w") as fh:
+            json.dump(report, fh, indent=1)
+    print(json.dumps(report))
+    assert sum(ids[i].size for i in ids) > 50000
+    assert worst <= 1.0, report
+
+
 @pytest.mark.parametrize("precision", [1, 2])
 def test_quantized_weights_bit_exact(pkg, orc, precision):
     """BF16 / FP8-e4m3 multipliers: the oracle runs on the dequantized values the
