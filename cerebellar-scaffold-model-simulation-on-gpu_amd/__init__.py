@@ -76,7 +76,8 @@ class ProjectionInfo(ctypes.Structure):
     _fields_ = [("type", ctypes.c_int32), ("pre", ctypes.c_int32), ("post", ctypes.c_int32),
                 ("pre_num", ctypes.c_int32), ("post_num", ctypes.c_int32), ("delay", ctypes.c_int32),
                 ("initial_weight", ctypes.c_float), ("w_bar", ctypes.c_float), ("nnz", ctypes.c_int64),
-                ("max_conv", ctypes.c_int32), ("lane_mode", ctypes.c_int32), ("fp8_scale", ctypes.c_float)]
+                ("max_conv", ctypes.c_int32), ("lane_mode", ctypes.c_int32), ("fp8_scale", ctypes.c_float),
+                ("sum_scale", ctypes.c_float)]
 
 
 class TuneResult(ctypes.Structure):

@@ -684,10 +684,12 @@ int scaffold_get_projection(scaffold_env *env, int index, scaffold_projection_in
     i.nnz = (int64_t)p.col.size();
     i.max_conv = p.max_conv;
     i.fp8_scale = 1.f;
+    i.sum_scale = 1.f;
     if (env->initialized && p.set) {
         const ScProj &sp = env->plan.projs[index];
         i.lane_mode = sp.mode == SC_MODE_LANE;
         i.fp8_scale = sp.scale;
+        i.sum_scale = sp.sum_scale;
     }
     *out = i;
     return SCAFFOLD_OK;

@@ -65,6 +65,7 @@ struct ScPop {
 struct ScProj {            // host-side summary of one projection
     float   W;             // |w_bar|
     float   scale;         // FP8 multiplier scale (a power of two), else 1
+    float   sum_scale;     // BF16 / FP8: generated over stored multiplier sum, else 1
     int32_t delay;
     int32_t exc;
     int32_t mode;          // SC_MODE_LANE / SC_MODE_ROW, -1: not registered

@@ -252,7 +252,7 @@ int engine_create(Engine **out, Plan &P)
                 d.in[q].delay = sp.delay;
                 d.in[q].exc = sp.exc;
                 d.in[q].W = sp.W;
-                d.in[q].scale = sp.scale;
+                d.in[q].scale = sp.scale * sp.sum_scale;  // exact: scale is a power of two
             }
             e->h_teams.push_back(d);
             e->h_team_t.push_back(T.team_waves);
@@ -280,7 +280,7 @@ int engine_create(Engine **out, Plan &P)
                 s.wprec = sp.wprec;
                 s.exc = sp.exc;
                 s.W = sp.W;
-                s.scale = sp.scale;
+                s.scale = sp.scale * sp.sum_scale;  // exact: scale is a power of two
                 if (s.val) l.wp = sp.wprec;
             }
             e->h_lanes.push_back(l);

@@ -248,6 +248,7 @@ int plan_build(scaffold_env *env, Plan &P)
         const Projection &src = env->projs[pi];
         ScProj &sp = P.projs[pi];
         sp.scale = 1.f;
+        sp.sum_scale = 1.f;
         if (!src.set) { sp.mode = -1; continue; }
         const ScPop &pre = P.pops[src.pre];
         sp.delay = src.delay();
@@ -265,6 +266,8 @@ int plan_build(scaffold_env *env, Plan &P)
         if (sp.wprec != SC_W_UNIT) {
             ev.resize(nnz);
             for (int64_t e = 0; e < nnz; ++e) ev[e] = src.val.empty() ? 1.f : src.val[e];
+            double gen_sum = 0.0, q_sum = 0.0;
+            for (float x : ev) gen_sum += x;
             if (sp.wprec == SC_W_FP8) {
                 float mx = 0.f;
                 for (float x : ev) mx = std::max(mx, std::fabs(x));
@@ -272,6 +275,13 @@ int plan_build(scaffold_env *env, Plan &P)
                 for (auto &x : ev) x = e4m3_to_f32(f32_to_e4m3(x / sp.scale)) * sp.scale;
             } else if (sp.wprec == SC_W_BF16) {
                 for (auto &x : ev) x = bf16_to_f32(f32_to_bf16(x));
+            }
+            // round to nearest biases the mean of a multiplier distribution that
+            // straddles a binade (U(0.8, 1.2) loses 0.3 % in FP8); the row sums of
+            // a quantized projection are scaled back to the generated mean
+            if (sp.wprec == SC_W_FP8 || sp.wprec == SC_W_BF16) {
+                for (float x : ev) q_sum += x;
+                if (q_sum > 0.0 && gen_sum > 0.0) sp.sum_scale = (float)(gen_sum / q_sum);
             }
             if (!row_sums_exact(ev, src.row_ptr))
                 return fail(SCAFFOLD_ERR_WEIGHT_RANGE,

@@ -276,6 +276,10 @@ typedef struct {
     int32_t max_conv;          /* largest row (the reference's ELL width)      */
     int32_t lane_mode;         /* 1: gathered one lane per post neuron         */
     float   fp8_scale;         /* power-of-two scale of FP8 multipliers        */
+    float   sum_scale;         /* BF16 / FP8: each row sum of the stored
+                                  multipliers is multiplied by this (the
+                                  generated over the stored multiplier sum of
+                                  the projection), else 1                      */
 } scaffold_projection_info;
 int scaffold_get_projection(scaffold_env *env, int index, scaffold_projection_info *out);
 /* Copy the CSR of a projection (row_ptr: post_num+1, col, val: nnz).  val is the

@@ -27,7 +27,7 @@ class OrcPop(ctypes.Structure):
 class OrcProj(ctypes.Structure):
     _fields_ = [("pre", ctypes.c_int), ("post", ctypes.c_int), ("w_bar", ctypes.c_float), ("delay", ctypes.c_int),
                 ("row_ptr", ctypes.POINTER(ctypes.c_int64)), ("col", ctypes.POINTER(ctypes.c_int32)),
-                ("val", ctypes.POINTER(ctypes.c_float))]
+                ("val", ctypes.POINTER(ctypes.c_float)), ("sum_scale", ctypes.c_float)]
 
 
 class OrcStim(ctypes.Structure):
@@ -207,6 +207,7 @@ class OracleSim:
             keep += [rp, col, val]
             o = OrcProj()
             o.pre, o.post, o.w_bar, o.delay = pi.pre, pi.post, pi.initial_weight, pi.delay
+            o.sum_scale = pi.sum_scale
             o.row_ptr = rp.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
             o.col = col.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
             o.val = None if val is None else val.ctypes.data_as(ctypes.POINTER(ctypes.c_float))

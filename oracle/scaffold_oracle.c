@@ -405,6 +405,7 @@ static void step_impl(orc_sim *s, int n, const uint8_t *ext)
         const int post_num = s->pops[pr->post].num;
         const int exc = pr->w_bar > 0;
         const float W = exc ? pr->w_bar : pr->w_bar * (-1.f);
+        const double ss = pr->sum_scale == 0.f ? 1.0 : (double)pr->sum_scale;
 #pragma omp parallel for schedule(static) num_threads(s->threads) if (s->threads > 1)
         for (int j = 0; j < post_num; ++j) {
             if (act && !act[post_base + j]) continue;
@@ -414,11 +415,12 @@ static void step_impl(orc_sim *s, int n, const uint8_t *ext)
                     double S = 0.0;
                     for (int64_t e = pr->row_ptr[j]; e < pr->row_ptr[j + 1]; ++e)
                         if (in[pr->col[e]]) S += pr->val ? (double)pr->val[e] : 1.0;
-                    Sf = (float)S;
+                    Sf = (float)(S * ss);
                 } else {
                     Sf = 0.f;
                     for (int64_t e = pr->row_ptr[j]; e < pr->row_ptr[j + 1]; ++e)
                         Sf = Sf + ((in[pr->col[e]]) ? (pr->val ? pr->val[e] : 1.f) : 0.f);
+                    if (ss != 1.0) Sf = (float)((double)Sf * ss);
                 }
                 float add = Sf * W;
                 float *buf = exc ? s->dge : s->dgi;
@@ -428,7 +430,7 @@ static void step_impl(orc_sim *s, int n, const uint8_t *ext)
                 for (int64_t e = pr->row_ptr[j]; e < pr->row_ptr[j + 1]; ++e)
                     if (in[pr->col[e]]) S += pr->val ? (double)pr->val[e] : 1.0;
                 double *buf = exc ? s->dge64 : s->dgi64;
-                buf[post_base + j] = buf[post_base + j] + S * (double)W;
+                buf[post_base + j] = buf[post_base + j] + S * ss * (double)W;
             }
         }
     }

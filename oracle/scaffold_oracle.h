@@ -49,6 +49,9 @@ typedef struct {
     const int64_t *row_ptr;   /* post population size + 1                      */
     const int32_t *col;       /* population-local pre ids                      */
     const float   *val;       /* per-synapse multiplier, NULL = 1.0            */
+    float sum_scale;          /* factor on each row sum (0 is read as 1); the
+                                 engine's correction for quantized multipliers,
+                                 1 in the reference                            */
 } orc_proj;
 
 typedef struct {

@@ -418,3 +418,40 @@ def test_driven_mode_takes_supplied_spikes(orc):
     _, g0, _ = base.state()
     _, g1, _ = drv.state()
     assert not np.array_equal(g0[140:180], g1[140:180])
+
+
+def test_row_sum_scale_equals_scaled_multipliers(orc):
+    # a row-sum factor of 2 is the same as doubling every multiplier (both exact),
+    # so the two networks step identically; factor 0 is read as 1
+    import ctypes
+    _, csr, pops = build_net(orc, 8, True, seed=11)
+
+    def make(mult, ss):
+        projs, keep = [], []
+        for pre, w, d, rp, col, val in csr:
+            v = (val * mult).astype(np.float32)
+            o = orc.OrcProj()
+            o.pre, o.post, o.w_bar, o.delay, o.sum_scale = pre, 2, w, d, ss
+            o.row_ptr = rp.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+            o.col = col.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+            o.val = v.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+            projs.append(o)
+            keep += [rp, col, v]
+        st = orc.make_stim(0, np.arange(40), seed=77, poisson_hz=300.0)
+        sim = orc.OracleSim(pops, projs, [st], 8, 12345, keep=keep)
+        sim.initialize()
+        return sim
+
+    doubled, scaled, plain, zero = make(2.0, 1.0), make(1.0, 2.0), make(1.0, 1.0), make(1.0, 0.0)
+    differ = 0
+    for n in range(60):
+        for s in (doubled, scaled, plain, zero):
+            s.step(n)
+        assert (doubled.spike_row(n) == scaled.spike_row(n)).all()
+        assert (plain.spike_row(n) == zero.spike_row(n)).all()
+        differ += int((doubled.spike_row(n) != plain.spike_row(n)).any())
+    for a, b in zip(doubled.state(), scaled.state()):
+        assert (np.asarray(a).view(np.uint32) == np.asarray(b).view(np.uint32)).all()
+    _, g2, _ = doubled.state()
+    _, g1, _ = plain.state()
+    assert differ > 0 or not np.array_equal(g1, g2)
