@@ -60,7 +60,8 @@ class Info(ctypes.Structure):
                 ("total_synapses", ctypes.c_int64), ("steps_since_init", ctypes.c_int64),
                 ("current_step", ctypes.c_int32), ("n_output_neurons", ctypes.c_int32),
                 ("device_bytes", ctypes.c_int64), ("algorithmic_bytes_per_step", ctypes.c_int64),
-                ("n_tasks", ctypes.c_int32), ("n_groups", ctypes.c_int32), ("synapses_16bit", ctypes.c_int64)]
+                ("n_tasks", ctypes.c_int32), ("n_groups", ctypes.c_int32), ("synapses_16bit", ctypes.c_int64),
+                ("n_blocks", ctypes.c_int32), ("n_staged_blocks", ctypes.c_int32)]
 
 
 class PopulationInfo(ctypes.Structure):

@@ -395,7 +395,10 @@ int scaffold_set_option(scaffold_env *env, const char *key, int64_t value)
         if (value < 1 || value > 100000) return fail(SCAFFOLD_ERR_INVALID_ARG, "graph_steps in [1,100000]");
         o.graph_steps = (int)value;
     } else if (!strcmp(key, "reset_g")) o.reset_g = value ? 1 : 0;
-    else return fail(SCAFFOLD_ERR_INVALID_ARG, "unknown option %s", key);
+    else if (!strcmp(key, "stage_spikes")) {
+        if (frozen) return frozen_err();
+        o.stage_spikes = value ? 1 : 0;
+    } else return fail(SCAFFOLD_ERR_INVALID_ARG, "unknown option %s", key);
     return SCAFFOLD_OK;
 }
 
@@ -414,6 +417,7 @@ int scaffold_get_option(scaffold_env *env, const char *key, int64_t *value)
     else if (!strcmp(key, "graph")) *value = o.graph;
     else if (!strcmp(key, "graph_steps")) *value = o.graph_steps;
     else if (!strcmp(key, "reset_g")) *value = o.reset_g;
+    else if (!strcmp(key, "stage_spikes")) *value = o.stage_spikes;
     else return fail(SCAFFOLD_ERR_INVALID_ARG, "unknown option %s", key);
     return SCAFFOLD_OK;
 }
@@ -624,6 +628,7 @@ int scaffold_get_info(scaffold_env *env, scaffold_info *out)
         i.n_tasks = (int)env->plan.tasks.size();
         i.n_groups = env->plan.n_groups;
         i.synapses_16bit = env->plan.syn16;
+        engine_blocks(env->eng, &i.n_blocks, &i.n_staged_blocks);
     }
     *out = i;
     return SCAFFOLD_OK;

@@ -136,6 +136,8 @@ struct ScSlice {           // one lane-mode projection of one 64-neuron group
     const void *val;       // slice multipliers (nullptr: unit)
     int32_t width, delay, wprec, exc;
     float W, scale;
+    int32_t lds;           // staged blocks: LDS word that stands for word 0 of the ring row
+    int32_t pad_;
 };
 
 struct ScLaneTask {        // 64 neurons of a lane-mode population
@@ -153,8 +155,22 @@ struct ScStimTask {        // 64 neurons of an INPUT population
 
 struct ScTeamIn {          // one incoming projection of a team-mode neuron
     int32_t end;           // end of its segment in the merged row
-    int32_t delay, exc, pad_;
+    int32_t delay, exc;
+    int32_t lds;           // staged blocks: LDS word that stands for word 0 of the ring row
     float W, scale;
+};
+
+// Spike-bitmap windows staged in LDS by one block before its gathers: words
+// [w0, w0 + nw) of ring row (n - delay) of instance inst go to LDS words
+// [off, off + nw).  LDS word 0 is kept zero (padding entries read it).
+#define SC_MAX_WIN   8
+#define SC_WIN_WORDS 16384   // 64 KB of LDS per block
+struct ScWin {
+    int32_t inst, delay, w0, nw, off, pad_;
+};
+struct ScStage {
+    int32_t n, pad_;
+    ScWin w[SC_MAX_WIN];
 };
 
 struct ScTeamTask {        // one neuron of a team-mode population

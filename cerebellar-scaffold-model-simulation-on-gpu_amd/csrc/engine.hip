@@ -12,6 +12,16 @@
 
 namespace sc {
 
+// Bit ranges one task reads (per lane slice, or per incoming projection of a team
+// neuron) and the 64-byte segments its gathers touch: the input of the staging
+// plan.  lo > hi: the slice / projection reads nothing.
+struct TaskWin {
+    int n = 0;
+    int32_t delay[SC_MAX_IN] = {}, pre[SC_MAX_IN] = {};
+    uint32_t lo[SC_MAX_IN] = {}, hi[SC_MAX_IN] = {};
+    int64_t segs = 0;
+};
+
 struct Engine {
     hipStream_t stream = nullptr;
     std::vector<void *> allocs;
@@ -24,6 +34,7 @@ struct Engine {
     std::vector<ScStimTask> h_stims;
     std::vector<ScTeamTask> h_teams;
     std::vector<int> h_team_t;  // team width (wavefronts) of each team task
+    std::vector<TaskWin> h_lane_win, h_team_win;
     ScStim *stims = nullptr;
     int n_stims = 0;
     int n_pops = 0;
@@ -98,20 +109,111 @@ static ScLayout make_layout(const std::vector<int> &team_t, int n_lanes, int n_s
     return L;
 }
 
+// Staging plan: for every block of lane groups or team neurons, the windows of
+// ring-row words its tasks read, one per (instance, delay, presynaptic
+// population) spanning the union of their bit ranges.  A block is staged when
+// the windows fit the LDS budget and copying them touches at most half the
+// 64-byte segments its gathers would (window words / 16 <= segments / 2); the
+// slices and projections of a staged block then carry the LDS word that stands
+// for word 0 of their ring row.
+static std::vector<ScStage> plan_stages(const ScLayout &L, int n_blocks, std::vector<ScLaneTask> &lanes,
+                                        const std::vector<TaskWin> &lane_win, std::vector<ScTeamTask> &teams,
+                                        const std::vector<TaskWin> &team_win, int *n_staged)
+{
+    std::vector<ScStage> st((size_t)n_blocks);
+    *n_staged = 0;
+    for (int c = 0; c <= SC_CLASS_LANE; ++c) {
+        const bool lane = c == SC_CLASS_LANE;
+        const int b0 = c ? L.cls_end[c - 1] : 0;
+        for (int b = b0; b < L.cls_end[c]; ++b) {
+            const int bi = b - b0, per = lane ? SC_LANE_PER_BLOCK : 1 << c;
+            const int first = (lane ? 0 : L.cls_task0[c]) + bi * per;
+            const int count = std::min(per, L.cls_ntask[c] - bi * per);
+            ScWin wins[SC_MAX_WIN];
+            int32_t pre_of[SC_MAX_WIN];
+            uint32_t lo_of[SC_MAX_WIN], hi_of[SC_MAX_WIN];
+            int nw = 0;
+            bool ok = true;
+            int64_t segs = 0;
+            auto key_of = [&](int inst, int delay, int pre) {
+                for (int i = 0; i < nw; ++i)
+                    if (wins[i].inst == inst && wins[i].delay == delay && pre_of[i] == pre) return i;
+                return -1;
+            };
+            for (int i = 0; i < count && ok; ++i) {
+                const TaskWin &W = lane ? lane_win[first + i] : team_win[first + i];
+                const int inst = lane ? lanes[first + i].inst : teams[first + i].inst;
+                segs += W.segs;
+                for (int r = 0; r < W.n && ok; ++r) {
+                    if (W.lo[r] > W.hi[r]) continue;
+                    int kx = key_of(inst, W.delay[r], W.pre[r]);
+                    if (kx < 0) {
+                        if (nw == SC_MAX_WIN) { ok = false; break; }
+                        kx = nw++;
+                        wins[kx] = ScWin{inst, W.delay[r], 0, 0, 0, 0};
+                        pre_of[kx] = W.pre[r];
+                        lo_of[kx] = W.lo[r];
+                        hi_of[kx] = W.hi[r];
+                    }
+                    lo_of[kx] = std::min(lo_of[kx], W.lo[r]);
+                    hi_of[kx] = std::max(hi_of[kx], W.hi[r]);
+                }
+            }
+            if (!ok || nw == 0) continue;
+            int64_t words = 1;  // LDS word 0: the zero word
+            for (int i = 0; i < nw; ++i) {
+                wins[i].w0 = (int32_t)(lo_of[i] >> 5);
+                wins[i].nw = (int32_t)((hi_of[i] >> 5) - (lo_of[i] >> 5) + 1);
+                wins[i].off = (int32_t)words;
+                words += wins[i].nw;
+            }
+            if (words > SC_WIN_WORDS || words > 8 * segs) continue;
+            ScStage &S = st[(size_t)b];
+            S.n = nw;
+            for (int i = 0; i < nw; ++i) S.w[i] = wins[i];
+            for (int i = 0; i < count; ++i) {
+                const TaskWin &W = lane ? lane_win[first + i] : team_win[first + i];
+                const int inst = lane ? lanes[first + i].inst : teams[first + i].inst;
+                for (int r = 0; r < W.n; ++r) {
+                    const int kx = W.lo[r] > W.hi[r] ? -1 : key_of(inst, W.delay[r], W.pre[r]);
+                    const int32_t lds = kx < 0 ? 0 : wins[kx].off - wins[kx].w0;
+                    if (lane) lanes[first + i].sl[r].lds = lds;
+                    else teams[first + i].in[r].lds = lds;
+                }
+            }
+            ++*n_staged;
+        }
+    }
+    return st;
+}
+
 static int upload_work(std::vector<void *> &allocs, int64_t *bytes, DevWork &w, const std::vector<int> &team_t,
-                       const std::vector<ScLaneTask> &lanes, const std::vector<ScStimTask> &stims,
-                       const std::vector<ScTeamTask> &teams)
+                       std::vector<ScLaneTask> lanes, const std::vector<ScStimTask> &stims,
+                       std::vector<ScTeamTask> teams, const std::vector<TaskWin> &lane_win,
+                       const std::vector<TaskWin> &team_win, bool stage)
 {
     for (size_t i = 1; i < team_t.size(); ++i)
         if (team_t[i] > team_t[i - 1]) return fail(SCAFFOLD_ERR_STATE, "team descriptors not ordered by width");
+    if (lane_win.size() != lanes.size() || team_win.size() != teams.size())
+        return fail(SCAFFOLD_ERR_STATE, "staging ranges do not match the descriptors");
     int rc;
     ScLaneTask *l;
     ScStimTask *s;
     ScTeamTask *t;
+    w.layout = make_layout(team_t, (int)lanes.size(), (int)stims.size(), &w.n_blocks);
+    int n_staged = 0;
+    std::vector<ScStage> st;
+    if (stage) st = plan_stages(w.layout, w.n_blocks, lanes, lane_win, teams, team_win, &n_staged);
+    w.stages = nullptr;
+    w.n_staged = n_staged;
+    if (n_staged > 0) {
+        ScStage *ds;
+        if ((rc = dupload(allocs, bytes, &ds, st))) return rc;
+        w.stages = ds;
+    }
     if ((rc = dupload(allocs, bytes, &l, lanes))) return rc;
     if ((rc = dupload(allocs, bytes, &s, stims))) return rc;
     if ((rc = dupload(allocs, bytes, &t, teams))) return rc;
-    w.layout = make_layout(team_t, (int)lanes.size(), (int)stims.size(), &w.n_blocks);
     w.lanes = l;
     w.stims = s;
     w.teams = t;
@@ -245,9 +347,18 @@ int engine_create(Engine **out, Plan &P)
             d.pop = pop;
             d.n_in = pp.n_in;
             d.wprec = T.wprec;
-            d.len = (int)(T.row_ptr[j + 1] - T.row_ptr[j]);
+            // 16-bit form: the padded group length; 32-bit: the synapse count
+            d.len = T.idx16 ? (int)(T.row_ptr[j + 1] - T.row_ptr[j])
+                            : (pp.n_in ? T.seg_end[(size_t)j * pp.n_in + pp.n_in - 1] : 0);
+            TaskWin tw;
+            tw.n = pp.n_in;
+            tw.segs = T.row_seg[j];
             for (int q = 0; q < pp.n_in; ++q) {
                 const ScProj &sp = P.projs[pp.in_proj[q]];
+                tw.delay[q] = sp.delay;
+                tw.pre[q] = sp.pre_pop;
+                tw.lo[q] = T.q_lo[(size_t)j * pp.n_in + q];
+                tw.hi[q] = T.q_hi[(size_t)j * pp.n_in + q];
                 d.in[q].end = T.seg_end[(size_t)j * pp.n_in + q];
                 d.in[q].delay = sp.delay;
                 d.in[q].exc = sp.exc;
@@ -256,11 +367,13 @@ int engine_create(Engine **out, Plan &P)
             }
             e->h_teams.push_back(d);
             e->h_team_t.push_back(T.team_waves);
+            e->h_team_win.push_back(tw);
         } else if (t.kind == SC_TASK_LANE) {
             const ScGroup &G = P.groups[t.group];
             const ScPop &pp = P.pops[G.pop];
             const int gi = t.group - pp.group_base;
             ScLaneTask l{};
+            TaskWin tw;
             l.wp = SC_W_UNIT;
             l.group = t.group;
             l.pop = G.pop;
@@ -269,6 +382,12 @@ int engine_create(Engine **out, Plan &P)
                 const int pi = pp.in_proj[q];
                 const ScProj &sp = P.projs[pi];
                 const HostProjArrays &H = P.parr[pi];
+                tw.delay[l.n_sl] = sp.delay;
+                tw.pre[l.n_sl] = sp.pre_pop;
+                tw.lo[l.n_sl] = H.sl_lo[gi];
+                tw.hi[l.n_sl] = H.sl_hi[gi];
+                tw.segs += H.sl_seg[gi];
+                tw.n = l.n_sl + 1;
                 ScSlice &s = l.sl[l.n_sl++];
                 const int64_t off = H.sell_off[gi];
                 s.col = H.idx16 ? pcol[pi] + H.sell_off16[gi] * 4 : pcol[pi] + off * 4;
@@ -284,6 +403,7 @@ int engine_create(Engine **out, Plan &P)
                 if (s.val) l.wp = sp.wprec;
             }
             e->h_lanes.push_back(l);
+            e->h_lane_win.push_back(tw);
         } else {
             const ScGroup &G = P.groups[t.group];
             ScStimTask s{};
@@ -293,7 +413,9 @@ int engine_create(Engine **out, Plan &P)
             e->h_stims.push_back(s);
         }
     }
-    if ((rc = upload_work(A, B, e->w, e->h_team_t, e->h_lanes, e->h_stims, e->h_teams))) return bail(rc);
+    if ((rc = upload_work(A, B, e->w, e->h_team_t, e->h_lanes, e->h_stims, e->h_teams, e->h_lane_win, e->h_team_win,
+                          P.stage != 0)))
+        return bail(rc);
     if ((rc = dalloc(A, B, &e->cnt_buf, 1))) return bail(rc);
     std::vector<ScInstance> one{I};
     if ((rc = dupload(A, B, &e->di, one))) return bail(rc);
@@ -312,6 +434,12 @@ void engine_destroy(Engine *e)
 }
 
 int64_t engine_device_bytes(Engine *e) { return e ? e->bytes : 0; }
+
+void engine_blocks(Engine *e, int *n_blocks, int *n_staged)
+{
+    *n_blocks = e ? e->w.n_blocks : 0;
+    *n_staged = e ? e->w.n_staged : 0;
+}
 
 int engine_sync(Engine *e)
 {
@@ -593,7 +721,8 @@ void batch_destroy(Batch *b)
 
 int batch_build(Batch **out, std::vector<Engine *> &engines, const std::vector<const Plan *> &plans)
 {
-    (void)plans;
+    int stage = 1;  // staged only if every instance allows it
+    for (const Plan *p : plans) stage = stage && p->stage;
     *out = nullptr;
     Batch *b = new Batch();
     auto bail = [&](int code) { batch_destroy(b); return code; };
@@ -603,26 +732,37 @@ int batch_build(Batch **out, std::vector<Engine *> &engines, const std::vector<c
     std::vector<ScLaneTask> lanes;
     std::vector<ScStimTask> stims;
     std::vector<std::pair<int, ScTeamTask>> teams;
+    std::vector<TaskWin> lane_win, team_win_u;
     b->min_R = 1 << 30;
     for (size_t i = 0; i < engines.size(); ++i) {
         const Engine *e = engines[i];
         insts.push_back(e->hi);
         b->min_R = std::min(b->min_R, e->hi.R);
         for (ScLaneTask t : e->h_lanes) { t.inst = (int)i; lanes.push_back(t); }
+        lane_win.insert(lane_win.end(), e->h_lane_win.begin(), e->h_lane_win.end());
         for (ScStimTask t : e->h_stims) { t.inst = (int)i; stims.push_back(t); }
         for (size_t j = 0; j < e->h_teams.size(); ++j) {
             ScTeamTask t = e->h_teams[j];
             t.inst = (int)i;
+            t.pad_ = (int32_t)team_win_u.size();  // index of its ranges, restored below
             teams.push_back({e->h_team_t[j], t});
+            team_win_u.push_back(e->h_team_win[j]);
         }
     }
     std::stable_sort(teams.begin(), teams.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
     std::vector<ScTeamTask> tt;
     std::vector<int> tw;
-    for (auto &p : teams) { tw.push_back(p.first); tt.push_back(p.second); }
+    std::vector<TaskWin> team_win;
+    for (auto &p : teams) {
+        tw.push_back(p.first);
+        team_win.push_back(team_win_u[(size_t)p.second.pad_]);
+        p.second.pad_ = 0;
+        tt.push_back(p.second);
+    }
     int rc;
     if ((rc = dupload(b->allocs, nullptr, &b->di, insts))) return bail(rc);
-    if ((rc = upload_work(b->allocs, nullptr, b->w, tw, lanes, stims, tt))) return bail(rc);
+    if ((rc = upload_work(b->allocs, nullptr, b->w, tw, lanes, stims, tt, lane_win, team_win, stage != 0)))
+        return bail(rc);
     b->n_inst = (int)insts.size();
     *out = b;
     return SCAFFOLD_OK;

@@ -101,6 +101,43 @@ __device__ __forceinline__ uint32_t wraw(const void *val, int64_t e)
     else return 0u;
 }
 
+// 4 and 2 dwords loaded by one 16- / 8-byte load.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+// Raw bits of the 4 multipliers of vector v (entries 4v .. 4v+3) in storage WP,
+// one load: 16 bytes (F32), 8 (BF16) or 4 (FP8).
+template <int WP>
+struct Raw4 {
+    uint32_t w[WP == SC_W_F32 ? 4 : WP == SC_W_BF16 ? 2 : 1];
+};
+template <int WP>
+__device__ __forceinline__ Raw4<WP> wraw4(const void *val, int64_t v)
+{
+    Raw4<WP> r;
+    if constexpr (WP == SC_W_F32) {
+        const u32x4 x = gbl(static_cast<const u32x4 *>(val))[v];
+        r.w[0] = x.x; r.w[1] = x.y; r.w[2] = x.z; r.w[3] = x.w;
+    } else if constexpr (WP == SC_W_BF16) {
+        const u32x2 x = gbl(static_cast<const u32x2 *>(val))[v];
+        r.w[0] = x.x; r.w[1] = x.y;
+    } else if constexpr (WP == SC_W_FP8) {
+        r.w[0] = gbl(static_cast<const uint32_t *>(val))[v];
+    } else {
+        r.w[0] = 0u;
+    }
+    return r;
+}
+// Raw bits of element i of a Raw4.
+template <int WP>
+__device__ __forceinline__ uint32_t wsel4(const Raw4<WP> &r, int i)
+{
+    if constexpr (WP == SC_W_F32) return r.w[i];
+    else if constexpr (WP == SC_W_BF16) return (r.w[i >> 1] >> (16 * (i & 1))) & 0xFFFFu;
+    else if constexpr (WP == SC_W_FP8) return (r.w[0] >> (8 * i)) & 0xFFu;
+    else return 0u;
+}
+
 // Multiplier value of raw bits in storage WP (every conversion to float is exact).
 template <int WP>
 __device__ __forceinline__ float wdec(uint32_t c)
@@ -190,6 +227,16 @@ __device__ __forceinline__ int lif_step(const PopT &P, float &u, float &g0, floa
     return lif_steps<0>(u, g0, g1, r, dtc, dtt0, dtt1, inj0, inj1, E0, E1, El, gL, Ie, Vth, Vr, ref_step);
 }
 
+// The spike word holding device bit c of a lane slice: from the ring row in
+// global memory, or in a staged block from its LDS window (lds: the LDS word that
+// stands for word 0 of the row); padding (bit zb) reads LDS word 0, kept zero.
+template <bool STAGED>
+__device__ __forceinline__ uint32_t spike_word(gptr<uint32_t> row32, const uint32_t *win, int lds, uint32_t c, uint32_t zb)
+{
+    if constexpr (STAGED) return win[c == zb ? 0 : lds + (int)(c >> 5)];
+    else return row32[c >> 5];
+}
+
 // 16-bit slices: lane `lane`'s offsets of columns 2p and 2p+1 share the 32-bit
 // word p*64 + lane (low half: column 2p), relative to the pair's header (a
 // wave-uniform base device bit); SC_PAD16 marks padding, which reads bit zb.
@@ -202,8 +249,9 @@ __device__ __forceinline__ uint32_t pair_bit(uint32_t word, int hi, uint32_t bas
 // Entries k >= k0 of one sliced-ELL row (slices wider than the batched part), in
 // batches of U like the head of the row: entry indices past the slice width are
 // clamped to its last column and masked out of the sum.  k0 and U are even.
-template <int WP, int U, bool IDX16>
-__device__ __forceinline__ double slice_tail(const ScSlice &sl, gptr<uint32_t> row32, uint32_t zb, int k0, int lane)
+template <int WP, int U, bool IDX16, bool STAGED>
+__device__ __forceinline__ double slice_tail(const ScSlice &sl, gptr<uint32_t> row32, const uint32_t *win, uint32_t zb,
+                                             int k0, int lane)
 {
     double S = 0.0;
     const int width = sl.width;
@@ -233,7 +281,7 @@ __device__ __forceinline__ double slice_tail(const ScSlice &sl, gptr<uint32_t> r
             }
         }
 #pragma unroll
-        for (int j = 0; j < U; ++j) w[j] = row32[c[j] >> 5];
+        for (int j = 0; j < U; ++j) w[j] = spike_word<STAGED>(row32, win, sl.lds, c[j], zb);
 #pragma unroll
         for (int j = 0; j < U; ++j)
             if (k + j < width && ((w[j] >> (c[j] & 31)) & 1u)) S += unit ? 1.0 : (double)wdec<WP>(raw[j]);
@@ -247,9 +295,9 @@ __device__ __forceinline__ double slice_tail(const ScSlice &sl, gptr<uint32_t> r
 // Slice widths and pointers are wave-uniform; slices without multipliers (unit
 // weights) skip the multiplier loads.
 #define SC_LANE_U 4
-template <int WP, bool IDX16>
+template <int WP, bool IDX16, bool STAGED>
 __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance &I, const ScPop &P, int row_n,
-                                           int lane)
+                                           const uint32_t *win, int lane)
 {
     const uint32_t zb = (uint32_t)(I.Wr - 1) * SC_WAVE;
     const int d = t.group * SC_WAVE + lane;
@@ -310,7 +358,7 @@ __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance
         for (int k = 0; k < U; ++k) {
             if constexpr (IDX16)
                 c[s][k] = pair_bit(pw[s][k >> 1], k & 1, __builtin_amdgcn_readlane(hv, 2 * s + (k >> 1)), zb);
-            sw[s][k] = row[s][c[s][k] >> 5];
+            sw[s][k] = spike_word<STAGED>(row[s], win, t.sl[s].lds, c[s][k], zb);
         }
     float dge = 0.f, dgi = 0.f;
 #pragma unroll
@@ -321,7 +369,7 @@ __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance
 #pragma unroll
         for (int k = 0; k < U; ++k)
             if ((sw[s][k] >> (c[s][k] & 31)) & 1u) S += unit ? 1.0 : (double)wdec<WP>(raw[s][k]);
-        if (t.sl[s].width > U) S += slice_tail<WP, U, IDX16>(t.sl[s], row[s], zb, U, lane);
+        if (t.sl[s].width > U) S += slice_tail<WP, U, IDX16, STAGED>(t.sl[s], row[s], win, zb, U, lane);
         // dg[post] += S * w_bar (Simulation_header.cu:98), projections in index order
         const float Sf = (float)(S * (double)t.sl[s].scale);
         const float add = Sf * t.sl[s].W;
@@ -353,9 +401,9 @@ __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance
 // load each) with two wave-uniform header words (projection slot and base bit of
 // the group's two segments); an entry is a 15-bit offset whose bit 15 selects the
 // segment.  Padding entries (SC_PAD16) read the ring row's zero word (bit zb).
-template <int WP, int U, bool IDX16>
-__device__ __forceinline__ void team_gather(const ScTeamTask &T, gptr<uint32_t> ring32, const uint32_t *rowoff,
-                                            uint32_t zb, int tid, int TT, double *acc)
+template <int WP, int U, bool IDX16, bool STAGED>
+__device__ __forceinline__ void team_gather(const ScTeamTask &T, gptr<uint32_t> ring32, const uint32_t *win,
+                                            const uint32_t *rowoff, uint32_t zb, int tid, int TT, double *acc)
 {
     const int len = T.len;
     const void *val = T.val;
@@ -383,7 +431,9 @@ __device__ __forceinline__ void team_gather(const ScTeamTask &T, gptr<uint32_t> 
                 const uint32_t hs = (o & 0x8000u) ? h1 : h0;
                 const uint32_t pad = 0u - (uint32_t)(o == SC_PAD16);  // all ones for padding
                 c[u] = ((hs + (o & 0x7FFFu)) & ~pad) | (((h0 & ~SC_BIT_MASK) | zb) & pad);
-                s[u] = ring32[rowoff[c[u] >> SC_Q_SHIFT] + ((c[u] & SC_BIT_MASK) >> 5)];
+                const uint32_t wi = rowoff[c[u] >> SC_Q_SHIFT] + ((c[u] & SC_BIT_MASK) >> 5);
+                if constexpr (STAGED) s[u] = win[pad ? 0u : wi];  // padding reads the zero word
+                else s[u] = ring32[wi];
             }
 #pragma unroll
             for (int u = 0; u < U; ++u)
@@ -391,22 +441,34 @@ __device__ __forceinline__ void team_gather(const ScTeamTask &T, gptr<uint32_t> 
                     atomicAdd(&acc[c[u] >> SC_Q_SHIFT], (double)wdec<WP>(raw[u]));
         }
     } else {
-        gptr<uint32_t> col = gbl(static_cast<const uint32_t *>(T.col));
-        for (int b = tid; b < len; b += TT * U) {
-            uint32_t c[U], raw[U], s[U];
+        // 32-bit form: rows padded to whole 16-byte vectors; each lane loads 4
+        // consecutive entries (columns and multipliers) per load
+        constexpr int V = U / 4;  // vectors per batch
+        gptr<u32x4> col = gbl(static_cast<const u32x4 *>(T.col));
+        const int nv = (len + 3) >> 2;
+        for (int b = tid; b < nv; b += TT * V) {
+            uint32_t c[V][4], s[V][4];
+            Raw4<WP> raw[V];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int i = min(b + u * TT, len - 1);
-                c[u] = col[i];
-                raw[u] = wraw<WP>(val, i);
+            for (int u = 0; u < V; ++u) {
+                const int v = min(b + u * TT, nv - 1);
+                const u32x4 x = col[v];
+                c[u][0] = x.x; c[u][1] = x.y; c[u][2] = x.z; c[u][3] = x.w;
+                raw[u] = wraw4<WP>(val, v);
             }
 #pragma unroll
-            for (int u = 0; u < U; ++u)
-                s[u] = ring32[rowoff[c[u] >> SC_Q_SHIFT] + ((c[u] & SC_BIT_MASK) >> 5)];
+            for (int u = 0; u < V; ++u)
 #pragma unroll
-            for (int u = 0; u < U; ++u)
-                if (b + u * TT < len && ((s[u] >> (c[u] & 31)) & 1u))
-                    atomicAdd(&acc[c[u] >> SC_Q_SHIFT], (double)wdec<WP>(raw[u]));
+                for (int i = 0; i < 4; ++i) {
+                    const uint32_t wi = rowoff[c[u][i] >> SC_Q_SHIFT] + ((c[u][i] & SC_BIT_MASK) >> 5);
+                    s[u][i] = STAGED ? win[wi] : ring32[wi];
+                }
+#pragma unroll
+            for (int u = 0; u < V; ++u)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if (4 * (b + u * TT) + i < len && ((s[u][i] >> (c[u][i] & 31)) & 1u))
+                        atomicAdd(&acc[c[u][i] >> SC_Q_SHIFT], (double)wdec<WP>(wsel4<WP>(raw[u], i)));
         }
     }
 }
@@ -454,6 +516,28 @@ __device__ __noinline__ int stim_fires(const ScStim &S, int entry, int64_t n, in
     return spk;
 }
 
+// One lane task: storage form and multiplier type of its slices.
+template <bool IDX16, bool STAGED>
+__device__ __forceinline__ void lane_dispatch(const ScLaneTask &t, const ScInstance &I, const ScPop &P, int row_n,
+                                              const uint32_t *win, int lane)
+{
+    if (IDX16 && t.idx16) {
+        switch (t.wp) {
+        case SC_W_F32: lane_group<SC_W_F32, true, STAGED>(t, I, P, row_n, win, lane); break;
+        case SC_W_BF16: lane_group<SC_W_BF16, true, STAGED>(t, I, P, row_n, win, lane); break;
+        case SC_W_FP8: lane_group<SC_W_FP8, true, STAGED>(t, I, P, row_n, win, lane); break;
+        default: lane_group<SC_W_UNIT, true, STAGED>(t, I, P, row_n, win, lane); break;
+        }
+    } else {
+        switch (t.wp) {
+        case SC_W_F32: lane_group<SC_W_F32, false, STAGED>(t, I, P, row_n, win, lane); break;
+        case SC_W_BF16: lane_group<SC_W_BF16, false, STAGED>(t, I, P, row_n, win, lane); break;
+        case SC_W_FP8: lane_group<SC_W_FP8, false, STAGED>(t, I, P, row_n, win, lane); break;
+        default: lane_group<SC_W_UNIT, false, STAGED>(t, I, P, row_n, win, lane); break;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------- kernels
 // SINGLE: one instance passed by value (kernel argument); otherwise the task's
 // instance is read from the instance table.
@@ -465,6 +549,28 @@ __device__ __forceinline__ const ScInstance &inst_of(const ScInstance &I0, const
     else return insts[i];
 }
 
+// Gather of one team: storage form and multiplier type of its row.
+template <bool IDX16, bool STAGED>
+__device__ __forceinline__ void team_dispatch(const ScTeamTask &T, gptr<uint32_t> ring32, const uint32_t *win,
+                                              const uint32_t *rowoff, uint32_t zb, int tid, int TT, double *acc)
+{
+    if (IDX16 && T.hdr) {
+        switch (T.wprec) {
+        case SC_W_F32: team_gather<SC_W_F32, 8, true, STAGED>(T, ring32, win, rowoff, zb, tid, TT, acc); break;
+        case SC_W_BF16: team_gather<SC_W_BF16, 8, true, STAGED>(T, ring32, win, rowoff, zb, tid, TT, acc); break;
+        case SC_W_FP8: team_gather<SC_W_FP8, 8, true, STAGED>(T, ring32, win, rowoff, zb, tid, TT, acc); break;
+        default: team_gather<SC_W_UNIT, 8, true, STAGED>(T, ring32, win, rowoff, zb, tid, TT, acc); break;
+        }
+    } else {
+        switch (T.wprec) {
+        case SC_W_F32: team_gather<SC_W_F32, 8, false, STAGED>(T, ring32, win, rowoff, zb, tid, TT, acc); break;
+        case SC_W_BF16: team_gather<SC_W_BF16, 8, false, STAGED>(T, ring32, win, rowoff, zb, tid, TT, acc); break;
+        case SC_W_FP8: team_gather<SC_W_FP8, 8, false, STAGED>(T, ring32, win, rowoff, zb, tid, TT, acc); break;
+        default: team_gather<SC_W_UNIT, 8, false, STAGED>(T, ring32, win, rowoff, zb, tid, TT, acc); break;
+        }
+    }
+}
+
 // Team-mode neurons.  Teams of one wavefront come 16 to a block, wider teams
 // fewer.  After the gather, each team's first lane turns its row sums into dg
 // and parks dg, the neuron state and the population parameters in LDS; then the
@@ -474,7 +580,7 @@ __device__ __forceinline__ const ScInstance &inst_of(const ScInstance &I0, const
 template <bool SINGLE, bool IDX16>
 __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *__restrict__ insts, int t,
                                         int first, int count, const ScTeamTask *__restrict__ teams, int row_single,
-                                        int k)
+                                        int k, const uint32_t *win)
 {
     constexpr int NT = SC_BLOCK / SC_WAVE;
     __shared__ double acc[NT][SC_MAX_IN];
@@ -498,8 +604,11 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
     if (active && tw == 0) {
         if (lane < SC_MAX_IN) {
             acc[team][lane] = 0.0;
-            rowoff[team][lane] =
-                lane < T->n_in ? (uint32_t)row_back(row_n, T->in[lane].delay, I.R) * (uint32_t)(2 * I.Wr) : 0u;
+            // word offset of each projection's ring row: in global memory, or in a
+            // staged block the LDS word that stands for word 0 of the row
+            rowoff[team][lane] = lane >= T->n_in ? 0u
+                                 : win ? (uint32_t)T->in[lane].lds
+                                       : (uint32_t)row_back(row_n, T->in[lane].delay, I.R) * (uint32_t)(2 * I.Wr);
         }
         if (gather_updates && lane < SC_POP_WORDS)
             par[team][lane] = gbl(reinterpret_cast<const uint32_t *>(&I.pops[T->pop]))[lane];
@@ -514,21 +623,8 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
         gptr<uint32_t> ring32 = gbl(reinterpret_cast<const uint32_t *>(I.ring));
         const int tid = tw * SC_WAVE + lane, TT = t * SC_WAVE;
         const uint32_t zb = (uint32_t)(I.Wr - 1) * SC_WAVE;
-        if (IDX16 && T->hdr) {
-            switch (T->wprec) {
-            case SC_W_F32: team_gather<SC_W_F32, 8, true>(*T, ring32, rowoff[team], zb, tid, TT, acc[team]); break;
-            case SC_W_BF16: team_gather<SC_W_BF16, 8, true>(*T, ring32, rowoff[team], zb, tid, TT, acc[team]); break;
-            case SC_W_FP8: team_gather<SC_W_FP8, 8, true>(*T, ring32, rowoff[team], zb, tid, TT, acc[team]); break;
-            default: team_gather<SC_W_UNIT, 8, true>(*T, ring32, rowoff[team], zb, tid, TT, acc[team]); break;
-            }
-        } else {
-            switch (T->wprec) {
-            case SC_W_F32: team_gather<SC_W_F32, 8, false>(*T, ring32, rowoff[team], zb, tid, TT, acc[team]); break;
-            case SC_W_BF16: team_gather<SC_W_BF16, 8, false>(*T, ring32, rowoff[team], zb, tid, TT, acc[team]); break;
-            case SC_W_FP8: team_gather<SC_W_FP8, 8, false>(*T, ring32, rowoff[team], zb, tid, TT, acc[team]); break;
-            default: team_gather<SC_W_UNIT, 8, false>(*T, ring32, rowoff[team], zb, tid, TT, acc[team]); break;
-            }
-        }
+        if (win) team_dispatch<IDX16, true>(*T, ring32, win, rowoff[team], zb, tid, TT, acc[team]);
+        else team_dispatch<IDX16, false>(*T, ring32, win, rowoff[team], zb, tid, TT, acc[team]);
     }
     __syncthreads();
     float dge = 0.f, dgi = 0.f;
@@ -611,7 +707,8 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
 template <bool SINGLE, bool IDX16>
 __device__ __forceinline__ void step_body(const ScInstance &I0, const ScInstance *__restrict__ insts, const ScLayout &L,
                                           const ScLaneTask *__restrict__ lanes, const ScStimTask *__restrict__ stt,
-                                          const ScTeamTask *__restrict__ teams, int k)
+                                          const ScTeamTask *__restrict__ teams, const ScStage *__restrict__ stages,
+                                          int k)
 {
     // the step counter load is issued first; it overlaps the descriptor loads
     int row_single = 0;
@@ -625,10 +722,34 @@ __device__ __forceinline__ void step_body(const ScInstance &I0, const ScInstance
     for (int j = 0; j < SC_CLASSES - 1; ++j) c += (b >= L.cls_end[j]) ? 1 : 0;
     const int bi = b - (c ? L.cls_end[c - 1] : 0);
     const int wave = uniform(threadIdx.x / SC_WAVE), lane = threadIdx.x & (SC_WAVE - 1);
+    // staged block: copy its spike-bitmap windows into LDS, so that its gathers
+    // read LDS instead of scattering over the ring rows in memory
+    __shared__ uint32_t win_lds[SC_WIN_WORDS];
+    const uint32_t *win = nullptr;
+    const int nwin = stages ? uniform(stages[b].n) : 0;
+    if (nwin > 0) {
+        for (int w = 0; w < nwin; ++w) {
+            const ScWin W = stages[b].w[w];
+            const ScInstance &I = inst_of<SINGLE>(I0, insts, W.inst);
+            const int rn = SINGLE ? row_single : (I.ctl->row + k) % I.R;
+            gptr<uint32_t> src = gbl(reinterpret_cast<const uint32_t *>(I.ring + (int64_t)row_back(rn, W.delay, I.R) * I.Wr) + W.w0);
+            for (int i0 = threadIdx.x; i0 < W.nw; i0 += 4 * SC_BLOCK) {
+                uint32_t v[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) v[j] = i0 + j * SC_BLOCK < W.nw ? src[i0 + j * SC_BLOCK] : 0u;
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (i0 + j * SC_BLOCK < W.nw) win_lds[W.off + i0 + j * SC_BLOCK] = v[j];
+            }
+        }
+        if (threadIdx.x == 0) win_lds[0] = 0u;
+        __syncthreads();
+        win = win_lds;
+    }
     if (c < SC_CLASS_LANE) {
         const int t = SC_MAX_TEAM >> c, per = 1 << c;
         do_team<SINGLE, IDX16>(I0, insts, t, L.cls_task0[c] + bi * per, min(per, L.cls_ntask[c] - bi * per), teams,
-                        row_single, k);
+                               row_single, k, win);
         return;
     }
     const int per_block = c == SC_CLASS_LANE ? SC_LANE_PER_BLOCK : SC_STIM_PER_BLOCK;
@@ -639,21 +760,8 @@ __device__ __forceinline__ void step_body(const ScInstance &I0, const ScInstance
         const ScInstance &I = inst_of<SINGLE>(I0, insts, t.inst);
         const int row_n = SINGLE ? row_single : (I.ctl->row + k) % I.R;
         const ScPop &P = I.pops[t.pop];
-        if (IDX16 && t.idx16) {
-            switch (t.wp) {
-            case SC_W_F32: lane_group<SC_W_F32, true>(t, I, P, row_n, lane); break;
-            case SC_W_BF16: lane_group<SC_W_BF16, true>(t, I, P, row_n, lane); break;
-            case SC_W_FP8: lane_group<SC_W_FP8, true>(t, I, P, row_n, lane); break;
-            default: lane_group<SC_W_UNIT, true>(t, I, P, row_n, lane); break;
-            }
-        } else {
-            switch (t.wp) {
-            case SC_W_F32: lane_group<SC_W_F32, false>(t, I, P, row_n, lane); break;
-            case SC_W_BF16: lane_group<SC_W_BF16, false>(t, I, P, row_n, lane); break;
-            case SC_W_FP8: lane_group<SC_W_FP8, false>(t, I, P, row_n, lane); break;
-            default: lane_group<SC_W_UNIT, false>(t, I, P, row_n, lane); break;
-            }
-        }
+        if (win) lane_dispatch<IDX16, true>(t, I, P, row_n, win, lane);
+        else lane_dispatch<IDX16, false>(t, I, P, row_n, win, lane);
     } else {
         const ScStimTask t = stt[idx];
         const ScInstance &I = inst_of<SINGLE>(I0, insts, t.inst);
@@ -676,9 +784,9 @@ __device__ __forceinline__ void step_body(const ScInstance &I0, const ScInstance
 template <bool IDX16>
 __global__ void __launch_bounds__(SC_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_step(ScInstance I0, ScLayout L, const ScLaneTask *__restrict__ lanes, const ScStimTask *__restrict__ stt,
-       const ScTeamTask *__restrict__ teams, int k)
+       const ScTeamTask *__restrict__ teams, const ScStage *__restrict__ stages, int k)
 {
-    step_body<true, IDX16>(I0, nullptr, L, lanes, stt, teams, k);
+    step_body<true, IDX16>(I0, nullptr, L, lanes, stt, teams, stages, k);
 }
 
 // Many instances: every task names its instance in the instance table.  Held to
@@ -688,9 +796,10 @@ template <bool IDX16>
 __global__ void __launch_bounds__(SC_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) k_step_batch(const ScInstance *__restrict__ insts, ScLayout L,
                                                          const ScLaneTask *__restrict__ lanes,
                                                          const ScStimTask *__restrict__ stt,
-                                                         const ScTeamTask *__restrict__ teams, int k)
+                                                         const ScTeamTask *__restrict__ teams,
+                                                         const ScStage *__restrict__ stages, int k)
 {
-    step_body<false, IDX16>(ScInstance{}, insts, L, lanes, stt, teams, k);
+    step_body<false, IDX16>(ScInstance{}, insts, L, lanes, stt, teams, stages, k);
 }
 
 // Per-population spike counts and synaptic events of K consecutive steps: one
@@ -822,13 +931,13 @@ void launch_step(hipStream_t s, const ScInstance *single, const ScInstance *inst
 {
     if (w.n_blocks <= 0) return;
     if (single && w.idx16)
-        k_step<true><<<w.n_blocks, SC_BLOCK, 0, s>>>(*single, w.layout, w.lanes, w.stims, w.teams, k);
+        k_step<true><<<w.n_blocks, SC_BLOCK, 0, s>>>(*single, w.layout, w.lanes, w.stims, w.teams, w.stages, k);
     else if (single)
-        k_step<false><<<w.n_blocks, SC_BLOCK, 0, s>>>(*single, w.layout, w.lanes, w.stims, w.teams, k);
+        k_step<false><<<w.n_blocks, SC_BLOCK, 0, s>>>(*single, w.layout, w.lanes, w.stims, w.teams, w.stages, k);
     else if (w.idx16)
-        k_step_batch<true><<<w.n_blocks, SC_BLOCK, 0, s>>>(insts, w.layout, w.lanes, w.stims, w.teams, k);
+        k_step_batch<true><<<w.n_blocks, SC_BLOCK, 0, s>>>(insts, w.layout, w.lanes, w.stims, w.teams, w.stages, k);
     else
-        k_step_batch<false><<<w.n_blocks, SC_BLOCK, 0, s>>>(insts, w.layout, w.lanes, w.stims, w.teams, k);
+        k_step_batch<false><<<w.n_blocks, SC_BLOCK, 0, s>>>(insts, w.layout, w.lanes, w.stims, w.teams, w.stages, k);
 }
 
 void launch_stats(hipStream_t s, const ScInstance *single, const ScInstance *insts, int n_inst, int K)

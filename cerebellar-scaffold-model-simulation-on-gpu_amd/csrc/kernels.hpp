@@ -21,6 +21,8 @@ struct DevWork {
     const ScStimTask *stims = nullptr;
     const ScTeamTask *teams = nullptr;
     bool idx16 = false;  // some descriptor holds 16-bit column offsets
+    const ScStage *stages = nullptr;  // per block: spike windows staged in LDS (nullptr: none)
+    int n_staged = 0;                 // blocks with staged windows
 };
 
 // All launchers are asynchronous on stream s.  `single` != nullptr selects the

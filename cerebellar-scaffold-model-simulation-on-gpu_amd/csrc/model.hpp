@@ -61,6 +61,7 @@ struct Options {
     int graph = 1;
     int graph_steps = 25;
     int reset_g = 0;
+    int stage_spikes = 1;
 };
 
 // Everything the engine uploads, built once by plan_build().
@@ -78,6 +79,10 @@ struct HostProjArrays {
     std::vector<int64_t> row_ptr;
     std::vector<int64_t> sell_off;
     std::vector<int32_t> sell_w;
+    // per slice (64-neuron group): lowest and highest device bit it reads (lo > hi:
+    // none) and the 64-byte segments its gathers touch, estimated per column
+    std::vector<uint32_t> sl_lo, sl_hi;
+    std::vector<int64_t> sl_seg;
 };
 
 // Team-mode population: one merged CSR row per post neuron holding its synapses of
@@ -90,12 +95,16 @@ struct TeamPop {
     std::vector<int32_t> seg_end;   // num x n_in: end of each projection's segment, relative to the row
     std::vector<uint32_t> col;      // device bit indices
     std::vector<uint8_t> val;       // encoded multipliers
-    // 16-bit form: every projection segment of a row padded to whole groups of 64
-    // entries; a group's header holds its projection slot (top 3 bits) and base
-    // device bit, its entries 16-bit offsets from the base (0xFFFF: padding)
+    // 16-bit form: rows of 64-entry groups of at most two segments; two header
+    // words per group (projection slot in the top 3 bits, base device bit); an
+    // entry is a 15-bit offset, bit 15 selecting the segment (0xFFFF: padding)
     bool idx16 = false;
     std::vector<uint16_t> col16;
     std::vector<uint32_t> hdr;
+    // per neuron and projection (num x n_in): lowest and highest device bit read
+    // (lo > hi: none); per neuron: estimated 64-byte segments its gathers touch
+    std::vector<uint32_t> q_lo, q_hi;
+    std::vector<int64_t> row_seg;
 };
 
 struct Plan {
@@ -115,6 +124,7 @@ struct Plan {
     std::vector<ScTask> tasks;            // step kernel work list (planner form)
     int64_t nnz = 0;
     int64_t algo_bytes = 0;
+    int stage = 1;  // stage spike windows in LDS where the plan finds it pays
     int64_t syn16 = 0;  // synapses stored as 16-bit column offsets
     int max_delay = 0;
     std::vector<int> out_pops;            // OUTPUT populations
@@ -141,6 +151,7 @@ int engine_get_totals(Engine *e, unsigned long long *totals);
 int engine_time(Engine *e, int n_steps, int use_graph, int graph_steps,
                 double *ms_total, double *ms_kernel_mean);
 int64_t engine_device_bytes(Engine *e);
+void engine_blocks(Engine *e, int *n_blocks, int *n_staged);
 int engine_copy_probe(int64_t bytes, int iters, double *ms);
 
 // Batching (engine.hip).

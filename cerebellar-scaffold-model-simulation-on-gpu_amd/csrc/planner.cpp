@@ -333,6 +333,31 @@ int plan_build(scaffold_env *env, Plan &P)
                 }
             }
     }
+    // bit ranges of the lane slices and the segments their gathers touch (a column
+    // of 64 entries spanning b bits touches about min(64, b / 512 + 1) segments)
+    for (int pi = 0; pi < nproj; ++pi) {
+        const ScProj &sp = P.projs[pi];
+        if (sp.mode != SC_MODE_LANE) continue;
+        HostProjArrays &A = P.parr[pi];
+        const size_t ng = A.sell_off.size();
+        A.sl_lo.assign(ng, UINT32_MAX);
+        A.sl_hi.assign(ng, 0);
+        A.sl_seg.assign(ng, 0);
+        for (size_t gi = 0; gi < ng; ++gi)
+            for (int k = 0; k < A.sell_w[gi]; ++k) {
+                uint32_t lo = UINT32_MAX, hi = 0;
+                for (int l = 0; l < SC_WAVE; ++l) {
+                    const uint32_t b = A.col[A.sell_off[gi] + (int64_t)k * SC_WAVE + l];
+                    if (b == sentinel) continue;
+                    lo = std::min(lo, b);
+                    hi = std::max(hi, b);
+                }
+                if (lo > hi) continue;
+                A.sl_lo[gi] = std::min(A.sl_lo[gi], lo);
+                A.sl_hi[gi] = std::max(A.sl_hi[gi], hi);
+                A.sl_seg[gi] += std::min<int64_t>(SC_WAVE, (hi - lo) / 512 + 1);
+            }
+    }
     // 16-bit form of lane-mode slices, per post population: columns are taken in
     // pairs (2p, 2p+1), whose 128 entries must span fewer than 65,535 device bits
     if (want16) {
@@ -413,7 +438,8 @@ int plan_build(scaffold_env *env, Plan &P)
                 len += src.row_ptr[j + 1] - src.row_ptr[j];
                 T.seg_end[(size_t)j * d.n_in + q] = (int32_t)len;
             }
-            T.row_ptr[j + 1] = T.row_ptr[j] + len;
+            // 32-bit form: rows padded to whole 4-entry vectors (16-byte loads)
+            T.row_ptr[j + 1] = T.row_ptr[j] + (want16 ? len : (len + 3) / 4 * 4);
         }
         const int64_t tot = T.row_ptr[d.num];
         T.col.resize(tot);
@@ -429,7 +455,31 @@ int plan_build(scaffold_env *env, Plan &P)
                     if (wb) put_w(T.val, pos, P.projs[pi], eff[pi][e]);
                 }
             }
+            // padding repeats the last entry (a bit the row reads anyway) with a
+            // zero multiplier; the kernel masks it by the row length as well
+            for (; pos < T.row_ptr[j + 1]; ++pos) T.col[pos] = T.col[pos - 1];
         }
+        // bit ranges per neuron and projection, segments per neuron (per 64 entries
+        // of a projection segment spanning b bits: min(64, b / 512 + 1))
+        T.q_lo.assign((size_t)d.num * d.n_in, UINT32_MAX);
+        T.q_hi.assign((size_t)d.num * d.n_in, 0);
+        T.row_seg.assign(d.num, 0);
+        for (int j = 0; j < d.num; ++j)
+            for (int q = 0; q < d.n_in; ++q) {
+                const int64_t s0 = T.row_ptr[j] + (q ? T.seg_end[(size_t)j * d.n_in + q - 1] : 0);
+                const int64_t s1 = T.row_ptr[j] + T.seg_end[(size_t)j * d.n_in + q];
+                uint32_t &qlo = T.q_lo[(size_t)j * d.n_in + q], &qhi = T.q_hi[(size_t)j * d.n_in + q];
+                for (int64_t a = s0; a < s1; a += SC_WAVE) {
+                    uint32_t lo = UINT32_MAX, hi = 0;
+                    for (int64_t e = a; e < std::min(s1, a + SC_WAVE); ++e) {
+                        lo = std::min(lo, T.col[e] & SC_BIT_MASK);
+                        hi = std::max(hi, T.col[e] & SC_BIT_MASK);
+                    }
+                    qlo = std::min(qlo, lo);
+                    qhi = std::max(qhi, hi);
+                    T.row_seg[j] += std::min<int64_t>(SC_WAVE, (hi - lo) / 512 + 1);
+                }
+            }
         if (want16) {
             // 16-bit form: each row is a sequence of 64-entry groups (one wavefront
             // load each).  A group holds at most two segments -- runs of entries of
@@ -500,6 +550,7 @@ int plan_build(scaffold_env *env, Plan &P)
         if (is16) P.syn16 += (int64_t)src.col.size();
     }
     P.algo_bytes = index_bytes + weight_bytes + (int64_t)ref * 32 + ref / 8;
+    P.stage = opt.stage_spikes;
     for (auto &g : P.groups) g.self_final = 1;
 
     // work list: team neurons (widest teams first), lane groups, stimulus groups

@@ -167,7 +167,10 @@ int scaffold_set_input_stim_ids(scaffold_env *env, int neuron_index,
  *   "graph"             1: replay captured hipGraphs of the step kernels
  *   "graph_steps"       exchange steps per captured graph
  *   "reset_g"           1: also zero conductances in scaffold_reset_network_env
- *                       (the reference keeps them, GPUScaffoldSimulation.cu:802-854) */
+ *                       (the reference keeps them, GPUScaffoldSimulation.cu:802-854)
+ *   "stage_spikes"      1 (default): blocks whose gathers scatter over a bounded
+ *                       range of presynaptic neurons first copy those spike words
+ *                       into LDS; 0: every gather reads the spike ring in memory */
 int scaffold_set_option(scaffold_env *env, const char *key, int64_t value);
 int scaffold_get_option(scaffold_env *env, const char *key, int64_t *value);
 
@@ -252,6 +255,8 @@ typedef struct {
     int64_t  algorithmic_bytes_per_step; /* nnz*(idx+w bytes) + N*32 + N/8     */
     int32_t  n_tasks, n_groups;
     int64_t  synapses_16bit;       /* synapses stored as 16-bit column offsets */
+    int32_t  n_blocks;             /* blocks of the step kernel                */
+    int32_t  n_staged_blocks;      /* of which stage spike windows in LDS      */
 } scaffold_info;
 int scaffold_get_info(scaffold_env *env, scaffold_info *out);
 

@@ -180,6 +180,9 @@ def test_options(pkg):
     assert env.get_option("neuron_seed") == 99
     env.set_option("index_bits", 16)
     assert env.get_option("index_bits") == 16
+    assert env.get_option("stage_spikes") == 1
+    env.set_option("stage_spikes", 0)
+    assert env.get_option("stage_spikes") == 0
     for k, v, code in [("no_such_option", 1, "ERR_INVALID_ARG"), ("weight_precision", 7, "ERR_INVALID_ARG"),
                        ("index_bits", 8, "ERR_INVALID_ARG"), ("graph_steps", 0, "ERR_INVALID_ARG")]:
         with pytest.raises(pkg.ScaffoldError) as e:

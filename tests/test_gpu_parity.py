@@ -77,13 +77,19 @@ def test_config0_bit_exact(pkg, orc, graph):
     env.finalize()
 
 
-def test_config1_bit_exact(pkg, orc):
+@pytest.mark.parametrize("stage", [1, 0])
+def test_config1_bit_exact(pkg, orc, stage):
+    """The 96.7k scaffold, with blocks staging their spike windows in LDS (the
+    default) and with every gather reading the ring in memory."""
     env = new_env(pkg, 1)
+    env.set_option("stage_spikes", stage)
     env.initialize(50, 25)
+    info = env.info()
+    assert (info.n_staged_blocks > 0) if stage else (info.n_staged_blocks == 0)
     ref = orc.OracleSim.from_env(env)
     ref.initialize()
     run_and_compare(env, ref, 0, 150, check_counts=True)
-    assert_same_state(env, ref, "config1 150 steps")
+    assert_same_state(env, ref, f"config1 150 steps, stage_spikes {stage}")
     env.finalize()
 
 
