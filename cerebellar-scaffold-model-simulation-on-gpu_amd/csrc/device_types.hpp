@@ -196,10 +196,15 @@ struct ScLayout {
     int32_t pad_;
 };
 
+// State of one device neuron: one 16-byte record, read and written by one load
+// and one store per exchange step.
+struct alignas(16) ScState {
+    float   V, g_exc, g_inh;
+    int32_t refr;
+};
+
 struct ScInstance {
-    float    *V;
-    float    *g;
-    int32_t  *refr;
+    ScState  *st;              // per device index
     uint64_t *ring;
     int32_t  *counts;          // R x n_pops
     unsigned long long *events; // R

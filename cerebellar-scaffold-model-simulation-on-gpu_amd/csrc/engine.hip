@@ -245,9 +245,7 @@ int engine_create(Engine **out, Plan &P)
     e->Ndev = P.Ndev;
     e->total_ref = P.total_ref;
     e->dev_to_ref = P.dev_to_ref;
-    if ((rc = dalloc(A, B, &I.V, P.Ndev))) return bail(rc);
-    if ((rc = dalloc(A, B, &I.g, 2 * (size_t)P.Ndev))) return bail(rc);
-    if ((rc = dalloc(A, B, &I.refr, P.Ndev))) return bail(rc);
+    if ((rc = dalloc(A, B, &I.st, P.Ndev))) return bail(rc);
     if ((rc = dalloc(A, B, &I.ring, (size_t)P.R * P.Wr))) return bail(rc);
     if ((rc = dalloc(A, B, &I.counts, (size_t)P.R * I.n_pops))) return bail(rc);
     if ((rc = dalloc(A, B, &I.events, (size_t)P.R))) return bail(rc);
@@ -456,7 +454,8 @@ int engine_init_params(Engine *e, uint64_t seed, int64_t call_index)
 
 int engine_zero_g(Engine *e)
 {
-    HIP_TRY(hipMemsetAsync(e->hi.g, 0, sizeof(float) * 2 * (size_t)e->Ndev, e->stream));
+    launch_zero_g(e->stream, e->hi.st, e->Ndev);
+    HIP_TRY(hipGetLastError());
     return engine_sync(e);
 }
 
@@ -589,17 +588,14 @@ int engine_get_state(Engine *e, float *V, float *g, int32_t *refr)
 {
     HIP_TRY(hipStreamSynchronize(e->stream));
     const size_t N = (size_t)e->Ndev;
-    std::vector<float> hv(N), hg(2 * N);
-    std::vector<int32_t> hr(N);
-    HIP_TRY(hipMemcpy(hv.data(), e->hi.V, sizeof(float) * N, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(hg.data(), e->hi.g, sizeof(float) * 2 * N, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(hr.data(), e->hi.refr, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
+    std::vector<ScState> hs(N);
+    HIP_TRY(hipMemcpy(hs.data(), e->hi.st, sizeof(ScState) * N, hipMemcpyDeviceToHost));
     for (size_t d = 0; d < N; ++d) {
         const int r = e->dev_to_ref[d];
         if (r < 0) continue;
-        if (V) V[r] = hv[d];
-        if (g) { g[2 * (size_t)r] = hg[2 * d]; g[2 * (size_t)r + 1] = hg[2 * d + 1]; }
-        if (refr) refr[r] = hr[d];
+        if (V) V[r] = hs[d].V;
+        if (g) { g[2 * (size_t)r] = hs[d].g_exc; g[2 * (size_t)r + 1] = hs[d].g_inh; }
+        if (refr) refr[r] = hs[d].refr;
     }
     return SCAFFOLD_OK;
 }
@@ -608,21 +604,16 @@ int engine_set_state(Engine *e, const float *V, const float *g, const int32_t *r
 {
     HIP_TRY(hipStreamSynchronize(e->stream));
     const size_t N = (size_t)e->Ndev;
-    std::vector<float> hv(N, 0.f), hg(2 * N, 0.f);
-    std::vector<int32_t> hr(N, 0);
-    HIP_TRY(hipMemcpy(hv.data(), e->hi.V, sizeof(float) * N, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(hg.data(), e->hi.g, sizeof(float) * 2 * N, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(hr.data(), e->hi.refr, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
+    std::vector<ScState> hs(N);
+    HIP_TRY(hipMemcpy(hs.data(), e->hi.st, sizeof(ScState) * N, hipMemcpyDeviceToHost));
     for (size_t d = 0; d < N; ++d) {
         const int r = e->dev_to_ref[d];
         if (r < 0) continue;
-        if (V) hv[d] = V[r];
-        if (g) { hg[2 * d] = g[2 * (size_t)r]; hg[2 * d + 1] = g[2 * (size_t)r + 1]; }
-        if (refr) hr[d] = refr[r];
+        if (V) hs[d].V = V[r];
+        if (g) { hs[d].g_exc = g[2 * (size_t)r]; hs[d].g_inh = g[2 * (size_t)r + 1]; }
+        if (refr) hs[d].refr = refr[r];
     }
-    HIP_TRY(hipMemcpy(e->hi.V, hv.data(), sizeof(float) * N, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(e->hi.g, hg.data(), sizeof(float) * 2 * N, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(e->hi.refr, hr.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(e->hi.st, hs.data(), sizeof(ScState) * N, hipMemcpyHostToDevice));
     return SCAFFOLD_OK;
 }
 

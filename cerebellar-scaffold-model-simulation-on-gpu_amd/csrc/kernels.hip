@@ -78,11 +78,26 @@ __device__ __forceinline__ gptr<T> gbl(const T *p)
     return (gptr<T>)p;
 }
 
-// Both conductances of device neuron d (interleaved exc, inh) in one 8-byte load.
-__device__ __forceinline__ float2 load_g(const float *g, int d)
+// 4 and 2 dwords loaded by one 16- / 8-byte load.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+// State record of device neuron d: one 16-byte load, one 16-byte store.
+__device__ __forceinline__ void load_state(const ScState *st, int d, float &u, float2 &g, int32_t &r)
 {
-    const uint64_t v = gbl(reinterpret_cast<const uint64_t *>(g))[d];
-    return make_float2(__uint_as_float((uint32_t)v), __uint_as_float((uint32_t)(v >> 32)));
+    const u32x4 x = gbl(reinterpret_cast<const u32x4 *>(st))[d];
+    u = __uint_as_float(x.x);
+    g = make_float2(__uint_as_float(x.y), __uint_as_float(x.z));
+    r = (int32_t)x.w;
+}
+__device__ __forceinline__ void store_state(ScState *st, int d, float u, float2 g, int32_t r)
+{
+    u32x4 x;
+    x.x = __float_as_uint(u);
+    x.y = __float_as_uint(g.x);
+    x.z = __float_as_uint(g.y);
+    x.w = (uint32_t)r;
+    reinterpret_cast<u32x4 *>(st)[d] = x;
 }
 
 // Wave-uniform value in a scalar register (descriptor indices derived from the
@@ -100,10 +115,6 @@ __device__ __forceinline__ uint32_t wraw(const void *val, int64_t e)
     else if constexpr (WP == SC_W_FP8) return gbl(static_cast<const uint8_t *>(val))[e];
     else return 0u;
 }
-
-// 4 and 2 dwords loaded by one 16- / 8-byte load.
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 // Raw bits of the 4 multipliers of vector v (entries 4v .. 4v+3) in storage WP,
 // one load: 16 bytes (F32), 8 (BF16) or 4 (FP8).
@@ -246,55 +257,61 @@ __device__ __forceinline__ uint32_t pair_bit(uint32_t word, int hi, uint32_t bas
     return o == SC_PAD16 ? zb : base + o;
 }
 
-// Entries k >= k0 of one sliced-ELL row (slices wider than the batched part), in
-// batches of U like the head of the row: entry indices past the slice width are
-// clamped to its last column and masked out of the sum.  k0 and U are even.
-template <int WP, int U, bool IDX16, bool STAGED>
+// Entries k >= k0 of one sliced-ELL row (slices wider than the first vector of
+// 4 columns), 8 columns a batch: two vectors of columns (32-bit: one 16-byte
+// load each; 16-bit: four column pairs) and two vectors of multipliers.  Vector
+// indices past the slice are clamped to its last vector and the entries past
+// its width masked out of the sum.  k0 is a multiple of 4.
+template <int WP, bool IDX16, bool STAGED>
 __device__ __forceinline__ double slice_tail(const ScSlice &sl, gptr<uint32_t> row32, const uint32_t *win, uint32_t zb,
                                              int k0, int lane)
 {
     double S = 0.0;
-    const int width = sl.width;
+    const int width = sl.width, nv = (width + 3) >> 2;
     const bool unit = WP == SC_W_UNIT || !sl.val;
-    gptr<uint32_t> col = gbl(static_cast<const uint32_t *>(sl.col));
 #pragma unroll 1
-    for (int k = k0; k < width; k += U) {
-        uint32_t c[U], raw[U], w[U];
+    for (int k = k0; k < width; k += 8) {
+        uint32_t c[8], w[8];
+        Raw4<WP> raw[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int v = min((k >> 2) + h, nv - 1);
+            if (!unit) raw[h] = wraw4<WP>(sl.val, (int64_t)v * SC_WAVE + lane);
+            else raw[h].w[0] = 0u;
+        }
         if constexpr (IDX16) {
-            uint32_t pw[U / 2];
+            gptr<uint32_t> col = gbl(static_cast<const uint32_t *>(sl.col));
             const int last = (width - 1) >> 1;
             // the batch's pair headers in one load (lane j: pair j), read back with readlane
-            const uint32_t hv = gbl(sl.hdr)[min((k >> 1) + min(lane, U / 2 - 1), last)];
+            const uint32_t hv = gbl(sl.hdr)[min((k >> 1) + min(lane, 3), last)];
+            uint32_t pw[4];
 #pragma unroll
-            for (int j = 0; j < U / 2; ++j) pw[j] = col[min((k >> 1) + j, last) * SC_WAVE + lane];
+            for (int j = 0; j < 4; ++j) pw[j] = col[min((k >> 1) + j, last) * SC_WAVE + lane];
 #pragma unroll
-            for (int j = 0; j < U; ++j)
-                raw[j] = unit ? 0u : wraw<WP>(sl.val, min(k + j, width - 1) * SC_WAVE + lane);
-#pragma unroll
-            for (int j = 0; j < U; ++j) c[j] = pair_bit(pw[j >> 1], j & 1, __builtin_amdgcn_readlane(hv, j >> 1), zb);
+            for (int j = 0; j < 8; ++j) c[j] = pair_bit(pw[j >> 1], j & 1, __builtin_amdgcn_readlane(hv, j >> 1), zb);
         } else {
+            gptr<u32x4> col = gbl(static_cast<const u32x4 *>(sl.col));
 #pragma unroll
-            for (int j = 0; j < U; ++j) {
-                const int kk = min(k + j, width - 1);
-                c[j] = col[kk * SC_WAVE + lane];
-                raw[j] = unit ? 0u : wraw<WP>(sl.val, kk * SC_WAVE + lane);
+            for (int h = 0; h < 2; ++h) {
+                const u32x4 x = col[min((k >> 2) + h, nv - 1) * SC_WAVE + lane];
+                c[4 * h] = x.x; c[4 * h + 1] = x.y; c[4 * h + 2] = x.z; c[4 * h + 3] = x.w;
             }
         }
 #pragma unroll
-        for (int j = 0; j < U; ++j) w[j] = spike_word<STAGED>(row32, win, sl.lds, c[j], zb);
+        for (int j = 0; j < 8; ++j) w[j] = spike_word<STAGED>(row32, win, sl.lds, c[j], zb);
 #pragma unroll
-        for (int j = 0; j < U; ++j)
-            if (k + j < width && ((w[j] >> (c[j] & 31)) & 1u)) S += unit ? 1.0 : (double)wdec<WP>(raw[j]);
+        for (int j = 0; j < 8; ++j)
+            if (k + j < width && ((w[j] >> (c[j] & 31)) & 1u))
+                S += unit ? 1.0 : (double)wdec<WP>(wsel4<WP>(raw[j >> 2], j & 3));
     }
     return S;
 }
 
-// One 64-neuron group of a lane-mode population: the first SC_LANE_U entries of
-// every slice are loaded in one batch (column and multiplier loads, then all
-// spike words), the neuron state is loaded alongside, then the LIF update.
+// One 64-neuron group of a lane-mode population: the first 4 columns of every
+// slice are loaded in one batch (one vector of columns and one of multipliers per
+// slice, then all spike words), the neuron state alongside, then the LIF update.
 // Slice widths and pointers are wave-uniform; slices without multipliers (unit
 // weights) skip the multiplier loads.
-#define SC_LANE_U 4
 template <int WP, bool IDX16, bool STAGED>
 __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance &I, const ScPop &P, int row_n,
                                            const uint32_t *win, int lane)
@@ -305,14 +322,10 @@ __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance
     float u = 0.f;
     float2 gg = {0.f, 0.f};
     int32_t rf = 0;
-    if (valid) {
-        u = gbl(I.V)[d];
-        gg = load_g(I.g, d);
-        rf = gbl(I.refr)[d];
-    }
-    constexpr int U = SC_LANE_U;
-    uint32_t c[SC_MAX_SLICES][U], raw[SC_MAX_SLICES][U], sw[SC_MAX_SLICES][U];
-    uint32_t pw[SC_MAX_SLICES][U / 2];  // 16-bit form: offset pairs
+    if (valid) load_state(I.st, d, u, gg, rf);
+    uint32_t c[SC_MAX_SLICES][4], sw[SC_MAX_SLICES][4];
+    Raw4<WP> raw[SC_MAX_SLICES];
+    uint32_t pw[SC_MAX_SLICES][2];  // 16-bit form: offset pairs
     gptr<uint32_t> row[SC_MAX_SLICES];
     const int n_sl = t.n_sl;
     uint32_t hv = 0u;  // 16-bit form: lane 2s+p holds the header of pair p of slice s
@@ -334,44 +347,51 @@ __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance
         const int width = on ? t.sl[s].width : 0;
         row[s] = gbl(reinterpret_cast<const uint32_t *>(I.ring + (int64_t)(on ? row_back(row_n, t.sl[s].delay, I.R) : 0) * I.Wr));
         const void *val = t.sl[s].val;
-        gptr<uint32_t> col = gbl(static_cast<const uint32_t *>(t.sl[s].col));
+        raw[s].w[0] = 0u;
+        if (WP != SC_W_UNIT && val && width > 0) raw[s] = wraw4<WP>(val, lane);
         if constexpr (IDX16) {
+            gptr<uint32_t> col = gbl(static_cast<const uint32_t *>(t.sl[s].col));
 #pragma unroll
-            for (int p = 0; p < U / 2; ++p) {
+            for (int p = 0; p < 2; ++p) {
                 pw[s][p] = 0xFFFFFFFFu;
                 if (2 * p < width) pw[s][p] = col[p * SC_WAVE + lane];
             }
-        }
-#pragma unroll
-        for (int k = 0; k < U; ++k) {
-            c[s][k] = zb;
-            raw[s][k] = 0u;
-            if (k < width) {
-                if constexpr (!IDX16) c[s][k] = col[k * SC_WAVE + lane];
-                if (WP != SC_W_UNIT && val) raw[s][k] = wraw<WP>(val, k * SC_WAVE + lane);
+        } else {
+            c[s][0] = c[s][1] = c[s][2] = c[s][3] = zb;
+            if (width > 0) {
+                const u32x4 x = gbl(static_cast<const u32x4 *>(t.sl[s].col))[lane];
+                c[s][0] = x.x; c[s][1] = x.y; c[s][2] = x.z; c[s][3] = x.w;
             }
         }
     }
 #pragma unroll
     for (int s = 0; s < SC_MAX_SLICES; ++s)
 #pragma unroll
-        for (int k = 0; k < U; ++k) {
+        for (int k = 0; k < 4; ++k) {
             if constexpr (IDX16)
                 c[s][k] = pair_bit(pw[s][k >> 1], k & 1, __builtin_amdgcn_readlane(hv, 2 * s + (k >> 1)), zb);
             sw[s][k] = spike_word<STAGED>(row[s], win, t.sl[s].lds, c[s][k], zb);
         }
+    // row sums of the first 4 columns of every slice, then the wider slices' tails
+    // (so the head batch's registers are free while a tail runs)
+    double S[SC_MAX_SLICES];
+#pragma unroll
+    for (int s = 0; s < SC_MAX_SLICES; ++s) {
+        const bool unit = WP == SC_W_UNIT || !t.sl[s].val;
+        const int width = s < n_sl ? t.sl[s].width : 0;
+        S[s] = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (k < width && ((sw[s][k] >> (c[s][k] & 31)) & 1u))
+                S[s] += unit ? 1.0 : (double)wdec<WP>(wsel4<WP>(raw[s], k));
+    }
     float dge = 0.f, dgi = 0.f;
 #pragma unroll
     for (int s = 0; s < SC_MAX_SLICES; ++s) {
         if (s >= n_sl) break;
-        const bool unit = WP == SC_W_UNIT || !t.sl[s].val;
-        double S = 0.0;
-#pragma unroll
-        for (int k = 0; k < U; ++k)
-            if ((sw[s][k] >> (c[s][k] & 31)) & 1u) S += unit ? 1.0 : (double)wdec<WP>(raw[s][k]);
-        if (t.sl[s].width > U) S += slice_tail<WP, U, IDX16, STAGED>(t.sl[s], row[s], win, zb, U, lane);
+        if (t.sl[s].width > 4) S[s] += slice_tail<WP, IDX16, STAGED>(t.sl[s], row[s], win, zb, 4, lane);
         // dg[post] += S * w_bar (Simulation_header.cu:98), projections in index order
-        const float Sf = (float)(S * (double)t.sl[s].scale);
+        const float Sf = (float)(S[s] * (double)t.sl[s].scale);
         const float add = Sf * t.sl[s].W;
         if (t.sl[s].exc) dge = dge + add;
         else dgi = dgi + add;
@@ -379,9 +399,7 @@ __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance
     int spk = 0;
     if (valid) {
         spk = lif_step(P, u, gg.x, gg.y, rf, dge, dgi);
-        I.V[d] = u;
-        reinterpret_cast<float2 *>(I.g)[d] = gg;
-        I.refr[d] = rf;
+        store_state(I.st, d, u, gg, rf);
     }
     const unsigned long long word = __ballot(spk != 0);
     if (lane == 0) I.ring[(int64_t)row_n * I.Wr + t.group] = word;
@@ -588,7 +606,7 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
     constexpr int PW = (SC_POP_WORDS + 1) / 2 * 2;
     __shared__ alignas(8) uint32_t par[NT][PW];    // population parameters (8-byte aligned rows)
     __shared__ float sv[NT][10];  // u, g_exc, g_inh, dge, dgi; refr, device index, ring row, Wr, R (as bits)
-    __shared__ void *ptr[NT][4];  // instance arrays V, g, refr, ring of each team's network
+    __shared__ void *ptr[NT][2];  // instance arrays (state, ring) of each team's network
     const int wave = uniform(threadIdx.x / SC_WAVE), lane = threadIdx.x & (SC_WAVE - 1);
     const int team = wave / t, tw = wave - team * t;
     const bool active = team < count;
@@ -612,11 +630,7 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
         }
         if (gather_updates && lane < SC_POP_WORDS)
             par[team][lane] = gbl(reinterpret_cast<const uint32_t *>(&I.pops[T->pop]))[lane];
-        if (lane == 0) {  // state of the team's neuron, loaded while the gather runs
-            u = gbl(I.V)[d];
-            gg = load_g(I.g, d);
-            rf = gbl(I.refr)[d];
-        }
+        if (lane == 0) load_state(I.st, d, u, gg, rf);  // loaded while the gather runs
     }
     __syncthreads();
     if (active) {
@@ -648,18 +662,15 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
                 sv[team][7] = __int_as_float(row_n);
                 sv[team][8] = __int_as_float(I.Wr);
                 sv[team][9] = __int_as_float(I.R);
-                ptr[team][0] = I.V;
-                ptr[team][1] = I.g;
-                ptr[team][2] = I.refr;
-                ptr[team][3] = I.ring;
+                ptr[team][0] = I.st;
+                ptr[team][1] = I.ring;
             }
         }
     }
     int j = team;  // the team whose update this lane runs
     bool upd = active && tw == 0 && lane == 0;
     int dj = d;
-    float *V = I.V, *G = I.g;
-    int32_t *RF = I.refr;
+    ScState *ST = I.st;
     uint64_t *ring = I.ring;
     int Wr = I.Wr, R = I.R, rown = row_n;
     if (gather_updates) {
@@ -677,10 +688,8 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
                 rown = __float_as_int(sv[j][7]);
                 Wr = __float_as_int(sv[j][8]);
                 R = __float_as_int(sv[j][9]);
-                V = static_cast<float *>(ptr[j][0]);
-                G = static_cast<float *>(ptr[j][1]);
-                RF = static_cast<int32_t *>(ptr[j][2]);
-                ring = static_cast<uint64_t *>(ptr[j][3]);
+                ST = static_cast<ScState *>(ptr[j][0]);
+                ring = static_cast<uint64_t *>(ptr[j][1]);
             }
         }
     }
@@ -689,9 +698,7 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
         if (gather_updates) spk = lif_step(*reinterpret_cast<const ScPop *>(par[j]), u, gg.x, gg.y, rf, dge, dgi);
         else spk = lif_step(I.pops[T->pop], u, gg.x, gg.y, rf, dge, dgi);
         const int group = dj / SC_WAVE, bit = dj & (SC_WAVE - 1);
-        V[dj] = u;
-        reinterpret_cast<float2 *>(G)[dj] = gg;
-        RF[dj] = rf;
+        store_state(ST, dj, u, gg, rf);
         if (spk) atomicOr(&ring[(int64_t)rown * Wr + group], 1ull << bit);
         // the word of the next step starts empty (that row is not read during this step)
         if (bit == 0) {
@@ -879,8 +886,18 @@ __global__ void k_init_params(const ScInstance *__restrict__ ip, uint32_t k0, ui
     philox10(c0, c1, c2, c3, k0, k1);
     const uint32_t w = (off & 3) == 0 ? c0 : (off & 3) == 1 ? c1 : (off & 3) == 2 ? c2 : c3;
     const float prod = P.Vr_span * u01(w);
-    I.V[d] = P.Vr + prod;
-    I.refr[d] = 0;
+    I.st[d].V = P.Vr + prod;
+    I.st[d].refr = 0;
+}
+
+// Conductances of every device neuron set to zero (the other fields kept).
+__global__ void k_zero_g(ScState *st, int n)
+{
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d < n) {
+        st[d].g_exc = 0.f;
+        st[d].g_inh = 0.f;
+    }
 }
 
 // Fired-neuron list of one ring row: the spike words are the ballots of spike
@@ -966,6 +983,11 @@ void launch_compact(hipStream_t s, const uint64_t *row, int n_groups, const ScGr
                     int32_t *out, int cap, int32_t *count)
 {
     k_compact<<<1, 1024, 0, s>>>(row, n_groups, groups, pops, out, cap, count);
+}
+
+void launch_zero_g(hipStream_t s, ScState *st, int n)
+{
+    if (n > 0) k_zero_g<<<(n + 255) / 256, 256, 0, s>>>(st, n);
 }
 
 void launch_copy(hipStream_t s, const void *src, void *dst, int64_t n16, int grid)

@@ -35,5 +35,6 @@ void launch_init_params(hipStream_t s, const ScInstance *di, int n_threads, uint
 void launch_compact(hipStream_t s, const uint64_t *row, int n_groups, const ScGroup *groups, const ScPop *pops,
                     int32_t *out, int cap, int32_t *count);
 void launch_copy(hipStream_t s, const void *src, void *dst, int64_t n16, int grid);
+void launch_zero_g(hipStream_t s, ScState *st, int n);
 
 }  // namespace sc

@@ -64,6 +64,12 @@ struct Options {
     int stage_spikes = 1;
 };
 
+// Entry (column k, lane l) of a lane slice that starts at entry `off`: the
+// columns of a lane come in groups of 4 adjacent entries (one 16-byte load of
+// 32-bit columns; multipliers the same way), groups of the 64 lanes one after
+// another; slice widths are padded to a multiple of 4.
+inline int64_t sell_pos(int64_t off, int k, int l) { return off + (int64_t)(k >> 2) * (4 * SC_WAVE) + l * 4 + (k & 3); }
+
 // Everything the engine uploads, built once by plan_build().
 struct HostProjArrays {
     std::vector<uint32_t> col;

@@ -317,7 +317,7 @@ int plan_build(scaffold_env *env, Plan &P)
             }
             A.sell_off[gi] = total;
             A.sell_w[gi] = w;
-            total += (int64_t)w * SC_WAVE;
+            total += (int64_t)(w + 3) / 4 * 4 * SC_WAVE;
         }
         A.col.assign(total, sentinel);
         if (wb) A.val.assign(total * wb, 0);
@@ -327,7 +327,7 @@ int plan_build(scaffold_env *env, Plan &P)
                 if (j >= rows) continue;
                 int k = 0;
                 for (int64_t e = src.row_ptr[j]; e < src.row_ptr[j + 1]; ++e, ++k) {
-                    int64_t pos = A.sell_off[gi] + (int64_t)k * SC_WAVE + l;
+                    const int64_t pos = sell_pos(A.sell_off[gi], k, l);
                     A.col[pos] = (uint32_t)(pre.dev_base + src.col[e]);
                     if (wb) put_w(A.val, pos, sp, eff[pi][e]);
                 }
@@ -347,7 +347,7 @@ int plan_build(scaffold_env *env, Plan &P)
             for (int k = 0; k < A.sell_w[gi]; ++k) {
                 uint32_t lo = UINT32_MAX, hi = 0;
                 for (int l = 0; l < SC_WAVE; ++l) {
-                    const uint32_t b = A.col[A.sell_off[gi] + (int64_t)k * SC_WAVE + l];
+                    const uint32_t b = A.col[sell_pos(A.sell_off[gi], k, l)];
                     if (b == sentinel) continue;
                     lo = std::min(lo, b);
                     hi = std::max(hi, b);
@@ -389,7 +389,7 @@ int plan_build(scaffold_env *env, Plan &P)
                         uint32_t lo = UINT32_MAX, hi = 0;
                         for (int k = 2 * pp; k < std::min(w, 2 * pp + 2); ++k)
                             for (int l = 0; l < SC_WAVE; ++l) {
-                                const uint32_t b = A.col[A.sell_off[gi] + (int64_t)k * SC_WAVE + l];
+                                const uint32_t b = A.col[sell_pos(A.sell_off[gi], k, l)];
                                 if (b == sentinel) continue;
                                 lo = std::min(lo, b);
                                 hi = std::max(hi, b);
@@ -400,7 +400,7 @@ int plan_build(scaffold_env *env, Plan &P)
                         C.h[wbase / SC_WAVE] = lo;
                         for (int k = 2 * pp; k < std::min(w, 2 * pp + 2); ++k)
                             for (int l = 0; l < SC_WAVE; ++l) {
-                                const uint32_t b = A.col[A.sell_off[gi] + (int64_t)k * SC_WAVE + l];
+                                const uint32_t b = A.col[sell_pos(A.sell_off[gi], k, l)];
                                 if (b != sentinel) C.c16[(wbase + l) * 2 + (k & 1)] = (uint16_t)(b - lo);
                             }
                     }
