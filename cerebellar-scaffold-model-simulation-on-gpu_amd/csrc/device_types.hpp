@@ -162,7 +162,8 @@ struct ScTeamIn {          // one incoming projection of a team-mode neuron
 
 // Spike-bitmap windows staged in LDS by one block before its gathers: words
 // [w0, w0 + nw) of ring row (n - delay) of instance inst go to LDS words
-// [off, off + nw).  LDS word 0 is kept zero (padding entries read it).
+// [off, off + nw); w0, nw and off are multiples of 4 (16-byte copies).  LDS words
+// 0..3 are kept zero (padding entries read word 0).
 #define SC_MAX_WIN   8
 #define SC_WIN_WORDS 16384   // 64 KB of LDS per block
 struct ScWin {

@@ -160,10 +160,10 @@ static std::vector<ScStage> plan_stages(const ScLayout &L, int n_blocks, std::ve
                 }
             }
             if (!ok || nw == 0) continue;
-            int64_t words = 1;  // LDS word 0: the zero word
-            for (int i = 0; i < nw; ++i) {
-                wins[i].w0 = (int32_t)(lo_of[i] >> 5);
-                wins[i].nw = (int32_t)((hi_of[i] >> 5) - (lo_of[i] >> 5) + 1);
+            int64_t words = 4;  // LDS words 0..3: zero (padding entries read word 0)
+            for (int i = 0; i < nw; ++i) {  // whole 16-byte vectors: rows are 16-byte aligned
+                wins[i].w0 = (int32_t)((lo_of[i] >> 5) & ~3u);
+                wins[i].nw = (int32_t)((((hi_of[i] >> 5) + 4) & ~3u) - (uint32_t)wins[i].w0);
                 wins[i].off = (int32_t)words;
                 words += wins[i].nw;
             }
@@ -246,7 +246,8 @@ int engine_create(Engine **out, Plan &P)
     e->total_ref = P.total_ref;
     e->dev_to_ref = P.dev_to_ref;
     if ((rc = dalloc(A, B, &I.st, P.Ndev))) return bail(rc);
-    if ((rc = dalloc(A, B, &I.ring, (size_t)P.R * P.Wr))) return bail(rc);
+    // 16 bytes of slack: a staged window's last 16-byte load may run past the last row
+    if ((rc = dalloc(A, B, &I.ring, (size_t)P.R * P.Wr + 2))) return bail(rc);
     if ((rc = dalloc(A, B, &I.counts, (size_t)P.R * I.n_pops))) return bail(rc);
     if ((rc = dalloc(A, B, &I.events, (size_t)P.R))) return bail(rc);
     if ((rc = dalloc(A, B, &I.totals, (size_t)I.n_pops + 1))) return bail(rc);

@@ -149,6 +149,10 @@ __device__ __forceinline__ uint32_t wsel4(const Raw4<WP> &r, int i)
     else return 0u;
 }
 
+// Value of element i (a constant after unrolling) of a Raw4.
+template <int WP>
+__device__ __forceinline__ float wval4(const Raw4<WP> &r, int i);
+
 // Multiplier value of raw bits in storage WP (every conversion to float is exact).
 template <int WP>
 __device__ __forceinline__ float wdec(uint32_t c)
@@ -158,12 +162,26 @@ __device__ __forceinline__ float wdec(uint32_t c)
     } else if constexpr (WP == SC_W_BF16) {
         return __uint_as_float(c << 16);
     } else if constexpr (WP == SC_W_FP8) {
-        // OCP e4m3: (1 + m/8) * 2^(e-7), subnormal m * 2^-9
-        const uint32_t ex = (c >> 3) & 15u, mt = c & 7u;
-        const float v = ex ? __uint_as_float(((ex + 120u) << 23) | (mt << 20)) : (float)mt * 0x1p-9f;
-        return (c & 0x80u) ? -v : v;
+        // OCP e4m3 ((1 + m/8) * 2^(e-7), subnormal m * 2^-9) by gfx950's conversion
+        // instruction, which decodes every code the planner stores (no NaN) exactly
+        return __builtin_amdgcn_cvt_f32_fp8((int)c, 0);
     } else {
         return 1.f;
+    }
+}
+
+template <int WP>
+__device__ __forceinline__ float wval4(const Raw4<WP> &r, int i)
+{
+    if constexpr (WP == SC_W_FP8) {
+        switch (i) {  // the byte select of the conversion is an immediate
+        case 0: return __builtin_amdgcn_cvt_f32_fp8((int)r.w[0], 0);
+        case 1: return __builtin_amdgcn_cvt_f32_fp8((int)r.w[0], 1);
+        case 2: return __builtin_amdgcn_cvt_f32_fp8((int)r.w[0], 2);
+        default: return __builtin_amdgcn_cvt_f32_fp8((int)r.w[0], 3);
+        }
+    } else {
+        return wdec<WP>(wsel4<WP>(r, i));
     }
 }
 
@@ -302,7 +320,7 @@ __device__ __forceinline__ double slice_tail(const ScSlice &sl, gptr<uint32_t> r
 #pragma unroll
         for (int j = 0; j < 8; ++j)
             if (k + j < width && ((w[j] >> (c[j] & 31)) & 1u))
-                S += unit ? 1.0 : (double)wdec<WP>(wsel4<WP>(raw[j >> 2], j & 3));
+                S += unit ? 1.0 : (double)wval4<WP>(raw[j >> 2], j & 3);
     }
     return S;
 }
@@ -383,7 +401,7 @@ __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance
 #pragma unroll
         for (int k = 0; k < 4; ++k)
             if (k < width && ((sw[s][k] >> (c[s][k] & 31)) & 1u))
-                S[s] += unit ? 1.0 : (double)wdec<WP>(wsel4<WP>(raw[s], k));
+                S[s] += unit ? 1.0 : (double)wval4<WP>(raw[s], k);
     }
     float dge = 0.f, dgi = 0.f;
 #pragma unroll
@@ -486,7 +504,7 @@ __device__ __forceinline__ void team_gather(const ScTeamTask &T, gptr<uint32_t> 
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
                     if (4 * (b + u * TT) + i < len && ((s[u][i] >> (c[u][i] & 31)) & 1u))
-                        atomicAdd(&acc[c[u][i] >> SC_Q_SHIFT], (double)wdec<WP>(wsel4<WP>(raw[u], i)));
+                        atomicAdd(&acc[c[u][i] >> SC_Q_SHIFT], (double)wval4<WP>(raw[u], i));
         }
     }
 }
@@ -731,25 +749,29 @@ __device__ __forceinline__ void step_body(const ScInstance &I0, const ScInstance
     const int wave = uniform(threadIdx.x / SC_WAVE), lane = threadIdx.x & (SC_WAVE - 1);
     // staged block: copy its spike-bitmap windows into LDS, so that its gathers
     // read LDS instead of scattering over the ring rows in memory
-    __shared__ uint32_t win_lds[SC_WIN_WORDS];
+    __shared__ __attribute__((aligned(16))) uint32_t win_lds[SC_WIN_WORDS];
     const uint32_t *win = nullptr;
     const int nwin = stages ? uniform(stages[b].n) : 0;
     if (nwin > 0) {
+        u32x4 *dst = reinterpret_cast<u32x4 *>(win_lds);
         for (int w = 0; w < nwin; ++w) {
             const ScWin W = stages[b].w[w];
             const ScInstance &I = inst_of<SINGLE>(I0, insts, W.inst);
             const int rn = SINGLE ? row_single : (I.ctl->row + k) % I.R;
-            gptr<uint32_t> src = gbl(reinterpret_cast<const uint32_t *>(I.ring + (int64_t)row_back(rn, W.delay, I.R) * I.Wr) + W.w0);
-            for (int i0 = threadIdx.x; i0 < W.nw; i0 += 4 * SC_BLOCK) {
-                uint32_t v[4];
+            gptr<u32x4> src = gbl(reinterpret_cast<const u32x4 *>(
+                reinterpret_cast<const uint32_t *>(I.ring + (int64_t)row_back(rn, W.delay, I.R) * I.Wr) + W.w0));
+            const int n4 = W.nw >> 2, o4 = W.off >> 2;
+            for (int i0 = threadIdx.x; i0 < n4; i0 += 2 * SC_BLOCK) {
+                u32x4 v[2];
 #pragma unroll
-                for (int j = 0; j < 4; ++j) v[j] = i0 + j * SC_BLOCK < W.nw ? src[i0 + j * SC_BLOCK] : 0u;
+                for (int j = 0; j < 2; ++j)
+                    if (i0 + j * SC_BLOCK < n4) v[j] = src[i0 + j * SC_BLOCK];
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    if (i0 + j * SC_BLOCK < W.nw) win_lds[W.off + i0 + j * SC_BLOCK] = v[j];
+                for (int j = 0; j < 2; ++j)
+                    if (i0 + j * SC_BLOCK < n4) dst[o4 + i0 + j * SC_BLOCK] = v[j];
             }
         }
-        if (threadIdx.x == 0) win_lds[0] = 0u;
+        if (threadIdx.x == 0) dst[0] = u32x4{0u, 0u, 0u, 0u};
         __syncthreads();
         win = win_lds;
     }

@@ -174,13 +174,13 @@ int plan_build(scaffold_env *env, Plan &P)
     }
     P.Ndev = dev;
     P.n_groups = dev / SC_WAVE;
-    P.Wr = P.n_groups + 1;
+    P.Wr = (P.n_groups + 2) & ~1;  // at least one always-zero word; even, so rows are 16-byte aligned
     P.total_ref = ref;
     P.n_output = 0;
     for (int p : P.out_pops) P.n_output += P.pops[p].num;
     if ((int64_t)dev + SC_WAVE > (int64_t)SC_BIT_MASK)
         return fail(SCAFFOLD_ERR_CAPACITY, "more than 2^29 neurons in one instance");
-    const uint32_t sentinel = (uint32_t)P.n_groups * SC_WAVE;  // a bit of the always-zero word
+    const uint32_t sentinel = (uint32_t)(P.Wr - 1) * SC_WAVE;  // a bit of the row's last, always-zero word
 
     P.dev_to_ref.assign(dev, -1);
     P.ref_to_dev.assign(ref, 0);
