@@ -297,6 +297,45 @@ def test_index16_bit_exact(pkg, orc, preset, precision, fanin):
     env.finalize()
 
 
+@pytest.mark.parametrize("fanin,bits", [(48, 32), (4096, 32), (48, 16), (4096, 16)])
+def test_fp8_every_code_bit_exact(pkg, orc, fanin, bits):
+    """FP8 multipliers are decoded on the GPU by the hardware conversion; the oracle
+    decodes them in software.  One projection whose every row holds all 126
+    positive finite e4m3 values (0x01..0x7E: subnormals up to 448, so the storage
+    scale is 1), driven by 126 Poisson inputs at 500 Hz: rasters and state agree
+    bit for bit, in team mode and in lane mode, with 32- and 16-bit columns."""
+    def e4m3(c):
+        e, m = (c >> 3) & 15, c & 7
+        return (1 + m / 8) * 2.0 ** (e - 7) if e else m * 2.0 ** -9
+    vals = np.array([e4m3(c) for c in range(1, 0x7F)], np.float32)
+    n = vals.size
+    granule = (3.0, -74.0, 1.5, 0.0, -84.0, -42.0, 1.5, (1., 0., 1.0, 0.5, 1., -85., 1.0, 10.0))
+    Cm, El, dt_ref, Ie, Vr, Vth, gL, ch = granule
+    env = pkg.NetworkEnvironment(1, 2, 1, 1, 0)
+    post = env.set_neuron_params(0, "post", 0, n, Cm, El, dt_ref, Ie, Vr, Vth, gL, pkg.NORMAL, ch)
+    inp = env.set_neuron_params(1, "inputs", 0, n, Cm, El, dt_ref, Ie, Vr, Vth, gL, pkg.INPUT, ch)
+    rp = np.arange(n + 1, dtype=np.int64) * n
+    col = np.tile(np.arange(n, dtype=np.int32), n)
+    val = np.concatenate([np.roll(vals, j) for j in range(n)])
+    env.set_connectivity_csr(0, n, n, inp, post, 2e-4, 1.0, rp, col, val)
+    f = pkg.StimFunc()
+    f.poisson_hz, f.poisson_on, f.poisson_off, f.seed = 500.0, -float("inf"), float("inf"), 99
+    env.set_input_stim_ids(inp, np.arange(n), f)
+    env.set_option("weight_precision", pkg.WEIGHTS_FP8)
+    env.set_option("lane_max_fanin", fanin)
+    env.set_option("index_bits", bits)
+    env.initialize(50, 25)
+    p = env.projection(0)
+    assert p.fp8_scale == 1.0 and p.sum_scale == 1.0 and p.lane_mode == (fanin > n)
+    _, _, v = env.projection_csr(0)
+    assert np.array_equal(v, val)          # every code is stored exactly
+    ref = orc.OracleSim.from_env(env)
+    ref.initialize()
+    run_and_compare(env, ref, 0, 200)
+    assert_same_state(env, ref, f"fp8 codes, lane_max_fanin {fanin}, index_bits {bits}")
+    env.finalize()
+
+
 def test_unit_weights_and_reference_delays(pkg, orc):
     env = new_env(pkg, 0, weight_jitter=0, reference_delays=1)
     env.initialize(50, 25)
