@@ -124,11 +124,9 @@ struct ScTask {
 #define SC_STIM_PER_BLOCK 4   // stimulus groups per block (the other wavefronts idle)
 
 // 16-bit column form: columns are stored as 16-bit offsets from base device bits
-// held in headers (lane slices: one per column pair; team rows: two per 64-entry
-// group, the second selected by bit 15 of the entry, offsets below 0x7FFF);
-// 0xFFFF marks padding.
+// held in headers (lane slices: one per column pair; team rows: one per 8-entry
+// chunk); 0xFFFF marks padding.
 #define SC_PAD16 0xFFFFu
-#define SC_OFF15_PAD 0x7FFFu
 
 struct ScSlice {           // one lane-mode projection of one 64-neuron group
     const void *col;       // slice columns, 64 x width, column-major (32-bit bits or 16-bit offsets)

@@ -338,16 +338,19 @@ int engine_create(Engine **out, Plan &P)
             const ScPop &pp = P.pops[pop];
             ScTeamTask d{};
             const int ti = P.team_of[pop];
-            d.col = tcol[ti] + T.row_ptr[j] * (T.idx16 ? 2 : 4);
-            d.hdr = thdr[ti] ? thdr[ti] + T.row_ptr[j] / SC_WAVE * 2 : nullptr;  // two header words per group
-            d.val = tval[P.team_of[pop]] ? tval[P.team_of[pop]] + T.row_ptr[j] * wbytes_of(T.wprec) : nullptr;
+            // 32-bit form: row_ptr counts entries; 16-bit form: 8-entry chunks (16 bytes
+            // of offsets, one header word and 8 multipliers each)
+            const int64_t e0 = T.idx16 ? T.row_ptr[j] * 8 : T.row_ptr[j];
+            d.col = tcol[ti] + e0 * (T.idx16 ? 2 : 4);
+            d.hdr = thdr[ti] ? thdr[ti] + T.row_ptr[j] : nullptr;
+            d.val = tval[ti] ? tval[ti] + e0 * wbytes_of(T.wprec) : nullptr;
             d.group = t.group;
             d.bit = j % SC_WAVE;
             d.pop = pop;
             d.n_in = pp.n_in;
             d.wprec = T.wprec;
             // 16-bit form: the padded group length; 32-bit: the synapse count
-            d.len = T.idx16 ? (int)(T.row_ptr[j + 1] - T.row_ptr[j])
+            d.len = T.idx16 ? (int)(T.row_ptr[j + 1] - T.row_ptr[j])  // chunks
                             : (pp.n_in ? T.seg_end[(size_t)j * pp.n_in + pp.n_in - 1] : 0);
             TaskWin tw;
             tw.n = pp.n_in;

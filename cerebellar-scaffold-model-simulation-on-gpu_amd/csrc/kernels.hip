@@ -185,6 +185,52 @@ __device__ __forceinline__ float wval4(const Raw4<WP> &r, int i)
     }
 }
 
+// Raw bits of the 8 multipliers of chunk v (entries 8v .. 8v+7): 32 bytes (F32,
+// two loads), 16 (BF16) or 8 (FP8).
+template <int WP>
+struct Raw8 {
+    uint32_t w[WP == SC_W_F32 ? 8 : WP == SC_W_BF16 ? 4 : 2];
+};
+template <int WP>
+__device__ __forceinline__ Raw8<WP> wraw8(const void *val, int64_t v)
+{
+    Raw8<WP> r;
+    if constexpr (WP == SC_W_F32) {
+        const u32x4 x = gbl(static_cast<const u32x4 *>(val))[2 * v], y = gbl(static_cast<const u32x4 *>(val))[2 * v + 1];
+        r.w[0] = x.x; r.w[1] = x.y; r.w[2] = x.z; r.w[3] = x.w;
+        r.w[4] = y.x; r.w[5] = y.y; r.w[6] = y.z; r.w[7] = y.w;
+    } else if constexpr (WP == SC_W_BF16) {
+        const u32x4 x = gbl(static_cast<const u32x4 *>(val))[v];
+        r.w[0] = x.x; r.w[1] = x.y; r.w[2] = x.z; r.w[3] = x.w;
+    } else if constexpr (WP == SC_W_FP8) {
+        const u32x2 x = gbl(static_cast<const u32x2 *>(val))[v];
+        r.w[0] = x.x; r.w[1] = x.y;
+    } else {
+        r.w[0] = r.w[1] = 0u;
+    }
+    return r;
+}
+// Value of element i (a constant after unrolling) of a Raw8.
+template <int WP>
+__device__ __forceinline__ float wval8(const Raw8<WP> &r, int i)
+{
+    if constexpr (WP == SC_W_FP8) {
+        const int x = (int)r.w[i >> 2];
+        switch (i & 3) {  // the byte select of the conversion is an immediate
+        case 0: return __builtin_amdgcn_cvt_f32_fp8(x, 0);
+        case 1: return __builtin_amdgcn_cvt_f32_fp8(x, 1);
+        case 2: return __builtin_amdgcn_cvt_f32_fp8(x, 2);
+        default: return __builtin_amdgcn_cvt_f32_fp8(x, 3);
+        }
+    } else if constexpr (WP == SC_W_BF16) {
+        return __uint_as_float(((r.w[i >> 1] >> (16 * (i & 1))) & 0xFFFFu) << 16);
+    } else if constexpr (WP == SC_W_F32) {
+        return __uint_as_float(r.w[i]);
+    } else {
+        return 1.f;
+    }
+}
+
 // Ring row of the step `back` steps before the step whose row is `row` (back < R).
 __device__ __forceinline__ int row_back(int row, int back, int R)
 {
@@ -298,13 +344,17 @@ __device__ __forceinline__ double slice_tail(const ScSlice &sl, gptr<uint32_t> r
             else raw[h].w[0] = 0u;
         }
         if constexpr (IDX16) {
-            gptr<uint32_t> col = gbl(static_cast<const uint32_t *>(sl.col));
-            const int last = (width - 1) >> 1;
-            // the batch's pair headers in one load (lane j: pair j), read back with readlane
-            const uint32_t hv = gbl(sl.hdr)[min((k >> 1) + min(lane, 3), last)];
+            // pairs 2g, 2g+1 of a lane in one 8-byte load; pair headers (stored for
+            // the even-padded pair count) in one load, read back with readlane
+            gptr<u32x2> col = gbl(static_cast<const u32x2 *>(sl.col));
+            const uint32_t hv = gbl(sl.hdr)[min((k >> 1) + min(lane, 3), 2 * nv - 1)];
             uint32_t pw[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) pw[j] = col[min((k >> 1) + j, last) * SC_WAVE + lane];
+            for (int h = 0; h < 2; ++h) {
+                const u32x2 x = col[min((k >> 2) + h, nv - 1) * SC_WAVE + lane];
+                pw[2 * h] = x.x;
+                pw[2 * h + 1] = x.y;
+            }
 #pragma unroll
             for (int j = 0; j < 8; ++j) c[j] = pair_bit(pw[j >> 1], j & 1, __builtin_amdgcn_readlane(hv, j >> 1), zb);
         } else {
@@ -368,11 +418,11 @@ __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance
         raw[s].w[0] = 0u;
         if (WP != SC_W_UNIT && val && width > 0) raw[s] = wraw4<WP>(val, lane);
         if constexpr (IDX16) {
-            gptr<uint32_t> col = gbl(static_cast<const uint32_t *>(t.sl[s].col));
-#pragma unroll
-            for (int p = 0; p < 2; ++p) {
-                pw[s][p] = 0xFFFFFFFFu;
-                if (2 * p < width) pw[s][p] = col[p * SC_WAVE + lane];
+            pw[s][0] = pw[s][1] = 0xFFFFFFFFu;
+            if (width > 0) {
+                const u32x2 x = gbl(static_cast<const u32x2 *>(t.sl[s].col))[lane];
+                pw[s][0] = x.x;
+                pw[s][1] = x.y;
             }
         } else {
             c[s][0] = c[s][1] = c[s][2] = c[s][3] = zb;
@@ -433,10 +483,10 @@ __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance
 // LDS accumulator of the projection: the sums are exact, so the order of the
 // additions cannot change them.
 //
-// In the 16-bit form the row is a sequence of 64-entry groups (one wavefront
-// load each) with two wave-uniform header words (projection slot and base bit of
-// the group's two segments); an entry is a 15-bit offset whose bit 15 selects the
-// segment.  Padding entries (SC_PAD16) read the ring row's zero word (bit zb).
+// In the 16-bit form the row is a sequence of 8-entry chunks, each with one
+// header word (projection slot and base bit) and 8 16-bit offsets from the base;
+// padding entries (SC_PAD16) read the ring row's zero word (bit zb) or, staged,
+// LDS word 0.
 template <int WP, int U, bool IDX16, bool STAGED>
 __device__ __forceinline__ void team_gather(const ScTeamTask &T, gptr<uint32_t> ring32, const uint32_t *win,
                                             const uint32_t *rowoff, uint32_t zb, int tid, int TT, double *acc)
@@ -444,37 +494,28 @@ __device__ __forceinline__ void team_gather(const ScTeamTask &T, gptr<uint32_t> 
     const int len = T.len;
     const void *val = T.val;
     if constexpr (IDX16) {
-        gptr<uint16_t> col = gbl(static_cast<const uint16_t *>(T.col));
+        // 16-bit form: T.len chunks of 8 entries; a lane takes one chunk a batch
+        // (one 16-byte load of offsets, its header word and its multipliers)
+        gptr<u32x4> col = gbl(static_cast<const u32x4 *>(T.col));
         gptr<uint32_t> hdr = gbl(T.hdr);
-        const int lane = tid & (SC_WAVE - 1), ng = len / SC_WAVE, gstep = TT / SC_WAVE;
-        for (int b = tid; b < len; b += TT * U) {
-            const int g0 = uniform((b - lane) / SC_WAVE);  // group of this wavefront's first entry
-            uint32_t c[U], raw[U], s[U];
-            // the 2U header words of the batch in one load: lane 2u+w holds word w
-            // of group u (lanes past 2U repeat the last group), read back per group
-            // with readlane
-            const uint32_t hv = hdr[2 * min(g0 + min(lane >> 1, U - 1) * gstep, ng - 1) + (lane & 1)];
+        for (int b = tid; b < len; b += TT) {
+            const u32x4 x = col[b];
+            const uint32_t h = hdr[b];
+            const Raw8<WP> raw = wraw8<WP>(val, b);
+            const uint32_t base = h & SC_BIT_MASK, ro = rowoff[h >> SC_Q_SHIFT];
+            uint32_t bit[8], s[8];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int g = min(g0 + u * gstep, ng - 1);
-                c[u] = col[g * SC_WAVE + lane];
-                raw[u] = wraw<WP>(val, g * SC_WAVE + lane);
+            for (int i = 0; i < 8; ++i) {
+                const uint32_t o = (x[i >> 1] >> (16 * (i & 1))) & 0xFFFFu;
+                const bool pad = o == SC_PAD16;
+                bit[i] = pad ? zb : base + o;  // padding reads the zero word
+                const uint32_t wi = ro + (bit[i] >> 5);
+                if constexpr (STAGED) s[i] = win[pad ? 0u : wi];
+                else s[i] = ring32[wi];
             }
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const uint32_t h0 = __builtin_amdgcn_readlane(hv, 2 * u), h1 = __builtin_amdgcn_readlane(hv, 2 * u + 1);
-                const uint32_t o = c[u];
-                const uint32_t hs = (o & 0x8000u) ? h1 : h0;
-                const uint32_t pad = 0u - (uint32_t)(o == SC_PAD16);  // all ones for padding
-                c[u] = ((hs + (o & 0x7FFFu)) & ~pad) | (((h0 & ~SC_BIT_MASK) | zb) & pad);
-                const uint32_t wi = rowoff[c[u] >> SC_Q_SHIFT] + ((c[u] & SC_BIT_MASK) >> 5);
-                if constexpr (STAGED) s[u] = win[pad ? 0u : wi];  // padding reads the zero word
-                else s[u] = ring32[wi];
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-                if (g0 + u * gstep < ng && ((s[u] >> (c[u] & 31)) & 1u))
-                    atomicAdd(&acc[c[u] >> SC_Q_SHIFT], (double)wdec<WP>(raw[u]));
+            for (int i = 0; i < 8; ++i)
+                if ((s[i] >> (bit[i] & 31)) & 1u) atomicAdd(&acc[h >> SC_Q_SHIFT], (double)wval8<WP>(raw, i));
         }
     } else {
         // 32-bit form: rows padded to whole 16-byte vectors; each lane loads 4

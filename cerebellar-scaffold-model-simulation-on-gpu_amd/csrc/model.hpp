@@ -74,9 +74,10 @@ inline int64_t sell_pos(int64_t off, int k, int l) { return off + (int64_t)(k >>
 struct HostProjArrays {
     std::vector<uint32_t> col;
     // 16-bit form of lane slices: the offsets of columns 2p and 2p+1 of a lane
-    // share one 32-bit word (low half: column 2p) at word p*64 + lane of the
-    // slice, both relative to the pair header hdr (base device bit); 0xFFFF marks
-    // padding.  sell_off16: start of each slice in 32-bit words.
+    // share one 32-bit word (low half: column 2p), both relative to the pair
+    // header hdr[p] (base device bit); the words of pairs 2g and 2g+1 of a lane
+    // are adjacent, at word (g*64 + lane)*2 of the slice; 0xFFFF marks padding.
+    // sell_off16: start of each slice in 32-bit words.
     std::vector<uint16_t> col16;
     std::vector<uint32_t> hdr;
     std::vector<int64_t> sell_off16;
@@ -101,9 +102,9 @@ struct TeamPop {
     std::vector<int32_t> seg_end;   // num x n_in: end of each projection's segment, relative to the row
     std::vector<uint32_t> col;      // device bit indices
     std::vector<uint8_t> val;       // encoded multipliers
-    // 16-bit form: rows of 64-entry groups of at most two segments; two header
-    // words per group (projection slot in the top 3 bits, base device bit); an
-    // entry is a 15-bit offset, bit 15 selecting the segment (0xFFFF: padding)
+    // 16-bit form: rows of 8-entry chunks of one projection each; one header word
+    // per chunk (projection slot in the top 3 bits, base device bit); an entry is
+    // its 16-bit offset from the base (0xFFFF: padding); row_ptr counts chunks
     bool idx16 = false;
     std::vector<uint16_t> col16;
     std::vector<uint32_t> hdr;

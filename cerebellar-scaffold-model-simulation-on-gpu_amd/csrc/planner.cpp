@@ -359,7 +359,9 @@ int plan_build(scaffold_env *env, Plan &P)
             }
     }
     // 16-bit form of lane-mode slices, per post population: columns are taken in
-    // pairs (2p, 2p+1), whose 128 entries must span fewer than 65,535 device bits
+    // pairs (2p, 2p+1), whose 128 entries must span fewer than 65,535 device bits.
+    // A lane's offsets of pairs 2g and 2g+1 (columns 4g .. 4g+3) are adjacent 32-bit
+    // words (one 8-byte load); the pair count is padded to even.
     if (want16) {
         struct Conv {
             std::vector<uint16_t> c16;
@@ -379,7 +381,7 @@ int plan_build(scaffold_env *env, Plan &P)
                 int64_t words = 0;
                 for (size_t gi = 0; gi < ng; ++gi) {
                     C.off16[gi] = words;
-                    words += (int64_t)(A.sell_w[gi] + 1) / 2 * SC_WAVE;
+                    words += (int64_t)(A.sell_w[gi] + 3) / 4 * 2 * SC_WAVE;
                 }
                 C.c16.assign(words * 2, (uint16_t)SC_PAD16);
                 C.h.assign(words / SC_WAVE, 0);
@@ -396,12 +398,12 @@ int plan_build(scaffold_env *env, Plan &P)
                             }
                         if (lo == UINT32_MAX) continue;
                         if (hi - lo >= SC_PAD16) { ok = false; break; }
-                        const int64_t wbase = C.off16[gi] + (int64_t)pp * SC_WAVE;
-                        C.h[wbase / SC_WAVE] = lo;
+                        C.h[C.off16[gi] / SC_WAVE + pp] = lo;
                         for (int k = 2 * pp; k < std::min(w, 2 * pp + 2); ++k)
                             for (int l = 0; l < SC_WAVE; ++l) {
                                 const uint32_t b = A.col[sell_pos(A.sell_off[gi], k, l)];
-                                if (b != sentinel) C.c16[(wbase + l) * 2 + (k & 1)] = (uint16_t)(b - lo);
+                                const int64_t word = C.off16[gi] + ((int64_t)(pp >> 1) * SC_WAVE + l) * 2 + (pp & 1);
+                                if (b != sentinel) C.c16[word * 2 + (k & 1)] = (uint16_t)(b - lo);
                             }
                     }
                 }
@@ -481,17 +483,18 @@ int plan_build(scaffold_env *env, Plan &P)
                 }
             }
         if (want16) {
-            // 16-bit form: each row is a sequence of 64-entry groups (one wavefront
-            // load each).  A group holds at most two segments -- runs of entries of
-            // one projection, ascending, spanning fewer than 32,767 device bits --
-            // and two header words, one per segment: q << 29 | base bit.  An entry
-            // is its 15-bit offset from its segment's base, bit 15 set in the second
-            // segment; 0xFFFF is padding (multiplier 0).
+            // 16-bit form: each row is a sequence of 8-entry chunks -- one 16-byte
+            // load of offsets per lane -- with one 32-bit header per chunk
+            // (q << 29 | base device bit).  A chunk holds entries of one projection,
+            // ascending, spanning fewer than 65,535 bits; an entry is its offset
+            // from the base; 0xFFFF is padding (multiplier 0).  row_ptr counts
+            // chunks.
+            constexpr int CH = 8;
             std::vector<int64_t> rp(d.num + 1, 0);
             std::vector<uint16_t> c16;
             std::vector<uint32_t> h;
             std::vector<uint8_t> v16;
-            c16.reserve(tot + tot / 8);
+            c16.reserve(tot + tot / 4);
             std::vector<int64_t> perm;
             for (int j = 0; j < d.num; ++j) {
                 // entries of each projection segment in ascending bit order (the
@@ -501,28 +504,20 @@ int plan_build(scaffold_env *env, Plan &P)
                 std::stable_sort(perm.begin(), perm.end(), [&](int64_t x, int64_t y) { return T.col[x] < T.col[y]; });
                 size_t e = 0;
                 while (e < perm.size()) {
-                    const int64_t g0 = (int64_t)c16.size();
-                    c16.resize(g0 + SC_WAVE, (uint16_t)SC_PAD16);
-                    if (wb) v16.resize((g0 + SC_WAVE) * wb, 0);
-                    h.resize(h.size() + 2, 0);
-                    uint32_t *hg = &h[h.size() - 2];
-                    int n = 0;
-                    for (int seg = 0; seg < 2 && e < perm.size() && n < SC_WAVE; ++seg) {
-                        const uint32_t first = T.col[perm[e]];
-                        const uint32_t q = first >> SC_Q_SHIFT, lo = first & SC_BIT_MASK;
-                        hg[seg] = first;
-                        if (seg == 0) hg[1] = first;
-                        while (e < perm.size() && n < SC_WAVE) {
-                            const uint32_t c = T.col[perm[e]];
-                            if ((c >> SC_Q_SHIFT) != q || (c & SC_BIT_MASK) - lo >= SC_OFF15_PAD) break;
-                            c16[g0 + n] = (uint16_t)(((c & SC_BIT_MASK) - lo) | (seg ? 0x8000u : 0u));
-                            if (wb) memcpy(&v16[(g0 + n) * wb], &T.val[perm[e] * wb], wb);
-                            ++n;
-                            ++e;
-                        }
+                    const int64_t c0 = (int64_t)c16.size();
+                    c16.resize(c0 + CH, (uint16_t)SC_PAD16);
+                    if (wb) v16.resize((c0 + CH) * wb, 0);
+                    const uint32_t first = T.col[perm[e]];
+                    const uint32_t q = first >> SC_Q_SHIFT, lo = first & SC_BIT_MASK;
+                    h.push_back(first);
+                    for (int n = 0; n < CH && e < perm.size(); ++n, ++e) {
+                        const uint32_t c = T.col[perm[e]];
+                        if ((c >> SC_Q_SHIFT) != q || (c & SC_BIT_MASK) - lo >= SC_PAD16) break;
+                        c16[c0 + n] = (uint16_t)((c & SC_BIT_MASK) - lo);
+                        if (wb) memcpy(&v16[(c0 + n) * wb], &T.val[perm[e] * wb], wb);
                     }
                 }
-                rp[j + 1] = (int64_t)c16.size();
+                rp[j + 1] = (int64_t)h.size();
             }
             T.idx16 = true;
             T.row_ptr = std::move(rp);
