@@ -131,15 +131,19 @@ def batched(pkg, n_inst, steps, warmup, threads):
         except Exception as ex:  # surfaced below
             errs.append(ex)
 
-    # autotune the layout on one instance of the reference size and use it for all
+    # autotune the layout on one instance of the largest scale (4x; the 4x
+    # instances hold 42 % of the batch's synapses, and at that size the step is
+    # throughput-bound like the batch) and use it for all
     probe = pkg.NetworkEnvironment(1, 7, 16, 2, 0)
     cfg = pkg.synth_preset(1)
+    cfg.scale = max(scales)
+    cfg.burst_n_mf = max(1, int(round(50 * cfg.scale)))
     cfg.mf_rate_hz = float(np.median(rates))
     probe.describe_synthetic(cfg)
     probe.set_option("weight_precision", pkg.WEIGHTS_BF16)
-    tuned = probe.autotune(200)
+    tuned = probe.autotune(100)
     probe.finalize()
-    layout = {k: tuned[k] for k in ("lane_max_fanin", "lane_synapses", "graph_steps")}
+    layout = {k: tuned[k] for k in ("lane_max_fanin", "lane_synapses", "graph_steps", "index_bits", "stage_spikes")}
 
     order = sorted(range(len(scales)), key=lambda i: -scales[i])
     parts = [order[k::threads] for k in range(threads)]
@@ -167,6 +171,29 @@ def batched(pkg, n_inst, steps, warmup, threads):
             "neurons": int(neurons), "synapses": int(syn), "steps": steps, "ms_per_step": wall * 1e3 / steps,
             "gpu_ms_per_step": ms / steps, "dtype": "bf16 multipliers, f32 state",
             "aggregate_real_time_factor": len(scales) * steps * 1e-3 / wall, "autotuned": tuned}
+
+
+def plain_variant(pkg, steps, warmup, **options):
+    """The plain FP32 96.7k scaffold with other storage options (e.g. 16-bit
+    column indices): same results bit for bit, other memory traffic."""
+    env = make_env(pkg, 1)
+    for k, v in options.items():
+        env.set_option(k, v)
+    env.initialize(50, 25)
+    info = env.info()
+    advance(env, 0, warmup)
+    _, ev0 = env.totals()
+    t0 = time.perf_counter()
+    env.time_steps(steps)
+    wall = time.perf_counter() - t0
+    _, ev1 = env.totals()
+    _, kmean = env.time_steps(100, kernel_mean=True)
+    algo = info.algorithmic_bytes_per_step
+    out = {"options": options, "value": (ev1 - ev0) / wall, "ms_per_step": wall * 1e3 / steps,
+           "real_time_factor": steps * 1e-3 / wall, "kernel_mean_ms": kmean, "algorithmic_bytes_per_step": algo,
+           "achieved_GBs": algo / (kmean * 1e-3) / 1e9, "synapses_16bit": info.synapses_16bit}
+    env.finalize()
+    return out
 
 
 def scaled(pkg, preset, steps, precisions):
@@ -280,6 +307,7 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic: seeded scaffold generator (reference population sizes, rule-based connectivity)",
         "config": {"workload": "scaffold-96k plain fp32 (1 s = 1000 exchange steps of 10 x 0.1 ms)",
+                   "column_index_bits": 32,
                    "neurons": info.total_neurons, "synapses": info.total_synapses,
                    "simulated_ms": args.steps, "input": "4 Hz mossy Poisson + 100 Hz burst to 50 fibres, 300-350 ms"},
         "real_time_factor": rtf,
@@ -295,6 +323,8 @@ def main():
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(env_for_cpu, args.cpu_steps, args.cpu_threads)
     env.finalize()
+    if not args.no_batched:  # the same network with 16-bit column offsets (bit-identical results)
+        line["plain_fp32_idx16"] = plain_variant(pkg, args.steps, args.warmup, index_bits=16)
     if not args.no_batched:
         b = batched(pkg, args.batch_instances, args.batch_steps, 50, threads=min(16, os.cpu_count() or 4))
         b["vs_plain"] = b["value"] / value

@@ -908,20 +908,28 @@ int scaffold_autotune(scaffold_env *env, int n_steps, scaffold_tune_result *out)
     scaffold_tune_result r{};
     r.ms_per_step = 1e300;
     r.ms_default = -1.0;
-    const int fanins[] = {16, 48, 128}, synapses[] = {8, 16, 32}, graphs[] = {25, 50};
+    const int fanins[] = {16, 48, 128}, synapses[] = {8, 16, 32};
+    r.index_bits = saved.index_bits;
+    r.stage_spikes = saved.stage_spikes;
+    // the winner so far as options, with one field changed
+    auto trial = [&](auto &&change, double *ms) {
+        env->opt = saved;
+        env->opt.lane_max_fanin = r.lane_max_fanin;
+        env->opt.lane_synapses = r.lane_synapses;
+        env->opt.graph_steps = r.graph_steps;
+        env->opt.index_bits = r.index_bits;
+        env->opt.stage_spikes = r.stage_spikes;
+        change(env->opt);
+        const int trc = trial_time(env, T_print, n_steps, ms);
+        env->opt = saved;
+        if (!trc) ++r.n_tried;
+        return trc;
+    };
     for (int lf : fanins)
         for (int ls : synapses) {
-            env->opt = saved;
-            env->opt.lane_max_fanin = lf;
-            env->opt.lane_synapses = ls;
-            env->opt.graph_steps = 25;
             double ms = 0.0;
-            rc = trial_time(env, T_print, n_steps, &ms);
-            if (rc) {
-                env->opt = saved;
+            if ((rc = trial([&](Options &o) { o.lane_max_fanin = lf; o.lane_synapses = ls; o.graph_steps = 25; }, &ms)))
                 return rc;
-            }
-            ++r.n_tried;
             if (lf == Options().lane_max_fanin && ls == Options().lane_synapses) r.ms_default = ms;
             if (ms < r.ms_per_step) {
                 r.ms_per_step = ms;
@@ -930,28 +938,21 @@ int scaffold_autotune(scaffold_env *env, int n_steps, scaffold_tune_result *out)
                 r.graph_steps = 25;
             }
         }
-    for (int gs : graphs) {
-        if (gs == 25) continue;
-        env->opt = saved;
-        env->opt.lane_max_fanin = r.lane_max_fanin;
-        env->opt.lane_synapses = r.lane_synapses;
-        env->opt.graph_steps = gs;
-        double ms = 0.0;
-        rc = trial_time(env, T_print, n_steps, &ms);
-        if (rc) {
-            env->opt = saved;
-            return rc;
-        }
-        ++r.n_tried;
-        if (ms < r.ms_per_step) {
-            r.ms_per_step = ms;
-            r.graph_steps = gs;
-        }
-    }
+    double ms = 0.0;
+    const int other_bits = r.index_bits == 16 ? 32 : 16;
+    if ((rc = trial([&](Options &o) { o.index_bits = other_bits; }, &ms))) return rc;
+    if (ms < r.ms_per_step) { r.ms_per_step = ms; r.index_bits = other_bits; }
+    const int other_stage = r.stage_spikes ? 0 : 1;
+    if ((rc = trial([&](Options &o) { o.stage_spikes = other_stage; }, &ms))) return rc;
+    if (ms < r.ms_per_step) { r.ms_per_step = ms; r.stage_spikes = other_stage; }
+    if ((rc = trial([&](Options &o) { o.graph_steps = 50; }, &ms))) return rc;
+    if (ms < r.ms_per_step) { r.ms_per_step = ms; r.graph_steps = 50; }
     env->opt = saved;
     env->opt.lane_max_fanin = r.lane_max_fanin;
     env->opt.lane_synapses = r.lane_synapses;
     env->opt.graph_steps = r.graph_steps;
+    env->opt.index_bits = r.index_bits;
+    env->opt.stage_spikes = r.stage_spikes;
     *out = r;
     return SCAFFOLD_OK;
 }

@@ -334,13 +334,16 @@ int scaffold_time_steps(scaffold_env *env, int n_steps, double *ms_total,
  * one lane per neuron over sliced-ELL slices, i.e. the ELL width) in {16, 48, 128}
  * and lane_synapses (synapses per lane, i.e. wavefronts per team-mode neuron) in
  * {8, 16, 32} -- times n_steps step kernels of each after 50 warm-up steps, then
- * times graph_steps in {25, 50} for the fastest layout, and sets the winning
- * options on env.  The network's own state, seeds and counters are untouched. */
+ * tries, one at a time on the fastest layout, the other column index width
+ * (index_bits 32 / 16), the other stage_spikes setting and graph_steps 50, and
+ * sets the winning options on env (12 trials).  The network's own state, seeds
+ * and counters are untouched. */
 typedef struct {
     int32_t lane_max_fanin, lane_synapses, graph_steps;
     int32_t n_tried;
     double ms_per_step;     /* of the winner, graph replays included */
     double ms_default;      /* of the default options, for comparison */
+    int32_t index_bits, stage_spikes;
 } scaffold_tune_result;
 int scaffold_autotune(scaffold_env *env, int n_steps, scaffold_tune_result *out);
 
