@@ -331,7 +331,9 @@ def main():
         b["vs_plain_synapses_scanned"] = b["synapses_scanned_per_s"] / line["synapses_scanned_per_s"]
         line["batched_bf16"] = b
     if not args.no_16x:
-        line["scaffold16x"] = scaled(pkg, 2, args.steps_16x, (pkg.WEIGHTS_F32, pkg.WEIGHTS_BF16, pkg.WEIGHTS_FP8))
+        line["scaffold16x"] = scaled(pkg, 2, args.steps_16x,
+                                     (pkg.WEIGHTS_F32, pkg.WEIGHTS_BF16, pkg.WEIGHTS_FP8, (pkg.WEIGHTS_F32, 16),
+                                      (pkg.WEIGHTS_BF16, 16), (pkg.WEIGHTS_FP8, 16)))
     if not args.no_100x:
         line["scaffold100x"] = scaled(pkg, 4, args.steps_100x,
                                       ((pkg.WEIGHTS_FP8, 16), pkg.WEIGHTS_FP8, (pkg.WEIGHTS_F32, 16), pkg.WEIGHTS_F32))
