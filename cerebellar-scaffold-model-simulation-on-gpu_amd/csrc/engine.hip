@@ -418,6 +418,7 @@ int engine_create(Engine **out, Plan &P)
     if ((rc = upload_work(A, B, e->w, e->h_team_t, e->h_lanes, e->h_stims, e->h_teams, e->h_lane_win, e->h_team_win,
                           P.stage != 0)))
         return bail(rc);
+    e->w.max_groups = P.n_groups;
     if ((rc = dalloc(A, B, &e->cnt_buf, 1))) return bail(rc);
     std::vector<ScInstance> one{I};
     if ((rc = dupload(A, B, &e->di, one))) return bail(rc);
@@ -490,7 +491,7 @@ static void launch_run(hipStream_t s, const ScInstance *single, const ScInstance
                        const DevWork &w, int K)
 {
     for (int k = 0; k < K; ++k) launch_step(s, single, insts, w, k);
-    launch_stats(s, single, insts, n_inst, K);
+    launch_stats(s, single, insts, n_inst, K, w.max_groups);
     launch_advance(s, insts, n_inst, K);
 }
 
@@ -564,7 +565,7 @@ int engine_time(Engine *e, int n_steps, int use_graph, int graph_steps, double *
             HIP_TRY(hipEventRecord(s0, e->stream));
             for (int k = 0; k < K; ++k) launch_step(e->stream, &e->hi, e->di, e->w, k);
             HIP_TRY(hipEventRecord(s1, e->stream));
-            launch_stats(e->stream, &e->hi, e->di, 1, K);
+            launch_stats(e->stream, &e->hi, e->di, 1, K, e->w.max_groups);
             launch_advance(e->stream, e->di, 1, K);
             done += K;
         }
@@ -758,6 +759,7 @@ int batch_build(Batch **out, std::vector<Engine *> &engines, const std::vector<c
     if ((rc = dupload(b->allocs, nullptr, &b->di, insts))) return bail(rc);
     if ((rc = upload_work(b->allocs, nullptr, b->w, tw, lanes, stims, tt, lane_win, team_win, stage != 0)))
         return bail(rc);
+    for (const ScInstance &x : insts) b->w.max_groups = std::max(b->w.max_groups, x.n_groups);
     b->n_inst = (int)insts.size();
     *out = b;
     return SCAFFOLD_OK;

@@ -872,22 +872,39 @@ __global__ void __launch_bounds__(SC_BLOCK) __attribute__((amdgpu_waves_per_eu(8
     step_body<false, IDX16>(ScInstance{}, insts, L, lanes, stt, teams, stages, k);
 }
 
-// Per-population spike counts and synaptic events of K consecutive steps: one
-// block per (instance, step) popcounts the spike words of that step's ring row
-// and sums the out-degrees of the set bits; the results go to the count / event
-// rows and the running totals.
+// Per-population spike counts and synaptic events of K consecutive steps.  The
+// ring row of each (instance, step) is cut into chunks of SC_STATS_CHUNK spike
+// words, one block each: popcounts per population and the out-degrees of the set
+// bits, summed in LDS, then added with integer atomics to the row's counts and
+// events (zeroed first by k_stats_zero) and to the running totals.  Integer sums:
+// the result does not depend on the order of the blocks.
+#define SC_STATS_CHUNK 2048
 template <bool SINGLE>
-__global__ void __launch_bounds__(256) k_stats(ScInstance I0, const ScInstance *__restrict__ insts, int K)
+__global__ void __launch_bounds__(256) k_stats_zero(ScInstance I0, const ScInstance *__restrict__ insts, int K)
 {
-    __shared__ unsigned long long acc[SC_MAX_POPS + 1];
     const int inst = blockIdx.x / K, kk = blockIdx.x - inst * K;
     const ScInstance &I = inst_of<SINGLE>(I0, insts, inst);
+    const int row = (I.ctl->row + kk) % I.R;
+    if ((int)threadIdx.x < I.n_pops) I.counts[(int64_t)row * I.n_pops + threadIdx.x] = 0;
+    if (threadIdx.x == 0) I.events[row] = 0ull;
+}
+
+template <bool SINGLE>
+__global__ void __launch_bounds__(256) k_stats(ScInstance I0, const ScInstance *__restrict__ insts, int K, int nchunk)
+{
+    __shared__ unsigned long long acc[SC_MAX_POPS + 1];
+    const int chunk = blockIdx.x % nchunk, rest = blockIdx.x / nchunk;
+    const int inst = rest / K, kk = rest - inst * K;
+    const ScInstance &I = inst_of<SINGLE>(I0, insts, inst);
+    const int g0 = chunk * SC_STATS_CHUNK;
+    if (g0 >= I.n_groups) return;  // the whole block: this instance has fewer groups
+    const int g1 = min(I.n_groups, g0 + SC_STATS_CHUNK);
     if (threadIdx.x <= SC_MAX_POPS) acc[threadIdx.x] = 0ull;
     __syncthreads();
     const int row = (I.ctl->row + kk) % I.R;
     const uint64_t *__restrict__ w = I.ring + (int64_t)row * I.Wr;
     unsigned long long ev = 0ull;
-    for (int g = threadIdx.x; g < I.n_groups; g += blockDim.x) {
+    for (int g = g0 + threadIdx.x; g < g1; g += blockDim.x) {
         unsigned long long word = w[g];
         if (!word) continue;
         atomicAdd(&acc[I.groups[g].pop], (unsigned long long)__popcll(word));
@@ -900,13 +917,13 @@ __global__ void __launch_bounds__(256) k_stats(ScInstance I0, const ScInstance *
     if (ev) atomicAdd(&acc[SC_MAX_POPS], ev);
     __syncthreads();
     const int p = threadIdx.x;
-    if (p < I.n_pops) {
-        I.counts[(int64_t)row * I.n_pops + p] = (int32_t)acc[p];
-        if (acc[p]) atomicAdd(&I.totals[p], acc[p]);
+    if (p < I.n_pops && acc[p]) {
+        atomicAdd(&I.counts[(int64_t)row * I.n_pops + p], (int32_t)acc[p]);
+        atomicAdd(&I.totals[p], acc[p]);
     }
-    if (p == 0) {
-        I.events[row] = acc[SC_MAX_POPS];
-        if (acc[SC_MAX_POPS]) atomicAdd(&I.totals[I.n_pops], acc[SC_MAX_POPS]);
+    if (p == 0 && acc[SC_MAX_POPS]) {
+        atomicAdd(&I.events[row], acc[SC_MAX_POPS]);
+        atomicAdd(&I.totals[I.n_pops], acc[SC_MAX_POPS]);
     }
 }
 
@@ -1020,11 +1037,18 @@ void launch_step(hipStream_t s, const ScInstance *single, const ScInstance *inst
         k_step_batch<false><<<w.n_blocks, SC_BLOCK, 0, s>>>(insts, w.layout, w.lanes, w.stims, w.teams, w.stages, k);
 }
 
-void launch_stats(hipStream_t s, const ScInstance *single, const ScInstance *insts, int n_inst, int K)
+void launch_stats(hipStream_t s, const ScInstance *single, const ScInstance *insts, int n_inst, int K,
+                  int max_groups)
 {
     if (K <= 0 || n_inst <= 0) return;
-    if (single) k_stats<true><<<K, 256, 0, s>>>(*single, insts, K);
-    else k_stats<false><<<K * n_inst, 256, 0, s>>>(ScInstance{}, insts, K);
+    const int nchunk = std::max(1, (max_groups + SC_STATS_CHUNK - 1) / SC_STATS_CHUNK);
+    if (single) {
+        k_stats_zero<true><<<K, 256, 0, s>>>(*single, insts, K);
+        k_stats<true><<<K * nchunk, 256, 0, s>>>(*single, insts, K, nchunk);
+    } else {
+        k_stats_zero<false><<<K * n_inst, 256, 0, s>>>(ScInstance{}, insts, K);
+        k_stats<false><<<K * n_inst * nchunk, 256, 0, s>>>(ScInstance{}, insts, K, nchunk);
+    }
 }
 
 void launch_advance(hipStream_t s, const ScInstance *insts, int n_inst, int K)

@@ -23,12 +23,14 @@ struct DevWork {
     bool idx16 = false;  // some descriptor holds 16-bit column offsets
     const ScStage *stages = nullptr;  // per block: spike windows staged in LDS (nullptr: none)
     int n_staged = 0;                 // blocks with staged windows
+    int max_groups = 0;               // most 64-neuron groups of one instance (statistics grid)
 };
 
 // All launchers are asynchronous on stream s.  `single` != nullptr selects the
 // kernels that take one instance by value; otherwise instances come from `insts`.
 void launch_step(hipStream_t s, const ScInstance *single, const ScInstance *insts, const DevWork &w, int k);
-void launch_stats(hipStream_t s, const ScInstance *single, const ScInstance *insts, int n_inst, int K);
+void launch_stats(hipStream_t s, const ScInstance *single, const ScInstance *insts, int n_inst, int K,
+                  int max_groups);
 void launch_advance(hipStream_t s, const ScInstance *insts, int n_inst, int K);
 void launch_set_ctl(hipStream_t s, ScCtl *ctl, int64_t n, int64_t m, int row);
 void launch_init_params(hipStream_t s, const ScInstance *di, int n_threads, uint64_t seed, uint64_t off);
