@@ -247,7 +247,11 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--kernel-steps", type=int, default=100)
     ap.add_argument("--only-scaled", type=int, default=0, choices=(0, 2, 4),
-                    help="only run the tiled scaffold of this preset (2: 16x, 4: 100x) with FP32 and exit")
+                    help="only run the tiled scaffold of this preset (2: 16x, 4: 100x) and exit")
+    ap.add_argument("--scaled-precision", default="fp32", choices=("fp32", "bf16", "fp8"),
+                    help="multipliers of the --only-scaled run")
+    ap.add_argument("--scaled-index-bits", type=int, default=32, choices=(32, 16),
+                    help="column index width of the --only-scaled run")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run the calibration copy kernel on this many MiB and exit")
     args = ap.parse_args()
@@ -258,7 +262,8 @@ def main():
     import __graft_entry__ as ge
     pkg = ge.load_package()
     if args.only_scaled:
-        print(json.dumps(scaled(pkg, args.only_scaled, args.steps, (pkg.WEIGHTS_F32,))))
+        prec = {"fp32": pkg.WEIGHTS_F32, "bf16": pkg.WEIGHTS_BF16, "fp8": pkg.WEIGHTS_FP8}[args.scaled_precision]
+        print(json.dumps(scaled(pkg, args.only_scaled, args.steps, ((prec, args.scaled_index_bits),))))
         return
     if args.copy_probe_mb:
         ms = pkg.copy_probe(args.copy_probe_mb << 20, 20)
