@@ -877,7 +877,8 @@ __global__ void __launch_bounds__(SC_BLOCK) __attribute__((amdgpu_waves_per_eu(8
 // words, one block each: popcounts per population and the out-degrees of the set
 // bits, summed in LDS, then added with integer atomics to the row's counts and
 // events (zeroed first by k_stats_zero) and to the running totals.  Integer sums:
-// the result does not depend on the order of the blocks.
+// the result does not depend on the order of the blocks.  With one chunk per row
+// (nchunk 1) the block writes the row's counts and events directly, unzeroed.
 #define SC_STATS_CHUNK 2048
 template <bool SINGLE>
 __global__ void __launch_bounds__(256) k_stats_zero(ScInstance I0, const ScInstance *__restrict__ insts, int K)
@@ -897,7 +898,7 @@ __global__ void __launch_bounds__(256) k_stats(ScInstance I0, const ScInstance *
     const int inst = rest / K, kk = rest - inst * K;
     const ScInstance &I = inst_of<SINGLE>(I0, insts, inst);
     const int g0 = chunk * SC_STATS_CHUNK;
-    if (g0 >= I.n_groups) return;  // the whole block: this instance has fewer groups
+    if (chunk > 0 && g0 >= I.n_groups) return;  // the whole block: this instance has fewer groups
     const int g1 = min(I.n_groups, g0 + SC_STATS_CHUNK);
     if (threadIdx.x <= SC_MAX_POPS) acc[threadIdx.x] = 0ull;
     __syncthreads();
@@ -917,13 +918,15 @@ __global__ void __launch_bounds__(256) k_stats(ScInstance I0, const ScInstance *
     if (ev) atomicAdd(&acc[SC_MAX_POPS], ev);
     __syncthreads();
     const int p = threadIdx.x;
-    if (p < I.n_pops && acc[p]) {
-        atomicAdd(&I.counts[(int64_t)row * I.n_pops + p], (int32_t)acc[p]);
-        atomicAdd(&I.totals[p], acc[p]);
+    if (p < I.n_pops) {
+        if (nchunk == 1) I.counts[(int64_t)row * I.n_pops + p] = (int32_t)acc[p];
+        else if (acc[p]) atomicAdd(&I.counts[(int64_t)row * I.n_pops + p], (int32_t)acc[p]);
+        if (acc[p]) atomicAdd(&I.totals[p], acc[p]);
     }
-    if (p == 0 && acc[SC_MAX_POPS]) {
-        atomicAdd(&I.events[row], acc[SC_MAX_POPS]);
-        atomicAdd(&I.totals[I.n_pops], acc[SC_MAX_POPS]);
+    if (p == 0) {
+        if (nchunk == 1) I.events[row] = acc[SC_MAX_POPS];
+        else if (acc[SC_MAX_POPS]) atomicAdd(&I.events[row], acc[SC_MAX_POPS]);
+        if (acc[SC_MAX_POPS]) atomicAdd(&I.totals[I.n_pops], acc[SC_MAX_POPS]);
     }
 }
 
@@ -1043,10 +1046,10 @@ void launch_stats(hipStream_t s, const ScInstance *single, const ScInstance *ins
     if (K <= 0 || n_inst <= 0) return;
     const int nchunk = std::max(1, (max_groups + SC_STATS_CHUNK - 1) / SC_STATS_CHUNK);
     if (single) {
-        k_stats_zero<true><<<K, 256, 0, s>>>(*single, insts, K);
+        if (nchunk > 1) k_stats_zero<true><<<K, 256, 0, s>>>(*single, insts, K);
         k_stats<true><<<K * nchunk, 256, 0, s>>>(*single, insts, K, nchunk);
     } else {
-        k_stats_zero<false><<<K * n_inst, 256, 0, s>>>(ScInstance{}, insts, K);
+        if (nchunk > 1) k_stats_zero<false><<<K * n_inst, 256, 0, s>>>(ScInstance{}, insts, K);
         k_stats<false><<<K * n_inst * nchunk, 256, 0, s>>>(ScInstance{}, insts, K, nchunk);
     }
 }
