@@ -336,6 +336,27 @@ def test_fp8_every_code_bit_exact(pkg, orc, fanin, bits):
     env.finalize()
 
 
+@pytest.mark.parametrize("bits", [32, 16])
+def test_scaffold16x_bit_exact(pkg, orc, bits):
+    """The 16x tiled scaffold (1.55M cells, 50.7M synapses) with FP8 multipliers:
+    at this size thousands of blocks stage spike windows and rows run to
+    thousands of entries; 50 steps against the oracle (OpenMP), bit for bit."""
+    import os
+    env = new_env(pkg, 2)
+    env.set_option("weight_precision", pkg.WEIGHTS_FP8)
+    env.set_option("index_bits", bits)
+    env.initialize(50, 25)
+    info = env.info()
+    assert info.n_staged_blocks > 1000
+    assert info.synapses_16bit == (info.total_synapses if bits == 16 else 0)
+    ref = orc.OracleSim.from_env(env)
+    ref.set_threads(min(16, os.cpu_count() or 1))
+    ref.initialize()
+    run_and_compare(env, ref, 0, 50, check_counts=False)
+    assert_same_state(env, ref, f"16x, index_bits {bits}")
+    env.finalize()
+
+
 def test_unit_weights_and_reference_delays(pkg, orc):
     env = new_env(pkg, 0, weight_jitter=0, reference_delays=1)
     env.initialize(50, 25)
