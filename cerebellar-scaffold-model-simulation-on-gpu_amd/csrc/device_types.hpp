@@ -152,7 +152,7 @@ struct ScStimTask {        // 64 neurons of an INPUT population
 };
 
 struct ScTeamIn {          // one incoming projection of a team-mode neuron
-    int32_t end;           // end of its segment in the merged row
+    int32_t end;           // end of its segment in the merged row (32-bit form; informational)
     int32_t delay, exc;
     int32_t lds;           // staged blocks: LDS word that stands for word 0 of the ring row
     float W, scale;

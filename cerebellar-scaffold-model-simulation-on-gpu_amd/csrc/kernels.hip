@@ -105,17 +105,8 @@ __device__ __forceinline__ void store_state(ScState *st, int d, float u, float2 
 // branches on them are scalar branches.
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Raw bits of multiplier e in storage WP.  Loads and decoding are separate so a
-// batch of loads is issued back to back and decoded after they arrive.
-template <int WP>
-__device__ __forceinline__ uint32_t wraw(const void *val, int64_t e)
-{
-    if constexpr (WP == SC_W_F32) return gbl(static_cast<const uint32_t *>(val))[e];
-    else if constexpr (WP == SC_W_BF16) return gbl(static_cast<const uint16_t *>(val))[e];
-    else if constexpr (WP == SC_W_FP8) return gbl(static_cast<const uint8_t *>(val))[e];
-    else return 0u;
-}
-
+// Multipliers are loaded as raw bits, 4 or 8 entries per load, and decoded only
+// after a batch of loads has arrived.
 // Raw bits of the 4 multipliers of vector v (entries 4v .. 4v+3) in storage WP,
 // one load: 16 bytes (F32), 8 (BF16) or 4 (FP8).
 template <int WP>
