@@ -7,9 +7,10 @@
 //         update, spike word = wavefront ballot, plain store;
 //   TEAM  1..16 wavefronts per post neuron of a team-mode population (long rows):
 //         the team strides over the neuron's merged CSR row and accumulates the
-//         exact f64 sum of every incoming projection in LDS; one lane runs the LIF
-//         update and sets the neuron's bit in the spike word with an integer
-//         atomicOr;
+//         exact f64 sum of every incoming projection in LDS; one lane (or, for
+//         blocks of one-wavefront teams, the first wavefront for all of them) runs
+//         the LIF update and sets the neuron's bit in the spike word with an
+//         integer atomicOr;
 //   STIM  a wavefront per 64 INPUT neurons: counter-based Philox4x32-10 draws and
 //         the stimulus rule, spike word = ballot.
 // A step reads ring rows of earlier steps only (every delay >= 1) and writes the
@@ -19,14 +20,20 @@
 // neurons (synaptic events); k_advance moves the device step counter.  Runs of
 // steps are captured once as a hipGraph and replayed.
 //
-// Latency: the work per step is tens of MB, so the step time is set by chains of
-// dependent memory accesses and by instruction fetch.  Every task is a descriptor
-// with absolute pointers (one dependent load to reach the data); the gathers
-// issue a whole batch of column, multiplier and spike-word loads before consuming
-// any of them; and the code a block runs is kept small (loops that are serial
-// anyway -- Euler sub-steps, Philox rounds -- are not unrolled), because a block
-// executes its path once per step and a long straight-line path stalls on
-// instruction fetch.
+// Blocks whose gathers read a bounded range of presynaptic neurons first copy
+// those spike words ("windows", planned per block by the engine) into LDS, so
+// their spike-word reads are LDS reads instead of scattered memory gathers.
+//
+// Latency: at the reference size the work per step is tens of MB, so the step
+// time is set by chains of dependent memory accesses and by instruction fetch;
+// at larger sizes by the number of load instructions and scattered cache lines.
+// Every task is a descriptor with absolute pointers (one dependent load to reach
+// the data); columns and multipliers come in 16- or 8-byte vectors (4 or 8
+// entries per lane per load); the gathers issue a whole batch of column,
+// multiplier and spike-word loads before consuming any of them; and the code a
+// block runs is kept small (loops that are serial anyway -- Euler sub-steps,
+// Philox rounds -- are not unrolled), because a block executes its path once per
+// step and a long straight-line path stalls on instruction fetch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
