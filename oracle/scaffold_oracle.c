@@ -12,7 +12,9 @@
  *       committed as tests/golden/philox_vectors.json.
  *   orc_uniform : rocRAND's uniform_distribution over the same u32 inputs
  *       (golden file).  Its equality with cuRAND's curand_uniform mapping is
- *       PARITY UNPINNED (cuRAND cannot be run here).
+ *       PARITY UNPINNED: cuRAND's mapping is not published as source and
+ *       cannot be run here.  All thresholds used are below 2^-8, where
+ *       v * 2^-32 is exact, so at most one v per threshold can differ.
  *   orc_lif_step_f32 / _f64 : closed forms -- free-running inter-spike interval
  *       from the geometric Euler recursion, the fixed point V = El + Ie/gL,
  *       conductance decay (1-DT/tau)^k, refractory hold of exactly ref_step

@@ -63,7 +63,7 @@ def test_free_running_interval(orc, name, f64):
     assert abs(span / (len(spikes) - 1) - isi) <= 10.0 / (len(spikes) - 1) + 1e-9
 
 
-def test_survey_intervals(orc):
+def test_free_running_intervals(orc):
     # free-running intervals in sub-steps quoted for the reference model
     want = {"purkinje": 276, "golgi": 984, "stellate": 563, "dcn": 387}
     for name, isi in want.items():
