@@ -647,11 +647,14 @@ __device__ __forceinline__ void team_dispatch(const ScTeamTask &T, gptr<uint32_t
 }
 
 // Team-mode neurons.  Teams of one wavefront come 16 to a block, wider teams
-// fewer.  After the gather, each team's first lane turns its row sums into dg
-// and parks dg, the neuron state and the population parameters in LDS; then the
-// block's first wavefront runs the neuron updates of all its teams side by side
-// (one lane per team) instead of one lane in each of up to 16 wavefronts, which
-// would make the SIMDs issue the long update 16 times.
+// fewer.  Before the gather, each team's first wavefront copies the population
+// parameters and the per-projection factors into LDS, so nothing after the gather
+// waits on global memory but the state store and the spike bit.  After the gather
+// that wavefront turns its row sums into dg and, in blocks of several teams,
+// parks dg and the neuron state in LDS; then the block's first wavefront runs the
+// neuron updates of all its teams side by side (one lane per team) instead of one
+// lane in each of up to 16 wavefronts, which would make the SIMDs issue the long
+// update 16 times.
 template <bool SINGLE, bool IDX16>
 __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *__restrict__ insts, int t,
                                         int first, int count, const ScTeamTask *__restrict__ teams, int row_single,
@@ -662,6 +665,7 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
     __shared__ uint32_t rowoff[NT][SC_MAX_IN];
     constexpr int PW = (SC_POP_WORDS + 1) / 2 * 2;
     __shared__ alignas(8) uint32_t par[NT][PW];    // population parameters (8-byte aligned rows)
+    __shared__ float fac[NT][3][SC_MAX_IN];  // per projection: scale, W, excitatory flag (as bits)
     __shared__ float sv[NT][10];  // u, g_exc, g_inh, dge, dgi; refr, device index, ring row, Wr, R (as bits)
     __shared__ void *ptr[NT][2];  // instance arrays (state, ring) of each team's network
     const int wave = uniform(threadIdx.x / SC_WAVE), lane = threadIdx.x & (SC_WAVE - 1);
@@ -684,8 +688,13 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
             rowoff[team][lane] = lane >= T->n_in ? 0u
                                  : win ? (uint32_t)T->in[lane].lds
                                        : (uint32_t)row_back(row_n, T->in[lane].delay, I.R) * (uint32_t)(2 * I.Wr);
+            if (lane < T->n_in) {  // the factors that turn row sums into dg, read before the gather
+                fac[team][0][lane] = T->in[lane].scale;
+                fac[team][1][lane] = T->in[lane].W;
+                fac[team][2][lane] = __int_as_float(T->in[lane].exc);
+            }
         }
-        if (gather_updates && lane < SC_POP_WORDS)
+        if (lane < SC_POP_WORDS)
             par[team][lane] = gbl(reinterpret_cast<const uint32_t *>(&I.pops[T->pop]))[lane];
         if (lane == 0) load_state(I.st, d, u, gg, rf);  // loaded while the gather runs
     }
@@ -699,15 +708,23 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
     }
     __syncthreads();
     float dge = 0.f, dgi = 0.f;
-    if (active && tw == 0 && lane == 0) {
-        for (int j = 0; j < T->n_in; ++j) {
-            // dg[post] += S * w_bar (Simulation_header.cu:98), projections in index order
-            const float Sf = (float)(acc[team][j] * (double)T->in[j].scale);
-            const float add = Sf * T->in[j].W;
-            if (T->in[j].exc) dge = dge + add;
-            else dgi = dgi + add;
+    if (active && tw == 0) {
+        // dg[post] += S * w_bar (Simulation_header.cu:98): lane j forms the term of
+        // projection j, then the terms are summed in projection index order
+        const int n_in = uniform(T->n_in);
+        float add = 0.f;
+        int ex = 0;
+        if (lane < n_in) {
+            add = (float)(acc[team][lane] * (double)fac[team][0][lane]) * fac[team][1][lane];
+            ex = __float_as_int(fac[team][2][lane]);
         }
-        if (gather_updates) {
+        const uint64_t em = __ballot(ex != 0);
+        for (int q = 0; q < n_in; ++q) {
+            const float a = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(add), q));
+            if ((em >> q) & 1) dge = dge + a;
+            else dgi = dgi + a;
+        }
+        if (gather_updates && lane == 0) {
             sv[team][0] = u;
             sv[team][1] = gg.x;
             sv[team][2] = gg.y;
@@ -751,9 +768,7 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
         }
     }
     if (upd) {
-        int spk;
-        if (gather_updates) spk = lif_step(*reinterpret_cast<const ScPop *>(par[j]), u, gg.x, gg.y, rf, dge, dgi);
-        else spk = lif_step(I.pops[T->pop], u, gg.x, gg.y, rf, dge, dgi);
+        const int spk = lif_step(*reinterpret_cast<const ScPop *>(par[j]), u, gg.x, gg.y, rf, dge, dgi);
         const int group = dj / SC_WAVE, bit = dj & (SC_WAVE - 1);
         store_state(ST, dj, u, gg, rf);
         if (spk) atomicOr(&ring[(int64_t)rown * Wr + group], 1ull << bit);
