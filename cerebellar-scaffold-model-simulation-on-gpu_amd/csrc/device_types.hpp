@@ -171,6 +171,8 @@ struct ScStage {
     int32_t n, pad_;
     ScWin w[SC_MAX_WIN];
 };
+#define SC_STAGE_DWORDS ((int)(sizeof(ScStage) / 4))  // at most 64: one dword per lane
+static_assert(sizeof(ScStage) / 4 <= 64, "a block's window table is loaded with one dword per lane");
 
 struct ScTeamTask {        // one neuron of a team-mode population
     const void *col;       // merged row columns (32-bit words, or 16-bit offsets)

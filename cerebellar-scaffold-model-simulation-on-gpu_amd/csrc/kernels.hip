@@ -21,8 +21,10 @@
 // steps are captured once as a hipGraph and replayed.
 //
 // Blocks whose gathers read a bounded range of presynaptic neurons first copy
-// those spike words ("windows", planned per block by the engine) into LDS, so
-// their spike-word reads are LDS reads instead of scattered memory gathers.
+// those spike words ("windows", planned per block by the engine) into LDS with
+// direct global-to-LDS loads, so their spike-word reads are LDS reads instead of
+// scattered memory gathers; the copies overlap the block's first descriptor,
+// state and column loads.
 //
 // Latency: at the reference size the work per step is tens of MB, so the step
 // time is set by chains of dependent memory accesses and by instruction fetch;
@@ -105,6 +107,16 @@ __device__ __forceinline__ void store_state(ScState *st, int d, float u, float2 
     x.z = __float_as_uint(g.y);
     x.w = (uint32_t)r;
     reinterpret_cast<u32x4 *>(st)[d] = x;
+}
+
+// One direct global-to-LDS load: lane l copies the 16 bytes at src (its own
+// address) to LDS at dst + 16 * l, without passing through registers; dst is
+// wave-uniform.  The copy counts as an outstanding vector-memory load of the
+// wavefront until it has been written to LDS.
+__device__ __forceinline__ void lds_load16(const u32x4 *src, u32x4 *dst)
+{
+    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
+                                     (__attribute__((address_space(3))) void *)dst, 16, 0, 0);
 }
 
 // Wave-uniform value in a scalar register (descriptor indices derived from the
@@ -430,6 +442,9 @@ __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance
             }
         }
     }
+    // staged block: the windows are complete past the block's barrier, which
+    // every wavefront of the block reaches once
+    if constexpr (STAGED) __syncthreads();
 #pragma unroll
     for (int s = 0; s < SC_MAX_SLICES; ++s)
 #pragma unroll
@@ -485,8 +500,22 @@ __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance
 // header word (projection slot and base bit) and 8 16-bit offsets from the base;
 // padding entries (SC_PAD16) read the ring row's zero word (bit zb) or, staged,
 // LDS word 0.
+// The wave-uniform fields of a team-mode neuron's descriptor, loaded once when
+// the block starts (with scalar loads) and kept in registers across the block's
+// barriers.
+struct TeamHead {
+    const void *col;
+    const uint32_t *hdr;
+    const void *val;
+    int32_t inst, group, bit, pop, n_in, wprec, len;
+};
+__device__ __forceinline__ TeamHead team_head(const ScTeamTask *T)
+{
+    return TeamHead{T->col, T->hdr, T->val, T->inst, T->group, T->bit, T->pop, T->n_in, T->wprec, T->len};
+}
+
 template <int WP, int U, bool IDX16, bool STAGED>
-__device__ __forceinline__ void team_gather(const ScTeamTask &T, gptr<uint32_t> ring32, const uint32_t *win,
+__device__ __forceinline__ void team_gather(const TeamHead &T, gptr<uint32_t> ring32, const uint32_t *win,
                                             const uint32_t *rowoff, uint32_t zb, int tid, int TT, double *acc)
 {
     const int len = T.len;
@@ -551,7 +580,7 @@ __device__ __forceinline__ void team_gather(const ScTeamTask &T, gptr<uint32_t> 
 // Whether one stimulus entry fires in exchange step n (draw index m):
 // InputStimulation (UserInputFunctions.cu:94-118) with the rule of ScStim.  The
 // three Philox blocks of the step run interleaved, round by round.
-__device__ __noinline__ int stim_fires(const ScStim &S, int entry, int64_t n, int64_t m)
+__device__ __forceinline__ int stim_fires(const ScStim &S, int entry, int64_t n, int64_t m)
 {
     const uint64_t blk = (uint64_t)(3 * m);
     uint32_t a0 = (uint32_t)blk, a1 = (uint32_t)(blk >> 32), a2 = (uint32_t)entry, a3 = 0u;
@@ -566,14 +595,17 @@ __device__ __noinline__ int stim_fires(const ScStim &S, int entry, int64_t n, in
         k0 += 0x9E3779B9u;
         k1 += 0xBB67AE85u;
     }
-    const uint32_t word[10] = {a0, a1, a2, a3, b0, b1, b2, b3, c0, c1};
     int spk = 0;
     const double nf = (double)(float)n;
 #pragma unroll 1
     for (int k = 0; k < kStepMax; ++k) {
         const float t = (float)(nf + (double)(float)k * kDT);
         const double td = (double)t;
-        const double rk = (double)u01(word[k]);
+        // draw k is word k of (a0..a3, b0..b3, c0, c1): the words move down one
+        // place a sub-step, so the loop indexes no array (which would live in
+        // scratch memory)
+        const double rk = (double)u01(a0);
+        a0 = a1; a1 = a2; a2 = a3; a3 = b0; b0 = b1; b1 = b2; b2 = b3; b3 = c0; c0 = c1;
         int f = 0;
         if (S.poisson_on <= td && td < S.poisson_off && rk < S.p_thr) f = 1;
         if (S.burst_on <= td && td < S.burst_off && rk < S.b_thr) f = 1;
@@ -626,7 +658,7 @@ __device__ __forceinline__ const ScInstance &inst_of(const ScInstance &I0, const
 
 // Gather of one team: storage form and multiplier type of its row.
 template <bool IDX16, bool STAGED>
-__device__ __forceinline__ void team_dispatch(const ScTeamTask &T, gptr<uint32_t> ring32, const uint32_t *win,
+__device__ __forceinline__ void team_dispatch(const TeamHead &T, gptr<uint32_t> ring32, const uint32_t *win,
                                               const uint32_t *rowoff, uint32_t zb, int tid, int TT, double *acc)
 {
     if (IDX16 && T.hdr) {
@@ -655,10 +687,10 @@ __device__ __forceinline__ void team_dispatch(const ScTeamTask &T, gptr<uint32_t
 // neuron updates of all its teams side by side (one lane per team) instead of one
 // lane in each of up to 16 wavefronts, which would make the SIMDs issue the long
 // update 16 times.
-template <bool SINGLE, bool IDX16>
-__device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *__restrict__ insts, int t,
-                                        int first, int count, const ScTeamTask *__restrict__ teams, int row_single,
-                                        int k, const uint32_t *win)
+template <bool SINGLE, bool IDX16, typename Stage>
+__device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *__restrict__ insts, int t, int count,
+                                        const TeamHead &h, const ScTeamIn &pin, int row_single, int k,
+                                        const Stage &stage)
 {
     constexpr int NT = SC_BLOCK / SC_WAVE;
     __shared__ double acc[NT][SC_MAX_IN];
@@ -672,46 +704,52 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
     const int team = wave / t, tw = wave - team * t;
     const bool active = team < count;
     const bool gather_updates = count > 1;
-    const ScTeamTask *T = active ? &teams[first + team] : nullptr;
-    const ScInstance &I = inst_of<SINGLE>(I0, insts, active ? T->inst : 0);
+    const ScInstance &I = inst_of<SINGLE>(I0, insts, h.inst);
     int row_n = row_single;
     if (!SINGLE && active) row_n = (I.ctl->row + k) % I.R;
     float u = 0.f;
     float2 gg = {0.f, 0.f};
     int32_t rf = 0;
-    const int d = active ? T->group * SC_WAVE + T->bit : 0;
+    const int d = h.group * SC_WAVE + h.bit;
+    // the neuron state and the population parameters are loaded (the state while
+    // the gather runs), then the block's window copies are issued, so both are in
+    // flight together
+    uint32_t pv = 0u;
+    if (active && tw == 0) {
+        if (lane < SC_POP_WORDS) pv = gbl(reinterpret_cast<const uint32_t *>(&I.pops[h.pop]))[lane];
+        if (lane == 0) load_state(I.st, d, u, gg, rf);
+    }
+    const uint32_t *win = stage();
     if (active && tw == 0) {
         if (lane < SC_MAX_IN) {
             acc[team][lane] = 0.0;
             // word offset of each projection's ring row: in global memory, or in a
             // staged block the LDS word that stands for word 0 of the row
-            rowoff[team][lane] = lane >= T->n_in ? 0u
-                                 : win ? (uint32_t)T->in[lane].lds
-                                       : (uint32_t)row_back(row_n, T->in[lane].delay, I.R) * (uint32_t)(2 * I.Wr);
-            if (lane < T->n_in) {  // the factors that turn row sums into dg, read before the gather
-                fac[team][0][lane] = T->in[lane].scale;
-                fac[team][1][lane] = T->in[lane].W;
-                fac[team][2][lane] = __int_as_float(T->in[lane].exc);
+            rowoff[team][lane] = lane >= h.n_in ? 0u
+                                 : win ? (uint32_t)pin.lds
+                                       : (uint32_t)row_back(row_n, pin.delay, I.R) * (uint32_t)(2 * I.Wr);
+            if (lane < h.n_in) {  // the factors that turn row sums into dg
+                fac[team][0][lane] = pin.scale;
+                fac[team][1][lane] = pin.W;
+                fac[team][2][lane] = __int_as_float(pin.exc);
             }
         }
-        if (lane < SC_POP_WORDS)
-            par[team][lane] = gbl(reinterpret_cast<const uint32_t *>(&I.pops[T->pop]))[lane];
-        if (lane == 0) load_state(I.st, d, u, gg, rf);  // loaded while the gather runs
+        if (lane < SC_POP_WORDS) par[team][lane] = pv;
     }
-    __syncthreads();
+    __syncthreads();  // staged block: the windows are complete past this barrier
     if (active) {
         gptr<uint32_t> ring32 = gbl(reinterpret_cast<const uint32_t *>(I.ring));
         const int tid = tw * SC_WAVE + lane, TT = t * SC_WAVE;
         const uint32_t zb = (uint32_t)(I.Wr - 1) * SC_WAVE;
-        if (win) team_dispatch<IDX16, true>(*T, ring32, win, rowoff[team], zb, tid, TT, acc[team]);
-        else team_dispatch<IDX16, false>(*T, ring32, win, rowoff[team], zb, tid, TT, acc[team]);
+        if (win) team_dispatch<IDX16, true>(h, ring32, win, rowoff[team], zb, tid, TT, acc[team]);
+        else team_dispatch<IDX16, false>(h, ring32, win, rowoff[team], zb, tid, TT, acc[team]);
     }
     __syncthreads();
     float dge = 0.f, dgi = 0.f;
     if (active && tw == 0) {
         // dg[post] += S * w_bar (Simulation_header.cu:98): lane j forms the term of
         // projection j, then the terms are summed in projection index order
-        const int n_in = uniform(T->n_in);
+        const int n_in = h.n_in;
         float add = 0.f;
         int ex = 0;
         if (lane < n_in) {
@@ -780,6 +818,39 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
     }
 }
 
+// Issues the copies of a staged block's windows into LDS (see ScStage); sd holds
+// dword `lane` of the block's window table.  Returns the LDS windows, or nullptr
+// for a block without windows.
+template <bool SINGLE>
+__device__ __forceinline__ const uint32_t *stage_windows(uint32_t sd, uint32_t *win_lds, const ScInstance &I0,
+                                                         const ScInstance *__restrict__ insts, int row_single,
+                                                         int k, int wave, int lane)
+{
+    constexpr int W0 = (int)(offsetof(ScStage, w) / 4), WS = (int)(sizeof(ScWin) / 4);
+    constexpr int F_INST = (int)(offsetof(ScWin, inst) / 4), F_DELAY = (int)(offsetof(ScWin, delay) / 4);
+    constexpr int F_W0 = (int)(offsetof(ScWin, w0) / 4), F_NW = (int)(offsetof(ScWin, nw) / 4);
+    constexpr int F_OFF = (int)(offsetof(ScWin, off) / 4);
+    const int nwin = __builtin_amdgcn_readlane((int)sd, (int)(offsetof(ScStage, n) / 4));
+    if (nwin <= 0) return nullptr;
+    u32x4 *dst = reinterpret_cast<u32x4 *>(win_lds);
+    for (int w = 0; w < nwin; ++w) {
+        const int base = W0 + w * WS;
+        const int inst = __builtin_amdgcn_readlane((int)sd, base + F_INST);
+        const int delay = __builtin_amdgcn_readlane((int)sd, base + F_DELAY);
+        const int w0 = __builtin_amdgcn_readlane((int)sd, base + F_W0);
+        const int n4 = __builtin_amdgcn_readlane((int)sd, base + F_NW) >> 2;
+        const int o4 = __builtin_amdgcn_readlane((int)sd, base + F_OFF) >> 2;
+        const ScInstance &I = inst_of<SINGLE>(I0, insts, inst);
+        const int rn = SINGLE ? row_single : (I.ctl->row + k) % I.R;
+        const u32x4 *src = reinterpret_cast<const u32x4 *>(
+            reinterpret_cast<const uint32_t *>(I.ring + (int64_t)row_back(rn, delay, I.R) * I.Wr) + w0);
+        for (int cb = wave * SC_WAVE; cb < n4; cb += SC_BLOCK)
+            if (cb + lane < n4) lds_load16(src + cb + lane, dst + o4 + cb);
+    }
+    if (threadIdx.x == 0) dst[0] = u32x4{0u, 0u, 0u, 0u};
+    return win_lds;
+}
+
 // The block layout is a kernel argument: block b belongs to the first class c with
 // b < cls_end[c]; classes 0..4 are team neurons of width 16, 8, 4, 2, 1
 // wavefronts, class 5 lane groups, class 6 stimulus groups.
@@ -801,43 +872,36 @@ __device__ __forceinline__ void step_body(const ScInstance &I0, const ScInstance
     for (int j = 0; j < SC_CLASSES - 1; ++j) c += (b >= L.cls_end[j]) ? 1 : 0;
     const int bi = b - (c ? L.cls_end[c - 1] : 0);
     const int wave = uniform(threadIdx.x / SC_WAVE), lane = threadIdx.x & (SC_WAVE - 1);
-    // staged block: copy its spike-bitmap windows into LDS, so that its gathers
-    // read LDS instead of scattering over the ring rows in memory
+    // staged block: its spike-bitmap windows are copied into LDS, so that its
+    // gathers read LDS instead of scattering over the ring rows in memory.  The
+    // block's window table arrives with one load (a dword per lane); the copies
+    // are direct global-to-LDS loads, issued once the block's first descriptor
+    // loads are under way and waited for only at the block's first barrier, so
+    // the copies overlap the loads of the neuron state, the population
+    // parameters and (lane blocks) the first columns.  Stimulus blocks are never
+    // staged.
     __shared__ __attribute__((aligned(16))) uint32_t win_lds[SC_WIN_WORDS];
-    const uint32_t *win = nullptr;
-    const int nwin = stages ? uniform(stages[b].n) : 0;
-    if (nwin > 0) {
-        u32x4 *dst = reinterpret_cast<u32x4 *>(win_lds);
-        for (int w = 0; w < nwin; ++w) {
-            const ScWin W = stages[b].w[w];
-            const ScInstance &I = inst_of<SINGLE>(I0, insts, W.inst);
-            const int rn = SINGLE ? row_single : (I.ctl->row + k) % I.R;
-            gptr<u32x4> src = gbl(reinterpret_cast<const u32x4 *>(
-                reinterpret_cast<const uint32_t *>(I.ring + (int64_t)row_back(rn, W.delay, I.R) * I.Wr) + W.w0));
-            const int n4 = W.nw >> 2, o4 = W.off >> 2;
-            for (int i0 = threadIdx.x; i0 < n4; i0 += 2 * SC_BLOCK) {
-                u32x4 v[2];
-#pragma unroll
-                for (int j = 0; j < 2; ++j)
-                    if (i0 + j * SC_BLOCK < n4) v[j] = src[i0 + j * SC_BLOCK];
-#pragma unroll
-                for (int j = 0; j < 2; ++j)
-                    if (i0 + j * SC_BLOCK < n4) dst[o4 + i0 + j * SC_BLOCK] = v[j];
-            }
-        }
-        if (threadIdx.x == 0) dst[0] = u32x4{0u, 0u, 0u, 0u};
-        __syncthreads();
-        win = win_lds;
-    }
+    uint32_t sd = 0u;
+    if (stages && lane < SC_STAGE_DWORDS) sd = gbl(reinterpret_cast<const uint32_t *>(stages + b))[lane];
+    const auto stage = [&]() { return stage_windows<SINGLE>(sd, win_lds, I0, insts, row_single, k, wave, lane); };
     if (c < SC_CLASS_LANE) {
         const int t = SC_MAX_TEAM >> c, per = 1 << c;
-        do_team<SINGLE, IDX16>(I0, insts, t, L.cls_task0[c] + bi * per, min(per, L.cls_ntask[c] - bi * per), teams,
-                               row_single, k, win);
+        const int first = L.cls_task0[c] + bi * per, count = min(per, L.cls_ntask[c] - bi * per);
+        // wavefronts past the block's teams read the last team's descriptor (unused)
+        const ScTeamTask *T = &teams[first + min(wave / t, count - 1)];
+        const TeamHead h = team_head(T);
+        ScTeamIn pin{};
+        if (lane < SC_MAX_IN) pin = T->in[lane];
+        do_team<SINGLE, IDX16>(I0, insts, t, count, h, pin, row_single, k, stage);
         return;
     }
+    const uint32_t *win = stage();
     const int per_block = c == SC_CLASS_LANE ? SC_LANE_PER_BLOCK : SC_STIM_PER_BLOCK;
     const int idx = uniform(bi * per_block + wave);
-    if (wave >= per_block || idx >= L.cls_ntask[c]) return;
+    if (wave >= per_block || idx >= L.cls_ntask[c]) {
+        if (win) __syncthreads();  // the wavefront's window copies complete before it ends
+        return;
+    }
     if (c == SC_CLASS_LANE) {
         const ScLaneTask &t = lanes[idx];
         const ScInstance &I = inst_of<SINGLE>(I0, insts, t.inst);
