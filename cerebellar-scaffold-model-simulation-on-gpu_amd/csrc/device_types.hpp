@@ -121,7 +121,7 @@ struct ScTask {
 #define SC_MAX_TEAM   16   // wavefronts per team-mode neuron (one 1024-thread block)
 #define SC_BLOCK      1024 // threads per block of the step kernel
 #define SC_LANE_PER_BLOCK 16  // lane-mode groups per block (the other wavefronts idle)
-#define SC_STIM_PER_BLOCK 4   // stimulus groups per block (the other wavefronts idle)
+#define SC_STIM_PER_BLOCK 16  // stimulus groups per block
 
 // 16-bit column form: columns are stored as 16-bit offsets from base device bits
 // held in headers (lane slices: one per column pair; team rows: one per 8-entry

@@ -388,11 +388,13 @@ __device__ __forceinline__ double slice_tail(const ScSlice &sl, gptr<uint32_t> r
 // One 64-neuron group of a lane-mode population: the first 4 columns of every
 // slice are loaded in one batch (one vector of columns and one of multipliers per
 // slice, then all spike words), the neuron state alongside, then the LIF update.
+// In a staged block the window copies are issued (stage()) once the first loads
+// are under way.
 // Slice widths and pointers are wave-uniform; slices without multipliers (unit
 // weights) skip the multiplier loads.
-template <int WP, bool IDX16, bool STAGED>
+template <int WP, bool IDX16, bool STAGED, typename Stage>
 __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance &I, const ScPop &P, int row_n,
-                                           const uint32_t *win, int lane)
+                                           const Stage &stage, int lane)
 {
     const uint32_t zb = (uint32_t)(I.Wr - 1) * SC_WAVE;
     const int d = t.group * SC_WAVE + lane;
@@ -442,9 +444,14 @@ __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance
             }
         }
     }
-    // staged block: the windows are complete past the block's barrier, which
-    // every wavefront of the block reaches once
-    if constexpr (STAGED) __syncthreads();
+    // staged block: the window copies are issued now, with the first loads in
+    // flight, and are complete past the block's barrier, which every wavefront
+    // of the block reaches once
+    const uint32_t *win = nullptr;
+    if constexpr (STAGED) {
+        win = stage();
+        __syncthreads();
+    }
 #pragma unroll
     for (int s = 0; s < SC_MAX_SLICES; ++s)
 #pragma unroll
@@ -624,23 +631,23 @@ __device__ __forceinline__ int stim_fires(const ScStim &S, int entry, int64_t n,
 }
 
 // One lane task: storage form and multiplier type of its slices.
-template <bool IDX16, bool STAGED>
+template <bool IDX16, bool STAGED, typename Stage>
 __device__ __forceinline__ void lane_dispatch(const ScLaneTask &t, const ScInstance &I, const ScPop &P, int row_n,
-                                              const uint32_t *win, int lane)
+                                              const Stage &stage, int lane)
 {
     if (IDX16 && t.idx16) {
         switch (t.wp) {
-        case SC_W_F32: lane_group<SC_W_F32, true, STAGED>(t, I, P, row_n, win, lane); break;
-        case SC_W_BF16: lane_group<SC_W_BF16, true, STAGED>(t, I, P, row_n, win, lane); break;
-        case SC_W_FP8: lane_group<SC_W_FP8, true, STAGED>(t, I, P, row_n, win, lane); break;
-        default: lane_group<SC_W_UNIT, true, STAGED>(t, I, P, row_n, win, lane); break;
+        case SC_W_F32: lane_group<SC_W_F32, true, STAGED>(t, I, P, row_n, stage, lane); break;
+        case SC_W_BF16: lane_group<SC_W_BF16, true, STAGED>(t, I, P, row_n, stage, lane); break;
+        case SC_W_FP8: lane_group<SC_W_FP8, true, STAGED>(t, I, P, row_n, stage, lane); break;
+        default: lane_group<SC_W_UNIT, true, STAGED>(t, I, P, row_n, stage, lane); break;
         }
     } else {
         switch (t.wp) {
-        case SC_W_F32: lane_group<SC_W_F32, false, STAGED>(t, I, P, row_n, win, lane); break;
-        case SC_W_BF16: lane_group<SC_W_BF16, false, STAGED>(t, I, P, row_n, win, lane); break;
-        case SC_W_FP8: lane_group<SC_W_FP8, false, STAGED>(t, I, P, row_n, win, lane); break;
-        default: lane_group<SC_W_UNIT, false, STAGED>(t, I, P, row_n, win, lane); break;
+        case SC_W_F32: lane_group<SC_W_F32, false, STAGED>(t, I, P, row_n, stage, lane); break;
+        case SC_W_BF16: lane_group<SC_W_BF16, false, STAGED>(t, I, P, row_n, stage, lane); break;
+        case SC_W_FP8: lane_group<SC_W_FP8, false, STAGED>(t, I, P, row_n, stage, lane); break;
+        default: lane_group<SC_W_UNIT, false, STAGED>(t, I, P, row_n, stage, lane); break;
         }
     }
 }
@@ -895,11 +902,14 @@ __device__ __forceinline__ void step_body(const ScInstance &I0, const ScInstance
         do_team<SINGLE, IDX16>(I0, insts, t, count, h, pin, row_single, k, stage);
         return;
     }
-    const uint32_t *win = stage();
     const int per_block = c == SC_CLASS_LANE ? SC_LANE_PER_BLOCK : SC_STIM_PER_BLOCK;
     const int idx = uniform(bi * per_block + wave);
+    const bool staged = c == SC_CLASS_LANE && __builtin_amdgcn_readlane((int)sd, (int)(offsetof(ScStage, n) / 4)) > 0;
     if (wave >= per_block || idx >= L.cls_ntask[c]) {
-        if (win) __syncthreads();  // the wavefront's window copies complete before it ends
+        if (staged) {  // the wavefront's share of the window copies, complete before it ends
+            stage();
+            __syncthreads();
+        }
         return;
     }
     if (c == SC_CLASS_LANE) {
@@ -907,9 +917,9 @@ __device__ __forceinline__ void step_body(const ScInstance &I0, const ScInstance
         const ScInstance &I = inst_of<SINGLE>(I0, insts, t.inst);
         const int row_n = SINGLE ? row_single : (I.ctl->row + k) % I.R;
         const ScPop &P = I.pops[t.pop];
-        if (win) lane_dispatch<IDX16, true>(t, I, P, row_n, win, lane);
-        else lane_dispatch<IDX16, false>(t, I, P, row_n, win, lane);
-    } else {
+        if (staged) lane_dispatch<IDX16, true>(t, I, P, row_n, stage, lane);
+        else lane_dispatch<IDX16, false>(t, I, P, row_n, stage, lane);
+    } else {  // stimulus blocks (never staged)
         const ScStimTask t = stt[idx];
         const ScInstance &I = inst_of<SINGLE>(I0, insts, t.inst);
         const int row_n = SINGLE ? row_single : (I.ctl->row + k) % I.R;
