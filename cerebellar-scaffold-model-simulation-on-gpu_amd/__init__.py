@@ -28,6 +28,9 @@ OK, ERR_INVALID_ARG, ERR_STATE, ERR_IO, ERR_FORMAT, ERR_UNSUPPORTED, ERR_HIP, ER
     ERR_WEIGHT_RANGE, ERR_CAPACITY = range(10)
 NORMAL, INPUT, OUTPUT = 0, 1, 2
 WEIGHTS_F32, WEIGHTS_BF16, WEIGHTS_FP8 = 0, 1, 2
+# plasticity rules and coefficient slots (STDP_RULE, STDP_COEFFICIENTS: struct_enum_def.h:120-121)
+STDP_HEBB, STDP_TEACHER = 0, 1
+A0, A1_I, A1_J, A2_JI, A2_IJ = range(5)
 
 
 class ScaffoldError(RuntimeError):
@@ -78,7 +81,7 @@ class ProjectionInfo(ctypes.Structure):
                 ("pre_num", ctypes.c_int32), ("post_num", ctypes.c_int32), ("delay", ctypes.c_int32),
                 ("initial_weight", ctypes.c_float), ("w_bar", ctypes.c_float), ("nnz", ctypes.c_int64),
                 ("max_conv", ctypes.c_int32), ("lane_mode", ctypes.c_int32), ("fp8_scale", ctypes.c_float),
-                ("sum_scale", ctypes.c_float)]
+                ("sum_scale", ctypes.c_float), ("plastic", ctypes.c_int32), ("grid_exp", ctypes.c_int32)]
 
 
 class TuneResult(ctypes.Structure):
@@ -120,6 +123,8 @@ def load_library(path=None):
                                              + [f32, f32, ctypes.c_int, P(ctypes.c_int)]),
         "scaffold_set_connectivity_csr": (ctypes.c_int, [vp, ctypes.c_int] + [ctypes.c_int] * 4
                                           + [f32, f32, P(i64), P(i32), P(f32), P(ctypes.c_int)]),
+        "scaffold_set_plasticity": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, P(f32),
+                                                   ctypes.c_uint32, ctypes.c_int, P(ctypes.c_int)]),
         "scaffold_set_input_stim": (ctypes.c_int, [vp, ctypes.c_int, cs, P(StimFunc), P(ctypes.c_int)]),
         "scaffold_set_input_stim_ids": (ctypes.c_int, [vp, ctypes.c_int, P(ctypes.c_uint32), ctypes.c_int, P(i64),
                                                        P(StimFunc), P(ctypes.c_int)]),
@@ -246,6 +251,21 @@ class NetworkEnvironment:
                                                        initial_weight, delay, _ptr(rp, ctypes.c_int64),
                                                        _ptr(cl, ctypes.c_int32), _ptr(vl, ctypes.c_float),
                                                        ctypes.byref(idx)))
+        return idx.value
+
+    def set_plasticity(self, rule, target, teacher=-1, coefficients=None, time_window=50):
+        """One plasticity rule (the table of Init_Plasticity, SynapticPlasticity.cu:20-33):
+        `coefficients` maps slots (A0, A1_I, A1_J, A2_JI, A2_IJ) to constants; absent
+        slots are the reference's NULL entries."""
+        coefficients = coefficients or {}
+        c = np.zeros(5, dtype=np.float32)
+        present = 0
+        for slot, v in coefficients.items():
+            c[slot] = v
+            present |= 1 << slot
+        idx = ctypes.c_int()
+        _check(self._lib.scaffold_set_plasticity(self._h, rule, target, teacher, _ptr(c, ctypes.c_float), present,
+                                                 time_window, ctypes.byref(idx)))
         return idx.value
 
     def set_input_stim(self, neuron_index, id_file, func):

@@ -133,8 +133,6 @@ int scaffold_create_network_env(scaffold_env **env, int num_of_dev, int num_of_n
         return fail(SCAFFOLD_ERR_INVALID_ARG, "num_of_neuron_types must be in [1, %d]", SC_MAX_POPS);
     if (num_of_connectivity_types < 0 || num_of_inputs < 0 || num_of_plasticities < 0)
         return fail(SCAFFOLD_ERR_INVALID_ARG, "negative table size");
-    if (num_of_plasticities > 0)
-        return fail(SCAFFOLD_ERR_UNSUPPORTED, "synaptic plasticity is not part of this engine");
     scaffold_env *e = new (std::nothrow) scaffold_env();
     if (!e) return fail(SCAFFOLD_ERR_CAPACITY, "out of host memory");
     e->n_types = num_of_neuron_types;
@@ -286,6 +284,42 @@ int scaffold_set_connectivity_csr(scaffold_env *env, int type, int preNum, int p
     p.set = true;
     env->projs[target] = std::move(p);
     if (index_out) *index_out = target;
+    return SCAFFOLD_OK;
+}
+
+int scaffold_set_plasticity(scaffold_env *env, int rule, int target, int teacher, const float coefficients[5],
+                            uint32_t present, int time_window, int *index_out)
+{
+    clear_error();
+    int rc = check_described(env);
+    if (rc) return rc;
+    if ((int)env->plast.size() >= env->n_plast)
+        return fail(SCAFFOLD_ERR_CAPACITY, "more plasticity rules than num_of_plasticities=%d", env->n_plast);
+    if (rule != SCAFFOLD_STDP_HEBB && rule != SCAFFOLD_STDP_TEACHER)
+        return fail(SCAFFOLD_ERR_INVALID_ARG, "rule %d: Hebb (0) and Teacher (1) are the reference's rules", rule);
+    if (target < 0 || target >= env->n_conn || !env->projs[target].set)
+        return fail(SCAFFOLD_ERR_INVALID_ARG, "target projection %d not registered", target);
+    if (rule == SCAFFOLD_STDP_TEACHER) {
+        if (teacher < 0 || teacher >= env->n_conn || !env->projs[teacher].set)
+            return fail(SCAFFOLD_ERR_INVALID_ARG, "teacher projection %d not registered", teacher);
+        if (env->projs[teacher].post != env->projs[target].post)
+            return fail(SCAFFOLD_ERR_INVALID_ARG, "the teacher projection must share the target's post population");
+    }
+    if (present && !coefficients) return fail(SCAFFOLD_ERR_INVALID_ARG, "null coefficients");
+    if (present >> 5) return fail(SCAFFOLD_ERR_INVALID_ARG, "present: bits 0-4 only");
+    if (time_window < 1) return fail(SCAFFOLD_ERR_INVALID_ARG, "time_window must be >= 1");
+    Plasticity r;
+    r.rule = rule;
+    r.target = target;
+    r.teacher = rule == SCAFFOLD_STDP_TEACHER ? teacher : -1;
+    r.window = time_window;
+    r.present = present;
+    for (int c = 0; c < 5; ++c) {
+        r.coef[c] = (present >> c) & 1 ? coefficients[c] : 0.f;
+        if (!std::isfinite(r.coef[c])) return fail(SCAFFOLD_ERR_INVALID_ARG, "coefficient %d is not finite", c);
+    }
+    env->plast.push_back(r);
+    if (index_out) *index_out = (int)env->plast.size() - 1;
     return SCAFFOLD_OK;
 }
 
@@ -690,11 +724,14 @@ int scaffold_get_projection(scaffold_env *env, int index, scaffold_projection_in
     i.max_conv = p.max_conv;
     i.fp8_scale = 1.f;
     i.sum_scale = 1.f;
+    for (const Plasticity &r : env->plast) i.plastic |= r.target == index ? 1 : 0;
     if (env->initialized && p.set) {
         const ScProj &sp = env->plan.projs[index];
         i.lane_mode = sp.mode == SC_MODE_LANE;
         i.fp8_scale = sp.scale;
         i.sum_scale = sp.sum_scale;
+        for (const PlasticPlan &x : env->plan.plastic)
+            if (x.rule.target == index) i.grid_exp = x.grid_exp;
     }
     *out = i;
     return SCAFFOLD_OK;
@@ -713,7 +750,10 @@ int scaffold_get_projection_csr(scaffold_env *env, int index, int64_t *row_ptr, 
     if (val) {
         const int64_t nnz = (int64_t)p.col.size();
         if (env->initialized) {
-            for (int64_t e = 0; e < nnz; ++e) val[e] = plan_multiplier(env, index, e);
+            rc = env->eng ? engine_read_plastic(env->eng, index, val) : 1;
+            if (rc > 1) return rc;
+            if (rc == 1)  // not plastic: the multipliers as planned
+                for (int64_t e = 0; e < nnz; ++e) val[e] = plan_multiplier(env, index, e);
         } else {
             for (int64_t e = 0; e < nnz; ++e) val[e] = p.val.empty() ? 1.f : p.val[e];
         }
@@ -992,6 +1032,8 @@ int scaffold_batch_add(scaffold_batch *batch, scaffold_env *env)
     if (rc) return rc;
     for (auto *e : batch->envs)
         if (e == env) return fail(SCAFFOLD_ERR_INVALID_ARG, "environment added twice");
+    if (!env->plast.empty())
+        return fail(SCAFFOLD_ERR_UNSUPPORTED, "environments with plasticity rules cannot join a batch");
     batch->envs.push_back(env);
     env->in_batches++;
     return SCAFFOLD_OK;

@@ -204,6 +204,26 @@ struct alignas(16) ScState {
     int32_t refr;
 };
 
+// One plasticity rule over the synapses of its target projection
+// (Hebb_plasticity / Teacher_plasticity, SynapticPlasticity.cu:38-121), applied
+// after every exchange step.  Per synapse (source order): the presynaptic device
+// bit, the postsynaptic device bit (Hebb) or the teacher's device bit (Teacher;
+// -1: none), and the index of its multiplier in `val` (the f32 multiplier array
+// of the layout that holds the projection).
+#define SC_STDP_HEBB    0
+#define SC_STDP_TEACHER 1
+struct ScPlastic {
+    const uint32_t *pre;
+    const int32_t  *post;     // Hebb: post bit; Teacher: teacher bit or -1
+    const uint32_t *vidx;
+    float *val;
+    int64_t n;
+    int32_t rule, delay, t_delay, window;
+    int32_t present;          // bit c: coefficient c is set (A0, A1_I, A1_J, A2_JI)
+    int32_t grid_exp;         // updated multipliers are rounded to multiples of 2^grid_exp
+    float coef[4];            // A0, A1_I, A1_J, A2_JI
+};
+
 struct ScInstance {
     ScState  *st;              // per device index
     uint64_t *ring;

@@ -43,6 +43,14 @@ struct Engine {
     int Ndev = 0, total_ref = 0;
     hipGraphExec_t gexec = nullptr;
     int gexec_steps = 0;
+    // plastic projections: multiplier array and per-synapse index, for read-back
+    struct PlasticView {
+        int proj;
+        const float *val;
+        int64_t n_val;
+        std::vector<uint32_t> vidx;
+    };
+    std::vector<PlasticView> plastic_views;
 };
 
 template <typename T>
@@ -418,6 +426,33 @@ int engine_create(Engine **out, Plan &P)
     if ((rc = upload_work(A, B, e->w, e->h_team_t, e->h_lanes, e->h_stims, e->h_teams, e->h_lane_win, e->h_team_win,
                           P.stage != 0)))
         return bail(rc);
+    // plasticity rules: per-synapse arrays; the multipliers they update are the
+    // f32 arrays of the target's lane slices or merged rows
+    e->w.plastic.clear();
+    for (const PlasticPlan &X : P.plastic) {
+        ScPlastic d{};
+        uint32_t *pre, *vidx;
+        int32_t *post;
+        if ((rc = dupload(A, B, &pre, X.pre))) return bail(rc);
+        if ((rc = dupload(A, B, &post, X.post))) return bail(rc);
+        if ((rc = dupload(A, B, &vidx, X.vidx))) return bail(rc);
+        const uint8_t *v = X.team ? tval[X.owner] : pval[X.owner];
+        if (!v && !X.pre.empty()) return bail(fail(SCAFFOLD_ERR_STATE, "plastic projection without multipliers"));
+        d.pre = pre;
+        d.post = post;
+        d.vidx = vidx;
+        d.val = const_cast<float *>(reinterpret_cast<const float *>(v));
+        d.n = (int64_t)X.pre.size();
+        d.rule = X.rule.rule;
+        d.delay = X.delay;
+        d.t_delay = X.t_delay;
+        d.window = X.rule.window;
+        d.present = (int32_t)(X.rule.present & 15u);
+        d.grid_exp = X.grid_exp;
+        for (int c = 0; c < 4; ++c) d.coef[c] = X.rule.coef[c];
+        e->w.plastic.push_back(d);
+        e->plastic_views.push_back(Engine::PlasticView{X.rule.target, d.val, X.n_val, X.vidx});
+    }
     e->w.max_groups = P.n_groups;
     if ((rc = dalloc(A, B, &e->cnt_buf, 1))) return bail(rc);
     std::vector<ScInstance> one{I};
@@ -434,6 +469,19 @@ void engine_destroy(Engine *e)
     for (void *p : e->allocs) (void)hipFree(p);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
+}
+
+int engine_read_plastic(Engine *e, int proj, float *out)
+{
+    for (const Engine::PlasticView &v : e->plastic_views) {
+        if (v.proj != proj) continue;
+        std::vector<float> buf((size_t)v.n_val);
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        if (v.n_val > 0) HIP_TRY(hipMemcpy(buf.data(), v.val, sizeof(float) * (size_t)v.n_val, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < v.vidx.size(); ++i) out[i] = buf[v.vidx[i]];
+        return SCAFFOLD_OK;
+    }
+    return 1;
 }
 
 int64_t engine_device_bytes(Engine *e) { return e ? e->bytes : 0; }
@@ -490,7 +538,10 @@ int engine_upload_stims(Engine *e, const std::vector<ScStim> &stims)
 static void launch_run(hipStream_t s, const ScInstance *single, const ScInstance *insts, int n_inst,
                        const DevWork &w, int K)
 {
-    for (int k = 0; k < K; ++k) launch_step(s, single, insts, w, k);
+    for (int k = 0; k < K; ++k) {
+        launch_step(s, single, insts, w, k);
+        if (single) launch_plastic(s, *single, w, k);
+    }
     launch_stats(s, single, insts, n_inst, K, w.max_groups);
     launch_advance(s, insts, n_inst, K);
 }
@@ -563,7 +614,10 @@ int engine_time(Engine *e, int n_steps, int use_graph, int graph_steps, double *
             ev.push_back(s0);
             ev.push_back(s1);
             HIP_TRY(hipEventRecord(s0, e->stream));
-            for (int k = 0; k < K; ++k) launch_step(e->stream, &e->hi, e->di, e->w, k);
+            for (int k = 0; k < K; ++k) {
+                launch_step(e->stream, &e->hi, e->di, e->w, k);
+                launch_plastic(e->stream, e->hi, e->w, k);
+            }
             HIP_TRY(hipEventRecord(s1, e->stream));
             launch_stats(e->stream, &e->hi, e->di, 1, K, e->w.max_groups);
             launch_advance(e->stream, e->di, 1, K);

@@ -1073,6 +1073,48 @@ __global__ void k_zero_g(ScState *st, int n)
 // Fired-neuron list of one ring row: the spike words are the ballots of spike
 // detection; one block popcounts them, prefix-sums the counts and writes the
 // reference global ids in ascending order.
+// One plasticity rule after exchange step n (launch offset k): Hebb_plasticity /
+// Teacher_plasticity (SynapticPlasticity.cu:38-121), one thread per synapse of
+// the target projection.  dw is summed in float in the reference's order (A0;
+// A1_J if the presynaptic neuron spiked at n - delay; A1_I if the postsynaptic
+// neuron spiked at n (Hebb) or the teacher at n - t_delay (Teacher); with that
+// spike, A2_JI once for every spike of the presynaptic neuron at n - delay - s,
+// s = 0 .. window-1); then w + dw is clamped (Hebb to [0, 1], Teacher to >= 0)
+// and rounded to a multiple of 2^grid_exp, which keeps every row sum of the
+// projection exact in f64 (so the order of the gather's additions cannot change
+// it).  Rows of step n and earlier are complete: this launch follows the step's.
+__global__ void __launch_bounds__(256) k_plastic(ScInstance I, ScPlastic P, int k)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= P.n) return;
+    const int R = I.R;
+    const int row = (I.ctl->row + k) % R;
+    const uint64_t *ring = I.ring;
+    const auto spiked = [&](int r, uint32_t b) {
+        return (int)((ring[(int64_t)r * I.Wr + (b >> 6)] >> (b & 63)) & 1ull);
+    };
+    const uint32_t pre = P.pre[i];
+    const int pre_t = row_back(row, P.delay, R);
+    const int s_pre = spiked(pre_t, pre);
+    const int32_t pb = P.post[i];
+    int s_post = 0;
+    if (P.rule == SC_STDP_HEBB) s_post = spiked(row, (uint32_t)pb);
+    else if (pb >= 0) s_post = spiked(row_back(row, P.t_delay, R), (uint32_t)pb);
+    float *vp = P.val + P.vidx[i];
+    const float w = *vp;
+    float dw = 0.f;
+    if (P.present & 1) dw += P.coef[0];
+    if ((P.present & 4) && s_pre) dw += P.coef[2];
+    if ((P.present & 2) && s_post) dw += P.coef[1];
+    if ((P.present & 8) && s_post)
+        for (int s = 0; s < P.window; ++s)
+            if (spiked(row_back(pre_t, s, R), pre)) dw += P.coef[3];
+    float nw = w + dw;
+    if (P.rule == SC_STDP_HEBB) nw = nw < 0.f ? 0.f : nw > 1.f ? 1.f : nw;
+    else nw = nw < 0.f ? 0.f : nw;
+    *vp = ldexpf(rintf(ldexpf(nw, -P.grid_exp)), P.grid_exp);
+}
+
 __global__ void __launch_bounds__(1024) k_compact(const uint64_t *__restrict__ row, int n_groups,
                                                   const ScGroup *__restrict__ groups,
                                                   const ScPop *__restrict__ pops, int32_t *__restrict__ out,
@@ -1165,6 +1207,12 @@ void launch_compact(hipStream_t s, const uint64_t *row, int n_groups, const ScGr
 void launch_zero_g(hipStream_t s, ScState *st, int n)
 {
     if (n > 0) k_zero_g<<<(n + 255) / 256, 256, 0, s>>>(st, n);
+}
+
+void launch_plastic(hipStream_t s, const ScInstance &I, const DevWork &w, int k)
+{
+    for (const ScPlastic &P : w.plastic)
+        if (P.n > 0) k_plastic<<<(unsigned)((P.n + 255) / 256), 256, 0, s>>>(I, P, k);
 }
 
 void launch_copy(hipStream_t s, const void *src, void *dst, int64_t n16, int grid)

@@ -24,6 +24,7 @@ struct DevWork {
     const ScStage *stages = nullptr;  // per block: spike windows staged in LDS (nullptr: none)
     int n_staged = 0;                 // blocks with staged windows
     int max_groups = 0;               // most 64-neuron groups of one instance (statistics grid)
+    std::vector<ScPlastic> plastic;   // plasticity rules, applied in order after each step
 };
 
 // All launchers are asynchronous on stream s.  `single` != nullptr selects the
@@ -38,5 +39,7 @@ void launch_compact(hipStream_t s, const uint64_t *row, int n_groups, const ScGr
                     int32_t *out, int cap, int32_t *count);
 void launch_copy(hipStream_t s, const void *src, void *dst, int64_t n16, int grid);
 void launch_zero_g(hipStream_t s, ScState *st, int n);
+// The plasticity rules of one network after step k of a run (single instance).
+void launch_plastic(hipStream_t s, const ScInstance &I, const DevWork &w, int k);
 
 }  // namespace sc

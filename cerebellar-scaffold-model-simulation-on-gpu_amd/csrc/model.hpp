@@ -43,6 +43,16 @@ struct Projection {
     int delay() const { return (int)delay_ms; }
 };
 
+// One plasticity rule (a row of the table Init_Plasticity fills,
+// SynapticPlasticity.cu:20-33): rule, target and teacher projections (indices),
+// constant coefficients (slot c used when bit c of `present` is set) and the
+// window of the A2_JI term in exchange steps.
+struct Plasticity {
+    int rule = 0, target = -1, teacher = -1, window = 50;
+    uint32_t present = 0;
+    float coef[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+};
+
 struct Stimulus {
     int pop = -1;
     std::vector<uint32_t> targets;
@@ -114,6 +124,21 @@ struct TeamPop {
     std::vector<int64_t> row_seg;
 };
 
+// A plasticity rule lowered to the device layout: per synapse of the target
+// projection (source order) its presynaptic device bit, its postsynaptic device
+// bit (Hebb) or teacher device bit (Teacher; -1: none) and the index of its
+// multiplier in the f32 array that holds it (a lane-mode projection's slices or
+// a team-mode population's merged rows).
+struct PlasticPlan {
+    Plasticity rule;
+    int delay = 0, t_delay = 0, grid_exp = 0;
+    bool team = false;
+    int owner = -1;          // team index (team) or projection index (lane)
+    int64_t n_val = 0;       // f32 elements of the owner's multiplier array
+    std::vector<uint32_t> pre, vidx;
+    std::vector<int32_t> post;
+};
+
 struct Plan {
     int Ndev = 0, n_groups = 0, Wr = 0, R = 0;
     int total_ref = 0;
@@ -136,6 +161,7 @@ struct Plan {
     int max_delay = 0;
     std::vector<int> out_pops;            // OUTPUT populations
     int n_output = 0;
+    std::vector<PlasticPlan> plastic;     // plasticity rules in table order
 };
 
 struct Engine;  // device side (engine.hip)
@@ -160,6 +186,9 @@ int engine_time(Engine *e, int n_steps, int use_graph, int graph_steps,
 int64_t engine_device_bytes(Engine *e);
 void engine_blocks(Engine *e, int *n_blocks, int *n_staged);
 int engine_copy_probe(int64_t bytes, int iters, double *ms);
+// Current multipliers of a plastic projection in source order; returns 1 if no
+// plasticity rule targets the projection.
+int engine_read_plastic(Engine *e, int proj, float *out);
 
 // Batching (engine.hip).
 struct Batch;
@@ -177,6 +206,7 @@ struct scaffold_env {
     std::vector<sc::Projection> projs;
     int gpu_next = 0, cpu_next = 0;
     std::vector<sc::Stimulus> stims;
+    std::vector<sc::Plasticity> plast;
     sc::Options opt;
     bool initialized = false;
     bool finalized = false;

@@ -236,8 +236,18 @@ int plan_build(scaffold_env *env, Plan &P)
         const double mean = (double)fanin[p] / d.num;
         team_pop[p] = (mean > (double)opt.lane_max_fanin || d.n_in > SC_MAX_SLICES) ? 1 : 0;
     }
-    auto storage_for = [&](bool has_val) {
+    // plasticity: the target projections hold f32 multipliers (1.0 where none were
+    // given), and so do the other weighted projections onto their post
+    // populations (a lane group and a merged row have one multiplier storage)
+    std::vector<char> plastic_proj(nproj, 0), plastic_post(npop, 0);
+    for (const Plasticity &r : env->plast) {
+        plastic_proj[r.target] = 1;
+        plastic_post[env->projs[r.target].post] = 1;
+        weighted[env->projs[r.target].post] = 1;
+    }
+    auto storage_for = [&](bool has_val, bool f32) {
         if (!has_val) return SC_W_UNIT;
+        if (f32) return SC_W_F32;
         return opt.weight_precision == SCAFFOLD_WEIGHTS_BF16 ? SC_W_BF16
              : opt.weight_precision == SCAFFOLD_WEIGHTS_FP8 ? SC_W_FP8 : SC_W_F32;
     };
@@ -261,7 +271,8 @@ int plan_build(scaffold_env *env, Plan &P)
         const int64_t nnz = (int64_t)src.col.size();
         P.nnz += nnz;
         sp.mode = team_pop[src.post] ? SC_MODE_ROW : SC_MODE_LANE;
-        sp.wprec = storage_for(sp.mode == SC_MODE_ROW ? weighted[src.post] : !src.val.empty());
+        sp.wprec = storage_for(sp.mode == SC_MODE_ROW ? weighted[src.post] : !src.val.empty() || plastic_proj[pi],
+                               plastic_post[src.post]);
         std::vector<float> &ev = eff[pi];
         if (sp.wprec != SC_W_UNIT) {
             ev.resize(nnz);
@@ -296,6 +307,11 @@ int plan_build(scaffold_env *env, Plan &P)
         else if (sp.wprec == SC_W_BF16) { uint16_t h = f32_to_bf16(x); memcpy(&out[pos * 2], &h, 2); }
         else if (sp.wprec == SC_W_FP8) out[pos] = f32_to_e4m3(x / sp.scale);
     };
+    // multiplier position of every synapse of a plastic projection in the array
+    // that holds it (lane slices or merged rows)
+    std::vector<std::vector<int64_t>> vpos(nproj);
+    for (int pi = 0; pi < nproj; ++pi)
+        if (plastic_proj[pi]) vpos[pi].assign(env->projs[pi].col.size(), -1);
     // lane-mode slices
     for (int pi = 0; pi < nproj; ++pi) {
         const Projection &src = env->projs[pi];
@@ -330,6 +346,7 @@ int plan_build(scaffold_env *env, Plan &P)
                     const int64_t pos = sell_pos(A.sell_off[gi], k, l);
                     A.col[pos] = (uint32_t)(pre.dev_base + src.col[e]);
                     if (wb) put_w(A.val, pos, sp, eff[pi][e]);
+                    if (plastic_proj[pi]) vpos[pi][e] = pos;
                 }
             }
     }
@@ -429,7 +446,7 @@ int plan_build(scaffold_env *env, Plan &P)
         const ScPop &d = P.pops[p];
         TeamPop T;
         T.pop = p;
-        T.wprec = storage_for(weighted[p]);
+        T.wprec = storage_for(weighted[p], plastic_post[p]);
         const int wb = wbytes_of(T.wprec);
         T.row_ptr.assign(d.num + 1, 0);
         T.seg_end.assign((size_t)d.num * d.n_in, 0);
@@ -455,6 +472,7 @@ int plan_build(scaffold_env *env, Plan &P)
                 for (int64_t e = src.row_ptr[j]; e < src.row_ptr[j + 1]; ++e, ++pos) {
                     T.col[pos] = ((uint32_t)q << SC_Q_SHIFT) | (uint32_t)(pre.dev_base + src.col[e]);
                     if (wb) put_w(T.val, pos, P.projs[pi], eff[pi][e]);
+                    if (plastic_proj[pi]) vpos[pi][e] = pos;
                 }
             }
             // padding repeats the last entry (a bit the row reads anyway) with a
@@ -496,6 +514,8 @@ int plan_build(scaffold_env *env, Plan &P)
             std::vector<uint8_t> v16;
             c16.reserve(tot + tot / 4);
             std::vector<int64_t> perm;
+            std::vector<int64_t> newpos;  // plastic post population: merged-row position -> chunk entry
+            if (plastic_post[p]) newpos.assign(tot, -1);
             for (int j = 0; j < d.num; ++j) {
                 // entries of each projection segment in ascending bit order (the
                 // row sums are exact, so the order cannot change them)
@@ -515,10 +535,13 @@ int plan_build(scaffold_env *env, Plan &P)
                         if ((c >> SC_Q_SHIFT) != q || (c & SC_BIT_MASK) - lo >= SC_PAD16) break;
                         c16[c0 + n] = (uint16_t)((c & SC_BIT_MASK) - lo);
                         if (wb) memcpy(&v16[(c0 + n) * wb], &T.val[perm[e] * wb], wb);
+                        if (!newpos.empty()) newpos[perm[e]] = c0 + n;
                     }
                 }
                 rp[j + 1] = (int64_t)h.size();
             }
+            for (int q = 0; q < d.n_in && !newpos.empty(); ++q)
+                for (int64_t &x : vpos[d.in_proj[q]]) x = newpos[x];
             T.idx16 = true;
             T.row_ptr = std::move(rp);
             T.col16 = std::move(c16);
@@ -533,6 +556,84 @@ int plan_build(scaffold_env *env, Plan &P)
         T.team_waves = t;
         P.team_of[p] = (int)P.teams.size();
         P.teams.push_back(std::move(T));
+    }
+    // plasticity rules lowered to the layout (rules in table order)
+    P.plastic.clear();
+    for (const Plasticity &r : env->plast) {
+        const Projection &src = env->projs[r.target];
+        const ScProj &sp = P.projs[r.target];
+        PlasticPlan X;
+        X.rule = r;
+        X.delay = sp.delay;
+        // the A2_JI window reads rows n - delay - s, s < window; the row of step
+        // n + 1 is cleared during step n, so it must lie outside the window
+        if (r.window < 1 || sp.delay + r.window + 2 > P.R)
+            return fail(SCAFFOLD_ERR_INVALID_ARG, "plasticity on projection %d: delay %d + window %d needs a spike ring "
+                        "of at least %d rows (T_print %d gives %d)", r.target, sp.delay, r.window,
+                        sp.delay + r.window + 2, P.R / 2, P.R);
+        const Projection *tp = r.rule == SC_STDP_TEACHER ? &env->projs[r.teacher] : nullptr;
+        if (tp) {
+            X.t_delay = tp->delay();
+            if (X.t_delay >= P.R) return fail(SCAFFOLD_ERR_INVALID_ARG, "teacher delay beyond the spike ring");
+        }
+        // Row sums of the projection stay exact in f64 while every multiplier is a
+        // multiple of 2^grid_exp and every row sums to less than 2^(grid_exp + 53):
+        // grid_exp is the finest grid for the largest row and multiplier bound
+        // (Hebb clamps to [0, 1]; Teacher only depresses, so its initial maximum
+        // bounds it).  Updated multipliers are rounded to that grid.
+        const std::vector<float> &ev = eff[r.target];
+        float wmax = 0.f;
+        for (float x : ev) wmax = std::max(wmax, std::fabs(x));
+        if (r.rule == SC_STDP_HEBB) {
+            wmax = std::max(wmax, 1.f);
+        } else {
+            for (int c = 0; c < 4; ++c)
+                if ((r.present >> c) & 1 && r.coef[c] > 0.f)
+                    return fail(SCAFFOLD_ERR_INVALID_ARG, "teacher rule with a potentiating coefficient has no upper bound "
+                                "for its multipliers");
+        }
+        int64_t lmax = 0;
+        for (size_t j = 0; j + 1 < src.row_ptr.size(); ++j) lmax = std::max(lmax, src.row_ptr[j + 1] - src.row_ptr[j]);
+        const double bound = (double)lmax * (double)wmax;
+        X.grid_exp = bound > 0.0 ? std::max(-126, (int)std::ceil(std::log2(bound)) - 52) : -126;
+        for (float x : ev) {
+            const float y = std::ldexp(x, -X.grid_exp);
+            if (y != std::rint(y))
+                return fail(SCAFFOLD_ERR_WEIGHT_RANGE, "plasticity on projection %d: multiplier %g is finer than the "
+                            "exact-sum grid 2^%d", r.target, (double)x, X.grid_exp);
+        }
+        if (sp.mode == SC_MODE_LANE) {
+            X.team = false;
+            X.owner = r.target;
+            X.n_val = (int64_t)P.parr[r.target].val.size() / 4;
+        } else {
+            X.team = true;
+            X.owner = P.team_of[src.post];
+            X.n_val = (int64_t)P.teams[X.owner].val.size() / 4;
+        }
+        const int64_t nnz = (int64_t)src.col.size();
+        X.pre.resize(nnz);
+        X.post.resize(nnz);
+        X.vidx.resize(nnz);
+        const ScPop &pre = P.pops[src.pre], &post = P.pops[src.post];
+        for (size_t j = 0; j + 1 < src.row_ptr.size(); ++j) {
+            int32_t pb = (int32_t)(post.dev_base + (int)j);
+            if (tp) {  // the teacher: the first entry of the teacher projection's row (SynapticPlasticity.cu:93)
+                pb = -1;
+                if (j + 1 < tp->row_ptr.size() && tp->row_ptr[j] < tp->row_ptr[j + 1])
+                    pb = (int32_t)(P.pops[tp->pre].dev_base + tp->col[tp->row_ptr[j]]);
+            }
+            for (int64_t e = src.row_ptr[j]; e < src.row_ptr[j + 1]; ++e) {
+                const int64_t v = vpos[r.target][e];
+                if (v < 0 || v >= X.n_val)  // every synapse has one place in the layout
+                    return fail(SCAFFOLD_ERR_STATE, "plasticity on projection %d: synapse %lld has no multiplier slot",
+                                r.target, (long long)e);
+                X.pre[e] = (uint32_t)(pre.dev_base + src.col[e]);
+                X.post[e] = pb;
+                X.vidx[e] = (uint32_t)v;
+            }
+        }
+        P.plastic.push_back(std::move(X));
     }
     int64_t index_bytes = 0;  // 2 B per synapse in the 16-bit form, else 4 B
     P.syn16 = 0;

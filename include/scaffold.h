@@ -128,6 +128,39 @@ int scaffold_set_connectivity_params(scaffold_env *env, int type, const char *fi
                                      float initial_weight, float delay,
                                      int use_parallel_reduction, int *index_out);
 
+/* Synaptic plasticity: one row of the rule table that Init_Plasticity fills
+ * (SynapticPlasticity.cu:20-33; the reference ships the table empty).  Each
+ * exchange step, after the neuron update, every synapse of projection `target`
+ * is updated as Hebb_plasticity / Teacher_plasticity do
+ * (SynapticPlasticity.cu:38-121, invoked at GPUScaffoldSimulation.cu:754):
+ *   dw = A0 + A1_J [pre spiked at n - delay]
+ *           + A1_I [Hebb: post spiked at n | Teacher: teacher spiked at n - t_delay]
+ *           + that spike x sum over s < time_window of A2_JI [pre spiked at n - delay - s]
+ * summed in float in that order; Hebb: w = clamp(w + dw, 0, 1); Teacher:
+ * w = max(w + dw, 0).  The teacher of a post neuron is the first entry of its
+ * row in projection `teacher` (the climbing fibre, SynapticPlasticity.cu:93),
+ * which must share the target's post population.  Coefficients are constants
+ * (the reference's coefficient functions return constants); slot c of
+ * `coefficients` is used when bit c of `present` is set (the reference's NULL
+ * entries are clear bits; A2_IJ is unused by both rules).  Rules apply in table
+ * order.  A plastic projection keeps f32 multipliers (1.0 where none were
+ * given), as do the other weighted projections onto its post population, and
+ * updated multipliers are rounded to multiples of 2^grid_exp (see
+ * scaffold_projection_info), which keeps every row sum exact; a Teacher rule
+ * must not potentiate (its multipliers would have no bound).  The spike ring
+ * must hold delay + time_window + 2 rows (2 x T_print).  Environments with
+ * plasticity cannot join a batch.  At most num_of_plasticities rules. */
+#define SCAFFOLD_STDP_HEBB    0  /* STDP_RULE (struct_enum_def.h:121) */
+#define SCAFFOLD_STDP_TEACHER 1
+#define SCAFFOLD_STDP_A0    0    /* STDP_COEFFICIENTS (struct_enum_def.h:120) */
+#define SCAFFOLD_STDP_A1_I  1
+#define SCAFFOLD_STDP_A1_J  2
+#define SCAFFOLD_STDP_A2_JI 3
+#define SCAFFOLD_STDP_A2_IJ 4
+int scaffold_set_plasticity(scaffold_env *env, int rule, int target, int teacher,
+                            const float coefficients[5], uint32_t present, int time_window,
+                            int *index_out);
+
 /* In-memory variant: CSR by post row.  row_ptr has postNum+1 entries (int64),
  * col holds population-local pre ids, val the per-synapse multipliers of
  * |initial_weight| (NULL = all 1.0, the reference loader's value). */
@@ -285,10 +318,16 @@ typedef struct {
                                   multipliers is multiplied by this (the
                                   generated over the stored multiplier sum of
                                   the projection), else 1                      */
+    int32_t plastic;           /* 1: a plasticity rule updates the multipliers */
+    int32_t grid_exp;          /* plastic: updated multipliers are rounded to
+                                  multiples of 2^grid_exp (keeps row sums exact
+                                  in f64); set at initialization               */
 } scaffold_projection_info;
 int scaffold_get_projection(scaffold_env *env, int index, scaffold_projection_info *out);
 /* Copy the CSR of a projection (row_ptr: post_num+1, col, val: nnz).  val is the
- * multiplier as the engine applies it (after BF16/FP8 rounding when selected). */
+ * multiplier as the engine applies it (after BF16/FP8 rounding when selected;
+ * for a plastic projection after initialization, its current value on the
+ * device). */
 int scaffold_get_projection_csr(scaffold_env *env, int index,
                                 int64_t *row_ptr, int32_t *col, float *val);
 

@@ -95,6 +95,11 @@ def lib():
                                        ctypes.c_double]
         L.orc_ref_step.argtypes = [ctypes.c_float]
         L.orc_stim_entry_fires.argtypes = [P(OrcStim), ctypes.c_float, ctypes.c_int, ctypes.c_int, i64]
+        L.orc_add_plasticity.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, P(ctypes.c_float),
+                                         ctypes.c_uint, ctypes.c_int, ctypes.c_int]
+        L.orc_add_plasticity.restype = ctypes.c_int
+        L.orc_get_multipliers.argtypes = [ctypes.c_void_p, ctypes.c_int, P(ctypes.c_float)]
+        L.orc_get_multipliers.restype = None
         _lib = L
     return _lib
 
@@ -288,6 +293,25 @@ class OracleSim:
         lib().orc_set_state(self._h, None if V is None else V.ctypes.data_as(P(ctypes.c_float)),
                             None if g is None else g.ctypes.data_as(P(ctypes.c_float)),
                             None if refr is None else refr.ctypes.data_as(P(ctypes.c_int32)))
+
+    def add_plasticity(self, rule, target, teacher=-1, coefficients=None, window=50, grid_exp=-126):
+        """A plasticity rule (SynapticPlasticity.cu:38-121); `coefficients` maps slots
+        0..3 (A0, A1_I, A1_J, A2_JI) to constants, absent slots are unset."""
+        c = (ctypes.c_float * 4)()
+        present = 0
+        for slot, v in (coefficients or {}).items():
+            if slot < 4:
+                c[slot] = v
+                present |= 1 << slot
+        if lib().orc_add_plasticity(self._h, rule, target, teacher, c, present, window, grid_exp) != 0:
+            raise ValueError("bad plasticity rule")
+
+    def multipliers(self, proj):
+        pr = self._projs[proj]
+        nnz = pr.row_ptr[self._pops[pr.post].num]
+        out = np.zeros(max(nnz, 1), dtype=np.float32)
+        lib().orc_get_multipliers(self._h, proj, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+        return out[:nnz]
 
     def spike_row(self, n):
         p = lib().orc_spike_row(self._h, n)

@@ -107,6 +107,10 @@ struct orc_sim {
     uint8_t  *ring;            /* R x total bytes (GPUScaffoldSimulation.cu:361) */
     int       threads;         /* OpenMP threads of orc_step (1: serial)  */
     uint8_t  *active;          /* driven mode: 1 = simulated here; NULL = all */
+    /* plasticity rules and the multipliers they own (per projection, NULL: not plastic) */
+    int       n_plast;
+    struct orc_plast { int rule, target, teacher, window, grid_exp; unsigned present; float coef[4]; } *plast;
+    float   **own_val;
 };
 
 /* Threads used by orc_step for propagation and the neuron update (both are
@@ -171,6 +175,10 @@ void orc_destroy(orc_sim *s)
     free(s->u); free(s->g); free(s->dge); free(s->dgi);
     free(s->u64); free(s->g64); free(s->dge64); free(s->dgi64);
     free(s->active);
+    if (s->own_val)
+        for (int i = 0; i < s->n_projs; ++i) free(s->own_val[i]);
+    free(s->own_val);
+    free(s->plast);
     free(s);
 }
 
@@ -360,6 +368,7 @@ int orc_lif_step_f64(const orc_pop *p, double *u, double *g, int32_t *refr,
  *      dg (:746-751; OUTPUT populations run the same rule on the host,
  *      GPUScaffoldSimulation.cu:41-46, Simulation_header.cu:385-465). */
 static void step_impl(orc_sim *s, int n, const uint8_t *ext);
+static void plasticity_step(orc_sim *s, int n);
 
 void orc_step(orc_sim *s, int n) { step_impl(s, n, NULL); }
 
@@ -461,6 +470,87 @@ static void step_impl(orc_sim *s, int n, const uint8_t *ext)
 #pragma omp parallel for schedule(static) num_threads(s->threads) if (s->threads > 1)
         for (int i = 0; i < s->total; ++i)
             if (!act || !act[i]) out[i] = ext[i];
+    }
+    /* 4. plasticity (invoke_stdp_plasticity, GPUScaffoldSimulation.cu:754) */
+    if (s->n_plast > 0) plasticity_step(s, n);
+}
+
+/* Plasticity.  Pinning: tests/test_oracle_plasticity.py drives spike trains
+ * (driven mode) and checks the multipliers against dw counted by hand from the
+ * rule text (A0, A1_J, A1_I, the A2_JI window count), both clamps, the window
+ * edges, the ring wrap, the teacher's delay and an empty teacher row, and the
+ * grid rounding. */
+int orc_add_plasticity(orc_sim *s, int rule, int target, int teacher, const float coef[4],
+                       unsigned present, int window, int grid_exp)
+{
+    if (target < 0 || target >= s->n_projs || (rule == 1 && (teacher < 0 || teacher >= s->n_projs))) return 1;
+    struct orc_plast *p = (struct orc_plast *)realloc(s->plast, sizeof(*p) * (size_t)(s->n_plast + 1));
+    if (!p) return 1;
+    s->plast = p;
+    struct orc_plast *r = &s->plast[s->n_plast++];
+    r->rule = rule; r->target = target; r->teacher = teacher; r->window = window;
+    r->grid_exp = grid_exp; r->present = present;
+    for (int c = 0; c < 4; ++c) r->coef[c] = coef[c];
+    if (!s->own_val) s->own_val = (float **)calloc((size_t)s->n_projs, sizeof(float *));
+    orc_proj *pr = &s->projs[target];
+    if (!s->own_val[target]) {
+        const int64_t nnz = pr->row_ptr[s->pops[pr->post].num];
+        float *v = (float *)malloc(sizeof(float) * (size_t)(nnz > 0 ? nnz : 1));
+        for (int64_t e = 0; e < nnz; ++e) v[e] = pr->val ? pr->val[e] : 1.f;
+        s->own_val[target] = v;
+        pr->val = v;
+    }
+    return 0;
+}
+
+void orc_get_multipliers(const orc_sim *s, int proj, float *out)
+{
+    const orc_proj *pr = &s->projs[proj];
+    const int64_t nnz = pr->row_ptr[s->pops[pr->post].num];
+    for (int64_t e = 0; e < nnz; ++e) out[e] = pr->val ? pr->val[e] : 1.f;
+}
+
+/* Hebb_plasticity / Teacher_plasticity (SynapticPlasticity.cu:38-121) after step n:
+ * dw summed in float in the kernels' order, then the clamp (Hebb [0, 1], :73;
+ * Teacher >= 0, :117) and the grid rounding. */
+static void plasticity_step(orc_sim *s, int n)
+{
+    const int row = orc_row(s, n);
+    const uint8_t *cur = s->ring + (size_t)row * (size_t)s->total;
+    for (int r = 0; r < s->n_plast; ++r) {
+        const struct orc_plast *P = &s->plast[r];
+        const orc_proj *pr = &s->projs[P->target];
+        float *val = s->own_val[P->target];
+        const int pre_base = s->base[pr->pre], post_base = s->base[pr->post];
+        const int post_num = s->pops[pr->post].num;
+        const uint8_t *pre_row = s->ring + (size_t)orc_row(s, n - pr->delay) * (size_t)s->total;
+        const orc_proj *tp = P->rule == 1 ? &s->projs[P->teacher] : NULL;
+        const uint8_t *t_row = tp ? s->ring + (size_t)orc_row(s, n - tp->delay) * (size_t)s->total : NULL;
+        for (int j = 0; j < post_num; ++j) {
+            int s_post;
+            if (!tp) {
+                s_post = cur[post_base + j];
+            } else {
+                const int64_t t0 = tp->row_ptr[j], t1 = tp->row_ptr[j + 1];
+                s_post = t0 < t1 ? t_row[s->base[tp->pre] + tp->col[t0]] : 0;
+            }
+            for (int64_t e = pr->row_ptr[j]; e < pr->row_ptr[j + 1]; ++e) {
+                const int pre = pre_base + pr->col[e];
+                const int s_pre = pre_row[pre];
+                float dw = 0.f;
+                if (P->present & 1u) dw += P->coef[0];
+                if ((P->present & 4u) && s_pre) dw += P->coef[2];
+                if ((P->present & 2u) && s_post) dw += P->coef[1];
+                if ((P->present & 8u) && s_post)
+                    for (int k = 0; k < P->window; ++k)
+                        if (s->ring[(size_t)orc_row(s, n - pr->delay - k) * (size_t)s->total + (size_t)pre])
+                            dw += P->coef[3];
+                float w = val[e] + dw;
+                if (P->rule == 0) w = w < 0.f ? 0.f : (w > 1.f ? 1.f : w);
+                else w = w < 0.f ? 0.f : w;
+                val[e] = ldexpf(rintf(ldexpf(w, -P->grid_exp)), P->grid_exp);
+            }
+        }
     }
 }
 

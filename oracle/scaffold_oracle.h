@@ -97,6 +97,18 @@ void orc_set_threads(orc_sim *s, int threads);
 void orc_set_active(orc_sim *s, const uint8_t *active);
 void orc_step_driven(orc_sim *s, int n, const uint8_t *ext);
 void orc_set_freq(orc_sim *s, const float *freq_list);
+/* Plasticity (Hebb_plasticity / Teacher_plasticity, SynapticPlasticity.cu:38-121;
+ * rule 0 Hebb, 1 Teacher), applied after every step to the multipliers of
+ * projection `target`, rules in the order added.  coef[c] (A0, A1_I, A1_J, A2_JI)
+ * is used when bit c of `present` is set.  The teacher of post neuron j is the
+ * first entry of row j of projection `teacher`.  Updated multipliers are rounded
+ * to multiples of 2^grid_exp (the engine's rule that keeps row sums exact; the
+ * reference stores w + dw unrounded).  The oracle keeps its own copy of the
+ * target's multipliers (1.0 where val is NULL). */
+int  orc_add_plasticity(orc_sim *s, int rule, int target, int teacher, const float coef[4],
+                        unsigned present, int window, int grid_exp);
+/* Current multipliers of projection `proj` (nnz floats). */
+void orc_get_multipliers(const orc_sim *s, int proj, float *out);
 
 /* State access (FP32 oracle: float arrays; FP64 oracle: the *_f64 calls). */
 void orc_get_state(const orc_sim *s, float *V, float *g, int32_t *refr);

@@ -25,9 +25,10 @@ def orc():
     return oracle
 
 
-def new_env(pkg, preset=0, **overrides):
-    """An environment holding the synthetic scaffold `preset` with config overrides."""
-    env = pkg.NetworkEnvironment(1, 7, 16, 2, 0)
+def new_env(pkg, preset=0, plasticities=0, **overrides):
+    """An environment holding the synthetic scaffold `preset` with config overrides
+    (room for `plasticities` plasticity rules)."""
+    env = pkg.NetworkEnvironment(1, 7, 16, 2, plasticities)
     cfg = pkg.synth_preset(preset)
     for k, v in overrides.items():
         setattr(cfg, k, v)

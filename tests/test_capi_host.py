@@ -41,7 +41,7 @@ def test_version_and_error_text(pkg):
 
 
 @pytest.mark.parametrize("args,code", [((1, 0, 1, 1, 0), "ERR_INVALID_ARG"), ((1, 7, -1, 1, 0), "ERR_INVALID_ARG"),
-                                       ((1, 7, 16, 2, 1), "ERR_UNSUPPORTED"), ((1, 33, 1, 1, 0), "ERR_INVALID_ARG")])
+                                       ((1, 7, 16, 2, -1), "ERR_INVALID_ARG"), ((1, 33, 1, 1, 0), "ERR_INVALID_ARG")])
 def test_create_validation(pkg, args, code):
     with pytest.raises(pkg.ScaffoldError) as e:
         pkg.NetworkEnvironment(*args)
@@ -296,3 +296,30 @@ def test_autotune_argument_checks(pkg):
         assert e.value.code in (pkg.ERR_NO_DEVICE, pkg.ERR_HIP)
         # a failed tuning leaves the options as they were
         assert env.get_option("lane_synapses") == 16 and env.get_option("lane_max_fanin") == 48
+
+
+def test_plasticity_registration(pkg):
+    """scaffold_set_plasticity validates the rule table before any GPU work."""
+    env = new_env(pkg, 0)
+    with pytest.raises(pkg.ScaffoldError) as e:   # table of size 0
+        env.set_plasticity(pkg.STDP_HEBB, 0, coefficients={pkg.A2_JI: -0.01})
+    assert e.value.code == pkg.ERR_CAPACITY
+    env = new_env(pkg, 0, plasticities=2)
+    idx = {env.projection(i).type: i for i in range(16)}   # reference connection type -> index
+    pf_pc, bc_pc, goc_grc = idx[0], idx[11], idx[15]
+    assert env.projection(pf_pc).post == env.projection(bc_pc).post
+    bad = [dict(rule=2, target=pf_pc), dict(rule=pkg.STDP_HEBB, target=99),
+           dict(rule=pkg.STDP_TEACHER, target=pf_pc, teacher=-1),
+           dict(rule=pkg.STDP_TEACHER, target=pf_pc, teacher=goc_grc),   # another post population
+           dict(rule=pkg.STDP_HEBB, target=pf_pc, time_window=0),
+           dict(rule=pkg.STDP_HEBB, target=pf_pc, coefficients={pkg.A0: float("nan")})]
+    for kw in bad:
+        with pytest.raises(pkg.ScaffoldError) as e:
+            env.set_plasticity(**kw)
+        assert e.value.code == pkg.ERR_INVALID_ARG, kw
+    assert env.set_plasticity(pkg.STDP_TEACHER, pf_pc, bc_pc, {pkg.A1_J: 0.0, pkg.A2_JI: -0.01}) == 0
+    assert env.set_plasticity(pkg.STDP_HEBB, goc_grc, coefficients={pkg.A1_I: 0.01}) == 1
+    assert env.projection(pf_pc).plastic == 1 and env.projection(bc_pc).plastic == 0
+    with pytest.raises(pkg.ScaffoldError) as e:
+        env.set_plasticity(pkg.STDP_HEBB, goc_grc)
+    assert e.value.code == pkg.ERR_CAPACITY

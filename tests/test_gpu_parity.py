@@ -522,3 +522,44 @@ def test_autotune_then_bit_exact(pkg, orc):
     run_and_compare(env, ref, 0, 100)
     assert_same_state(env, ref, "after autotune")
     env.finalize()
+
+
+@pytest.mark.parametrize("index_bits,precision", [(32, 0), (16, 1)])
+def test_plasticity_bit_exact(pkg, orc, index_bits, precision):
+    """Synaptic plasticity on the GPU against the oracle, 200 steps: the
+    reference's parallel-fibre-to-Purkinje Teacher rule (A1_J 0, A2_JI -0.01,
+    window 50; SynapticPlasticity.cu:10-16) with basket-to-Purkinje as the
+    teacher projection (team-mode rows), and a Hebb rule with all four terms on
+    Golgi-to-granule (lane-mode slices).  Rasters every step, the state and every
+    updated multiplier must match bit for bit; 16-bit columns move the team rows'
+    multipliers into chunk order, BF16 elsewhere leaves the plastic post
+    populations in f32."""
+    env = new_env(pkg, 0, plasticities=2)
+    idx = {env.projection(i).type: i for i in range(env.info().n_projections)}
+    pf_pc, bc_pc, goc_grc = idx[0], idx[11], idx[15]
+    teacher = dict(rule=pkg.STDP_TEACHER, target=pf_pc, teacher=bc_pc,
+                   coefficients={pkg.A1_J: 0.0, pkg.A2_JI: -0.01}, time_window=50)
+    hebb = dict(rule=pkg.STDP_HEBB, target=goc_grc, teacher=-1,
+                coefficients={pkg.A0: -1e-4, pkg.A1_J: 2e-3, pkg.A1_I: -1e-3, pkg.A2_JI: 5e-4}, time_window=20)
+    for r in (teacher, hebb):
+        env.set_plasticity(**r)
+    env.set_option("index_bits", index_bits)
+    env.set_option("weight_precision", pkg.WEIGHTS_BF16 if precision else pkg.WEIGHTS_F32)
+    env.initialize(50, 25)
+    assert env.projection(pf_pc).lane_mode == 0 and env.projection(goc_grc).lane_mode == 1
+    ref = orc.OracleSim.from_env(env)
+    ref.initialize()
+    w0 = {}
+    for r in (teacher, hebb):
+        p = env.projection(r["target"])
+        assert p.plastic == 1 and p.grid_exp < -30
+        ref.add_plasticity(r["rule"], r["target"], r["teacher"], r["coefficients"], r["time_window"], p.grid_exp)
+        w0[r["target"]] = env.projection_csr(r["target"])[2].copy()
+    run_and_compare(env, ref, 0, 200, check_counts=False)
+    assert_same_state(env, ref, "plasticity 200 steps")
+    for t in (pf_pc, goc_grc):
+        got, want = env.projection_csr(t)[2], ref.multipliers(t)
+        bad = np.nonzero(bits(got) != bits(want))[0]
+        assert bad.size == 0, f"projection {t}: multipliers differ at {bad[:10]}: {got[bad[:5]]} vs {want[bad[:5]]}"
+        assert (got != w0[t]).mean() > 0.01, f"projection {t}: too few multipliers changed"
+    env.finalize()
