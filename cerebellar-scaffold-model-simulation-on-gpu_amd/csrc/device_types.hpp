@@ -206,22 +206,36 @@ struct alignas(16) ScState {
 
 // One plasticity rule over the synapses of its target projection
 // (Hebb_plasticity / Teacher_plasticity, SynapticPlasticity.cu:38-121), applied
-// after every exchange step.  Per synapse (source order): the presynaptic device
-// bit, the postsynaptic device bit (Hebb) or the teacher's device bit (Teacher;
-// -1: none), and the index of its multiplier in `val` (the f32 multiplier array
-// of the layout that holds the projection).
-#define SC_STDP_HEBB    0
-#define SC_STDP_TEACHER 1
+// after every exchange step in two launches.  k_plastic_hist writes, per neuron of
+// the presynaptic population, one byte `hist`: bits 0..6 count its spikes at
+// n - delay - s, s < window (window <= 127), bit 7 is its spike at n - delay.
+// k_plastic then updates the synapses (source order, grouped by post row) in
+// chunks of at most SC_PLASTIC_CHUNK entries of one row, one wavefront each; a
+// chunk carries its row's trigger: the postsynaptic device bit (Hebb) or the
+// teacher's device bit (Teacher; -1: none).  Per synapse: the presynaptic
+// population-local id and the index of its multiplier in `val` (the f32
+// multiplier array of the layout that holds the projection).  skip_quiet: a
+// synapse whose trigger did not spike keeps its multiplier (no A0 term, a zero
+// or absent A1_J, every multiplier inside the clamp range), so quiet chunks stop
+// after reading the trigger.
+#define SC_STDP_HEBB     0
+#define SC_STDP_TEACHER  1
+#define SC_PLASTIC_CHUNK 256
+#define SC_PLASTIC_WINDOW_MAX 127
 struct ScPlastic {
-    const uint32_t *pre;
-    const int32_t  *post;     // Hebb: post bit; Teacher: teacher bit or -1
+    const uint32_t *pre;       // population-local presynaptic id
     const uint32_t *vidx;
+    const uint32_t *ch_begin;  // n_chunks + 1 synapse offsets
+    const int32_t  *ch_trig;   // per chunk
     float *val;
-    int64_t n;
+    uint8_t *hist;             // per presynaptic neuron (groups x 64 bytes)
+    int32_t n_chunks;
+    int32_t pre_group0, pre_groups;  // the presynaptic population's 64-neuron groups
     int32_t rule, delay, t_delay, window;
-    int32_t present;          // bit c: coefficient c is set (A0, A1_I, A1_J, A2_JI)
-    int32_t grid_exp;         // updated multipliers are rounded to multiples of 2^grid_exp
-    float coef[4];            // A0, A1_I, A1_J, A2_JI
+    int32_t present;           // bit c: coefficient c is set (A0, A1_I, A1_J, A2_JI)
+    int32_t grid_exp;          // updated multipliers are rounded to multiples of 2^grid_exp
+    int32_t skip_quiet;
+    float coef[4];             // A0, A1_I, A1_J, A2_JI
 };
 
 struct ScInstance {

@@ -431,18 +431,27 @@ int engine_create(Engine **out, Plan &P)
     e->w.plastic.clear();
     for (const PlasticPlan &X : P.plastic) {
         ScPlastic d{};
-        uint32_t *pre, *vidx;
-        int32_t *post;
+        uint32_t *pre, *vidx, *cb;
+        int32_t *ct;
         if ((rc = dupload(A, B, &pre, X.pre))) return bail(rc);
-        if ((rc = dupload(A, B, &post, X.post))) return bail(rc);
         if ((rc = dupload(A, B, &vidx, X.vidx))) return bail(rc);
+        if ((rc = dupload(A, B, &cb, X.ch_begin))) return bail(rc);
+        if ((rc = dupload(A, B, &ct, X.ch_trig))) return bail(rc);
         const uint8_t *v = X.team ? tval[X.owner] : pval[X.owner];
         if (!v && !X.pre.empty()) return bail(fail(SCAFFOLD_ERR_STATE, "plastic projection without multipliers"));
+        const ScPop &pp = P.pops[P.projs[X.rule.target].pre_pop];
+        uint8_t *hist;
+        if ((rc = dalloc(A, B, &hist, (size_t)(pp.num + SC_WAVE - 1) / SC_WAVE * SC_WAVE))) return bail(rc);
+        d.hist = hist;
+        d.pre_group0 = pp.group_base;
+        d.pre_groups = (pp.num + SC_WAVE - 1) / SC_WAVE;
         d.pre = pre;
-        d.post = post;
         d.vidx = vidx;
+        d.ch_begin = cb;
+        d.ch_trig = ct;
         d.val = const_cast<float *>(reinterpret_cast<const float *>(v));
-        d.n = (int64_t)X.pre.size();
+        d.n_chunks = (int32_t)X.ch_trig.size();
+        d.skip_quiet = X.skip_quiet ? 1 : 0;
         d.rule = X.rule.rule;
         d.delay = X.delay;
         d.t_delay = X.t_delay;

@@ -1073,46 +1073,87 @@ __global__ void k_zero_g(ScState *st, int n)
 // Fired-neuron list of one ring row: the spike words are the ballots of spike
 // detection; one block popcounts them, prefix-sums the counts and writes the
 // reference global ids in ascending order.
+// Window counts of one plasticity rule after exchange step n (launch offset k):
+// one wavefront per 64-neuron group of the presynaptic population, one lane per
+// neuron.  The group's word of each window row is one wave-uniform load; the
+// loads go out 16 at a time.
+__global__ void __launch_bounds__(256) k_plastic_hist(ScInstance I, ScPlastic P, int k)
+{
+    const int g = uniform((int)blockIdx.x * 4 + (int)(threadIdx.x / SC_WAVE));
+    const int lane = threadIdx.x & (SC_WAVE - 1);
+    if (g >= P.pre_groups) return;
+    const int R = I.R;
+    const int pre_t = row_back((I.ctl->row + k) % R, P.delay, R);
+    const uint64_t *col = I.ring + (P.pre_group0 + g);
+    int cnt = 0, s0 = 0;
+    for (int b = 0; b < P.window; b += 16) {
+        uint64_t w[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) w[u] = b + u < P.window ? col[(int64_t)row_back(pre_t, b + u, R) * I.Wr] : 0ull;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) cnt += (int)((w[u] >> lane) & 1ull);
+        if (b == 0) s0 = (int)(w[0] >> lane) & 1;
+    }
+    P.hist[(int64_t)g * SC_WAVE + lane] = (uint8_t)(cnt | (s0 << 7));
+}
+
 // One plasticity rule after exchange step n (launch offset k): Hebb_plasticity /
-// Teacher_plasticity (SynapticPlasticity.cu:38-121), one thread per synapse of
-// the target projection.  dw is summed in float in the reference's order (A0;
-// A1_J if the presynaptic neuron spiked at n - delay; A1_I if the postsynaptic
-// neuron spiked at n (Hebb) or the teacher at n - t_delay (Teacher); with that
-// spike, A2_JI once for every spike of the presynaptic neuron at n - delay - s,
-// s = 0 .. window-1); then w + dw is clamped (Hebb to [0, 1], Teacher to >= 0)
-// and rounded to a multiple of 2^grid_exp, which keeps every row sum of the
-// projection exact in f64 (so the order of the gather's additions cannot change
-// it).  Rows of step n and earlier are complete: this launch follows the step's.
+// Teacher_plasticity (SynapticPlasticity.cu:38-121), one wavefront per chunk of
+// synapses of one post row, 4 synapses per lane (their loads in flight
+// together).  dw is summed in float in the
+// reference's order: A0; A1_J if the presynaptic neuron spiked at n - delay; A1_I
+// if the row's trigger spiked -- the postsynaptic neuron at n (Hebb) or the
+// teacher at n - t_delay (Teacher); with that spike, A2_JI once for every spike
+// of the presynaptic neuron at n - delay - s, s < window (all A2_JI terms are the
+// same constant, so their count fixes the float sum).  Then w + dw is clamped
+// (Hebb to [0, 1], Teacher to >= 0) and rounded to a multiple of 2^grid_exp,
+// which keeps every row sum of the projection exact in f64 (so the order of the
+// gather's additions cannot change it).
 __global__ void __launch_bounds__(256) k_plastic(ScInstance I, ScPlastic P, int k)
 {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= P.n) return;
+    const int c = uniform((int)blockIdx.x * 4 + (int)(threadIdx.x / SC_WAVE));
+    const int lane = threadIdx.x & (SC_WAVE - 1);
+    if (c >= P.n_chunks) return;
     const int R = I.R;
     const int row = (I.ctl->row + k) % R;
-    const uint64_t *ring = I.ring;
-    const auto spiked = [&](int r, uint32_t b) {
-        return (int)((ring[(int64_t)r * I.Wr + (b >> 6)] >> (b & 63)) & 1ull);
-    };
-    const uint32_t pre = P.pre[i];
-    const int pre_t = row_back(row, P.delay, R);
-    const int s_pre = spiked(pre_t, pre);
-    const int32_t pb = P.post[i];
+    const int32_t trig = P.ch_trig[c];
     int s_post = 0;
-    if (P.rule == SC_STDP_HEBB) s_post = spiked(row, (uint32_t)pb);
-    else if (pb >= 0) s_post = spiked(row_back(row, P.t_delay, R), (uint32_t)pb);
-    float *vp = P.val + P.vidx[i];
-    const float w = *vp;
-    float dw = 0.f;
-    if (P.present & 1) dw += P.coef[0];
-    if ((P.present & 4) && s_pre) dw += P.coef[2];
-    if ((P.present & 2) && s_post) dw += P.coef[1];
-    if ((P.present & 8) && s_post)
-        for (int s = 0; s < P.window; ++s)
-            if (spiked(row_back(pre_t, s, R), pre)) dw += P.coef[3];
-    float nw = w + dw;
-    if (P.rule == SC_STDP_HEBB) nw = nw < 0.f ? 0.f : nw > 1.f ? 1.f : nw;
-    else nw = nw < 0.f ? 0.f : nw;
-    *vp = ldexpf(rintf(ldexpf(nw, -P.grid_exp)), P.grid_exp);
+    if (trig >= 0) {
+        const int r = P.rule == SC_STDP_HEBB ? row : row_back(row, P.t_delay, R);
+        s_post = (int)((I.ring[(int64_t)r * I.Wr + (trig >> 6)] >> (trig & 63)) & 1ull);
+    }
+    s_post = uniform(s_post);
+    if (P.skip_quiet && !s_post) return;
+    const uint32_t b0 = P.ch_begin[c], b1 = P.ch_begin[c + 1];
+    constexpr int U = SC_PLASTIC_CHUNK / SC_WAVE;
+    uint32_t h[U], vi[U];
+    float w[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint32_t i = b0 + lane + u * SC_WAVE;
+        h[u] = 0u;
+        vi[u] = 0u;
+        if (i < b1) {
+            vi[u] = P.vidx[i];
+            h[u] = P.hist[P.pre[i]];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) w[u] = b0 + lane + u * SC_WAVE < b1 ? P.val[vi[u]] : 0.f;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if (b0 + lane + u * SC_WAVE >= b1) continue;
+        float dw = 0.f;
+        if (P.present & 1) dw += P.coef[0];
+        if ((P.present & 4) && (h[u] >> 7)) dw += P.coef[2];
+        if ((P.present & 2) && s_post) dw += P.coef[1];
+        if ((P.present & 8) && s_post)
+            for (int q = 0; q < (int)(h[u] & 127u); ++q) dw += P.coef[3];
+        float nw = w[u] + dw;
+        if (P.rule == SC_STDP_HEBB) nw = nw < 0.f ? 0.f : nw > 1.f ? 1.f : nw;
+        else nw = nw < 0.f ? 0.f : nw;
+        P.val[vi[u]] = ldexpf(rintf(ldexpf(nw, -P.grid_exp)), P.grid_exp);
+    }
 }
 
 __global__ void __launch_bounds__(1024) k_compact(const uint64_t *__restrict__ row, int n_groups,
@@ -1211,8 +1252,11 @@ void launch_zero_g(hipStream_t s, ScState *st, int n)
 
 void launch_plastic(hipStream_t s, const ScInstance &I, const DevWork &w, int k)
 {
-    for (const ScPlastic &P : w.plastic)
-        if (P.n > 0) k_plastic<<<(unsigned)((P.n + 255) / 256), 256, 0, s>>>(I, P, k);
+    for (const ScPlastic &P : w.plastic) {
+        if (P.n_chunks <= 0) continue;
+        if (P.pre_groups > 0) k_plastic_hist<<<(unsigned)((P.pre_groups + 3) / 4), 256, 0, s>>>(I, P, k);
+        k_plastic<<<(unsigned)((P.n_chunks + 3) / 4), 256, 0, s>>>(I, P, k);
+    }
 }
 
 void launch_copy(hipStream_t s, const void *src, void *dst, int64_t n16, int grid)

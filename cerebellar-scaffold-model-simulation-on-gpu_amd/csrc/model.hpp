@@ -125,18 +125,20 @@ struct TeamPop {
 };
 
 // A plasticity rule lowered to the device layout: per synapse of the target
-// projection (source order) its presynaptic device bit, its postsynaptic device
-// bit (Hebb) or teacher device bit (Teacher; -1: none) and the index of its
+// projection (source order) its presynaptic device bit and the index of its
 // multiplier in the f32 array that holds it (a lane-mode projection's slices or
-// a team-mode population's merged rows).
+// a team-mode population's merged rows); per chunk of one row's synapses the
+// row's trigger bit.
 struct PlasticPlan {
     Plasticity rule;
     int delay = 0, t_delay = 0, grid_exp = 0;
+    bool skip_quiet = false;  // synapses whose trigger did not spike keep their multiplier
     bool team = false;
-    int owner = -1;          // team index (team) or projection index (lane)
-    int64_t n_val = 0;       // f32 elements of the owner's multiplier array
+    int owner = -1;           // team index (team) or projection index (lane)
+    int64_t n_val = 0;        // f32 elements of the owner's multiplier array
     std::vector<uint32_t> pre, vidx;
-    std::vector<int32_t> post;
+    std::vector<uint32_t> ch_begin;  // chunks of <= SC_PLASTIC_CHUNK synapses of one row
+    std::vector<int32_t> ch_trig;    // per chunk: post bit (Hebb) / teacher bit (Teacher), -1: none
 };
 
 struct Plan {

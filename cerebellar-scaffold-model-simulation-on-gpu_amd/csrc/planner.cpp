@@ -567,6 +567,8 @@ int plan_build(scaffold_env *env, Plan &P)
         X.delay = sp.delay;
         // the A2_JI window reads rows n - delay - s, s < window; the row of step
         // n + 1 is cleared during step n, so it must lie outside the window
+        if (r.window > SC_PLASTIC_WINDOW_MAX)
+            return fail(SCAFFOLD_ERR_CAPACITY, "plasticity window %d above %d steps", r.window, SC_PLASTIC_WINDOW_MAX);
         if (r.window < 1 || sp.delay + r.window + 2 > P.R)
             return fail(SCAFFOLD_ERR_INVALID_ARG, "plasticity on projection %d: delay %d + window %d needs a spike ring "
                         "of at least %d rows (T_print %d gives %d)", r.target, sp.delay, r.window,
@@ -612,12 +614,20 @@ int plan_build(scaffold_env *env, Plan &P)
             X.n_val = (int64_t)P.teams[X.owner].val.size() / 4;
         }
         const int64_t nnz = (int64_t)src.col.size();
+        if (nnz >= (int64_t)UINT32_MAX) return fail(SCAFFOLD_ERR_CAPACITY, "plastic projection above 2^32 synapses");
         X.pre.resize(nnz);
-        X.post.resize(nnz);
         X.vidx.resize(nnz);
+        X.ch_begin.assign(1, 0);
+        // quiet synapses keep their multiplier when dw = +0 leaves w as it is:
+        // no A0 term, A1_J absent or zero, every multiplier inside the clamp
+        // range and none of them -0
+        bool quiet_keeps = !(r.present & 1u) && (!(r.present & 4u) || r.coef[2] == 0.f);
+        for (float x : ev)
+            quiet_keeps = quiet_keeps && !std::signbit(x) && (r.rule != SC_STDP_HEBB || x <= 1.f);
+        X.skip_quiet = quiet_keeps;
         const ScPop &pre = P.pops[src.pre], &post = P.pops[src.post];
         for (size_t j = 0; j + 1 < src.row_ptr.size(); ++j) {
-            int32_t pb = (int32_t)(post.dev_base + (int)j);
+            int32_t pb = (int32_t)(post.dev_base + (int)j);  // the row's trigger
             if (tp) {  // the teacher: the first entry of the teacher projection's row (SynapticPlasticity.cu:93)
                 pb = -1;
                 if (j + 1 < tp->row_ptr.size() && tp->row_ptr[j] < tp->row_ptr[j + 1])
@@ -628,9 +638,12 @@ int plan_build(scaffold_env *env, Plan &P)
                 if (v < 0 || v >= X.n_val)  // every synapse has one place in the layout
                     return fail(SCAFFOLD_ERR_STATE, "plasticity on projection %d: synapse %lld has no multiplier slot",
                                 r.target, (long long)e);
-                X.pre[e] = (uint32_t)(pre.dev_base + src.col[e]);
-                X.post[e] = pb;
+                X.pre[e] = (uint32_t)src.col[e];
                 X.vidx[e] = (uint32_t)v;
+            }
+            for (int64_t b = src.row_ptr[j]; b < src.row_ptr[j + 1]; b += SC_PLASTIC_CHUNK) {
+                X.ch_trig.push_back(pb);
+                X.ch_begin.push_back((uint32_t)std::min<int64_t>(b + SC_PLASTIC_CHUNK, src.row_ptr[j + 1]));
             }
         }
         P.plastic.push_back(std::move(X));
