@@ -35,8 +35,8 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_env(pkg, preset, **over):
-    env = pkg.NetworkEnvironment(1, 7, 16, 2, 0)
+def make_env(pkg, preset, plasticities=0, **over):
+    env = pkg.NetworkEnvironment(1, 7, 16, 2, plasticities)
     cfg = pkg.synth_preset(preset)
     for k, v in over.items():
         setattr(cfg, k, v)
@@ -173,12 +173,18 @@ def batched(pkg, n_inst, steps, warmup, threads):
             "aggregate_real_time_factor": len(scales) * steps * 1e-3 / wall, "autotuned": tuned}
 
 
-def plain_variant(pkg, steps, warmup, **options):
+def plain_variant(pkg, steps, warmup, plastic=False, **options):
     """The plain FP32 96.7k scaffold with other storage options (e.g. 16-bit
-    column indices): same results bit for bit, other memory traffic."""
-    env = make_env(pkg, 1)
+    column indices: same results bit for bit, other memory traffic), or with the
+    reference's parallel-fibre-to-Purkinje Teacher rule (A1_J 0, A2_JI -0.01,
+    window 50; SynapticPlasticity.cu:10-16), basket-to-Purkinje standing in for
+    the climbing fibre as the teacher projection."""
+    env = make_env(pkg, 1, plasticities=1 if plastic else 0)
     for k, v in options.items():
         env.set_option(k, v)
+    if plastic:
+        idx = {env.projection(i).type: i for i in range(env.info().n_projections)}
+        env.set_plasticity(pkg.STDP_TEACHER, idx[0], idx[11], {pkg.A1_J: 0.0, pkg.A2_JI: -0.01}, 50)
     env.initialize(50, 25)
     info = env.info()
     advance(env, 0, warmup)
@@ -189,7 +195,8 @@ def plain_variant(pkg, steps, warmup, **options):
     _, ev1 = env.totals()
     _, kmean = env.time_steps(100, kernel_mean=True)
     algo = info.algorithmic_bytes_per_step
-    out = {"options": options, "value": (ev1 - ev0) / wall, "ms_per_step": wall * 1e3 / steps,
+    out = {"options": dict(options, plasticity="teacher pf->pc" if plastic else None),
+           "value": (ev1 - ev0) / wall, "ms_per_step": wall * 1e3 / steps,
            "real_time_factor": steps * 1e-3 / wall, "kernel_mean_ms": kmean, "algorithmic_bytes_per_step": algo,
            "achieved_GBs": algo / (kmean * 1e-3) / 1e9, "synapses_16bit": info.synapses_16bit}
     env.finalize()
@@ -239,6 +246,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-plastic", action="store_true")
     ap.add_argument("--no-16x", action="store_true")
     ap.add_argument("--steps-16x", type=int, default=200)
     ap.add_argument("--no-100x", action="store_true")
@@ -330,6 +338,8 @@ def main():
     env.finalize()
     if not args.no_batched:  # the same network with 16-bit column offsets (bit-identical results)
         line["plain_fp32_idx16"] = plain_variant(pkg, args.steps, args.warmup, index_bits=16)
+    if not args.no_plastic:  # the same network with the reference's Purkinje plasticity rule switched on
+        line["plain_fp32_plastic"] = plain_variant(pkg, args.steps, args.warmup, plastic=True)
     if not args.no_batched:
         b = batched(pkg, args.batch_instances, args.batch_steps, 50, threads=min(16, os.cpu_count() or 4))
         b["vs_plain"] = b["value"] / value
