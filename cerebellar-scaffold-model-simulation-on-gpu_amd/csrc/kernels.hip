@@ -992,15 +992,39 @@ __global__ void __launch_bounds__(256) k_stats(ScInstance I0, const ScInstance *
     const int row = (I.ctl->row + kk) % I.R;
     const uint64_t *__restrict__ w = I.ring + (int64_t)row * I.Wr;
     unsigned long long ev = 0ull;
-    for (int g = g0 + threadIdx.x; g < g1; g += blockDim.x) {
-        unsigned long long word = w[g];
-        if (!word) continue;
-        atomicAdd(&acc[I.groups[g].pop], (unsigned long long)__popcll(word));
+    // popcount into the population's counter, then the out-degrees of the set
+    // bits, 8 loads in flight at a time
+    const auto word_stats = [&](uint64_t word, int pop, int g) {
+        if (!word) return;
+        atomicAdd(&acc[pop], (unsigned long long)__popcll(word));
+        const uint32_t *od = I.outdeg + (int64_t)g * SC_WAVE;
         while (word) {
-            const int bt = __ffsll((long long)word) - 1;
-            word &= word - 1;
-            ev += I.outdeg[g * SC_WAVE + bt];
+            uint32_t v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                v[u] = 0u;
+                if (word) {
+                    v[u] = od[__ffsll((long long)word) - 1];
+                    word &= word - 1;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) ev += v[u];
         }
+    };
+    // two words per thread and pass, their population ids loaded alongside
+    for (int g = g0 + threadIdx.x; g < g1; g += 2 * blockDim.x) {
+        const int gb = g + blockDim.x;
+        const uint64_t wa = w[g];
+        const int pa = I.groups[g].pop;
+        uint64_t wb = 0ull;
+        int pb = 0;
+        if (gb < g1) {
+            wb = w[gb];
+            pb = I.groups[gb].pop;
+        }
+        word_stats(wa, pa, g);
+        word_stats(wb, pb, gb);
     }
     if (ev) atomicAdd(&acc[SC_MAX_POPS], ev);
     __syncthreads();
