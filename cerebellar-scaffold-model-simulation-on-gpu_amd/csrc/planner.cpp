@@ -648,6 +648,11 @@ int plan_build(scaffold_env *env, Plan &P)
         }
         P.plastic.push_back(std::move(X));
     }
+    // rules sharing a target share its grid: the coarsest of theirs, which bounds
+    // every value any of them can produce
+    for (PlasticPlan &X : P.plastic)
+        for (const PlasticPlan &Y : P.plastic)
+            if (Y.rule.target == X.rule.target) X.grid_exp = std::max(X.grid_exp, Y.grid_exp);
     int64_t index_bytes = 0;  // 2 B per synapse in the 16-bit form, else 4 B
     P.syn16 = 0;
     for (int pi = 0; pi < nproj; ++pi) {

@@ -323,3 +323,29 @@ def test_plasticity_registration(pkg):
     with pytest.raises(pkg.ScaffoldError) as e:
         env.set_plasticity(pkg.STDP_HEBB, goc_grc)
     assert e.value.code == pkg.ERR_CAPACITY
+
+
+def test_plasticity_plan_checks(pkg):
+    """The planner's plasticity checks run at initialization, before any device
+    work: the spike ring must hold delay + window + 2 rows, the window fits the
+    one-byte counts, and a Teacher rule must not potentiate."""
+    def env_with(**rule):
+        env = new_env(pkg, 0, plasticities=1)
+        idx = {env.projection(i).type: i for i in range(env.info().n_projections)}
+        env.set_plasticity(pkg.STDP_TEACHER, idx[0], idx[11], **rule)
+        return env
+    cases = [(dict(coefficients={pkg.A2_JI: -0.01}, time_window=50), 20, pkg.ERR_INVALID_ARG),  # ring 40 < 57
+             (dict(coefficients={pkg.A2_JI: -0.01}, time_window=128), 100, pkg.ERR_CAPACITY),
+             (dict(coefficients={pkg.A1_J: 0.01}, time_window=50), 50, pkg.ERR_INVALID_ARG)]
+    for rule, t_print, code in cases:
+        env = env_with(**rule)
+        with pytest.raises(pkg.ScaffoldError) as e:
+            env.initialize(t_print, 10 if t_print == 20 else 25)
+        assert e.value.code == code, (rule, t_print, str(e.value))
+        env.finalize()
+    if not HAS_GPU_NODE:  # a valid rule passes the planner and stops at the device
+        env = env_with(coefficients={pkg.A1_J: 0.0, pkg.A2_JI: -0.01}, time_window=50)
+        with pytest.raises(pkg.ScaffoldError) as e:
+            env.initialize(50, 25)
+        assert e.value.code in (pkg.ERR_NO_DEVICE, pkg.ERR_HIP)
+        env.finalize()
