@@ -432,6 +432,10 @@ int scaffold_set_option(scaffold_env *env, const char *key, int64_t value)
     else if (!strcmp(key, "stage_spikes")) {
         if (frozen) return frozen_err();
         o.stage_spikes = value ? 1 : 0;
+    } else if (!strcmp(key, "nontemporal")) {
+        if (frozen) return frozen_err();
+        if (value < -1 || value > 1) return fail(SCAFFOLD_ERR_INVALID_ARG, "nontemporal is -1, 0 or 1");
+        o.nontemporal = (int)value;
     } else return fail(SCAFFOLD_ERR_INVALID_ARG, "unknown option %s", key);
     return SCAFFOLD_OK;
 }
@@ -452,6 +456,7 @@ int scaffold_get_option(scaffold_env *env, const char *key, int64_t *value)
     else if (!strcmp(key, "graph_steps")) *value = o.graph_steps;
     else if (!strcmp(key, "reset_g")) *value = o.reset_g;
     else if (!strcmp(key, "stage_spikes")) *value = o.stage_spikes;
+    else if (!strcmp(key, "nontemporal")) *value = o.nontemporal;
     else return fail(SCAFFOLD_ERR_INVALID_ARG, "unknown option %s", key);
     return SCAFFOLD_OK;
 }

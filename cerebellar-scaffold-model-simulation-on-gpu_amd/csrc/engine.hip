@@ -463,6 +463,7 @@ int engine_create(Engine **out, Plan &P)
         e->plastic_views.push_back(Engine::PlasticView{X.rule.target, d.val, X.n_val, X.vidx});
     }
     e->w.max_groups = P.n_groups;
+    e->w.nontemporal = P.nontemporal;
     if ((rc = dalloc(A, B, &e->cnt_buf, 1))) return bail(rc);
     std::vector<ScInstance> one{I};
     if ((rc = dupload(A, B, &e->di, one))) return bail(rc);
@@ -782,6 +783,14 @@ int batch_build(Batch **out, std::vector<Engine *> &engines, const std::vector<c
 {
     int stage = 1;  // staged only if every instance allows it
     for (const Plan *p : plans) stage = stage && p->stage;
+    // nontemporal streams when the instances' streams together exceed 64 MB, or
+    // when every instance's own plan loads them so
+    int64_t algo = 0;
+    bool nt_all = true;
+    for (const Plan *p : plans) {
+        algo += p->algo_bytes;
+        nt_all = nt_all && p->nontemporal;
+    }
     *out = nullptr;
     Batch *b = new Batch();
     auto bail = [&](int code) { batch_destroy(b); return code; };
@@ -823,6 +832,7 @@ int batch_build(Batch **out, std::vector<Engine *> &engines, const std::vector<c
     if ((rc = upload_work(b->allocs, nullptr, b->w, tw, lanes, stims, tt, lane_win, team_win, stage != 0)))
         return bail(rc);
     for (const ScInstance &x : insts) b->w.max_groups = std::max(b->w.max_groups, x.n_groups);
+    b->w.nontemporal = nt_all || algo > ((int64_t)64 << 20);
     b->n_inst = (int)insts.size();
     *out = b;
     return SCAFFOLD_OK;

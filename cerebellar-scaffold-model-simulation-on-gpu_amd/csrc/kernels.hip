@@ -87,6 +87,16 @@ __device__ __forceinline__ gptr<T> gbl(const T *p)
     return (gptr<T>)p;
 }
 
+// A load of the synapse streams (columns, headers, multipliers).  NT marks it
+// nontemporal: on networks whose streams exceed the L2, the streamed lines then
+// do not evict the descriptors and spike rows the latency chains wait on.
+template <bool NT, typename T>
+__device__ __forceinline__ T ldg(gptr<T> p)
+{
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
 // 4 and 2 dwords loaded by one 16- / 8-byte load.
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
@@ -132,18 +142,18 @@ template <int WP>
 struct Raw4 {
     uint32_t w[WP == SC_W_F32 ? 4 : WP == SC_W_BF16 ? 2 : 1];
 };
-template <int WP>
+template <int WP, bool NT>
 __device__ __forceinline__ Raw4<WP> wraw4(const void *val, int64_t v)
 {
     Raw4<WP> r;
     if constexpr (WP == SC_W_F32) {
-        const u32x4 x = gbl(static_cast<const u32x4 *>(val))[v];
+        const u32x4 x = ldg<NT>(gbl(static_cast<const u32x4 *>(val)) + v);
         r.w[0] = x.x; r.w[1] = x.y; r.w[2] = x.z; r.w[3] = x.w;
     } else if constexpr (WP == SC_W_BF16) {
-        const u32x2 x = gbl(static_cast<const u32x2 *>(val))[v];
+        const u32x2 x = ldg<NT>(gbl(static_cast<const u32x2 *>(val)) + v);
         r.w[0] = x.x; r.w[1] = x.y;
     } else if constexpr (WP == SC_W_FP8) {
-        r.w[0] = gbl(static_cast<const uint32_t *>(val))[v];
+        r.w[0] = ldg<NT>(gbl(static_cast<const uint32_t *>(val)) + v);
     } else {
         r.w[0] = 0u;
     }
@@ -201,19 +211,20 @@ template <int WP>
 struct Raw8 {
     uint32_t w[WP == SC_W_F32 ? 8 : WP == SC_W_BF16 ? 4 : 2];
 };
-template <int WP>
+template <int WP, bool NT>
 __device__ __forceinline__ Raw8<WP> wraw8(const void *val, int64_t v)
 {
     Raw8<WP> r;
     if constexpr (WP == SC_W_F32) {
-        const u32x4 x = gbl(static_cast<const u32x4 *>(val))[2 * v], y = gbl(static_cast<const u32x4 *>(val))[2 * v + 1];
+        const u32x4 x = ldg<NT>(gbl(static_cast<const u32x4 *>(val)) + 2 * v);
+        const u32x4 y = ldg<NT>(gbl(static_cast<const u32x4 *>(val)) + 2 * v + 1);
         r.w[0] = x.x; r.w[1] = x.y; r.w[2] = x.z; r.w[3] = x.w;
         r.w[4] = y.x; r.w[5] = y.y; r.w[6] = y.z; r.w[7] = y.w;
     } else if constexpr (WP == SC_W_BF16) {
-        const u32x4 x = gbl(static_cast<const u32x4 *>(val))[v];
+        const u32x4 x = ldg<NT>(gbl(static_cast<const u32x4 *>(val)) + v);
         r.w[0] = x.x; r.w[1] = x.y; r.w[2] = x.z; r.w[3] = x.w;
     } else if constexpr (WP == SC_W_FP8) {
-        const u32x2 x = gbl(static_cast<const u32x2 *>(val))[v];
+        const u32x2 x = ldg<NT>(gbl(static_cast<const u32x2 *>(val)) + v);
         r.w[0] = x.x; r.w[1] = x.y;
     } else {
         r.w[0] = r.w[1] = 0u;
@@ -336,7 +347,7 @@ __device__ __forceinline__ uint32_t pair_bit(uint32_t word, int hi, uint32_t bas
 // load each; 16-bit: four column pairs) and two vectors of multipliers.  Vector
 // indices past the slice are clamped to its last vector and the entries past
 // its width masked out of the sum.  k0 is a multiple of 4.
-template <int WP, bool IDX16, bool STAGED>
+template <int WP, bool IDX16, bool STAGED, bool NT>
 __device__ __forceinline__ double slice_tail(const ScSlice &sl, gptr<uint32_t> row32, const uint32_t *win, uint32_t zb,
                                              int k0, int lane)
 {
@@ -350,18 +361,18 @@ __device__ __forceinline__ double slice_tail(const ScSlice &sl, gptr<uint32_t> r
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int v = min((k >> 2) + h, nv - 1);
-            if (!unit) raw[h] = wraw4<WP>(sl.val, (int64_t)v * SC_WAVE + lane);
+            if (!unit) raw[h] = wraw4<WP, NT>(sl.val, (int64_t)v * SC_WAVE + lane);
             else raw[h].w[0] = 0u;
         }
         if constexpr (IDX16) {
             // pairs 2g, 2g+1 of a lane in one 8-byte load; pair headers (stored for
             // the even-padded pair count) in one load, read back with readlane
             gptr<u32x2> col = gbl(static_cast<const u32x2 *>(sl.col));
-            const uint32_t hv = gbl(sl.hdr)[min((k >> 1) + min(lane, 3), 2 * nv - 1)];
+            const uint32_t hv = ldg<NT>(gbl(sl.hdr) + min((k >> 1) + min(lane, 3), 2 * nv - 1));
             uint32_t pw[4];
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                const u32x2 x = col[min((k >> 2) + h, nv - 1) * SC_WAVE + lane];
+                const u32x2 x = ldg<NT>(col + (min((k >> 2) + h, nv - 1) * SC_WAVE + lane));
                 pw[2 * h] = x.x;
                 pw[2 * h + 1] = x.y;
             }
@@ -371,7 +382,7 @@ __device__ __forceinline__ double slice_tail(const ScSlice &sl, gptr<uint32_t> r
             gptr<u32x4> col = gbl(static_cast<const u32x4 *>(sl.col));
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                const u32x4 x = col[min((k >> 2) + h, nv - 1) * SC_WAVE + lane];
+                const u32x4 x = ldg<NT>(col + (min((k >> 2) + h, nv - 1) * SC_WAVE + lane));
                 c[4 * h] = x.x; c[4 * h + 1] = x.y; c[4 * h + 2] = x.z; c[4 * h + 3] = x.w;
             }
         }
@@ -392,7 +403,7 @@ __device__ __forceinline__ double slice_tail(const ScSlice &sl, gptr<uint32_t> r
 // are under way.
 // Slice widths and pointers are wave-uniform; slices without multipliers (unit
 // weights) skip the multiplier loads.
-template <int WP, bool IDX16, bool STAGED, typename Stage>
+template <int WP, bool IDX16, bool STAGED, bool NT, typename Stage>
 __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance &I, const ScPop &P, int row_n,
                                            const Stage &stage, int lane)
 {
@@ -419,7 +430,7 @@ __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance
                 h = t.sl[q].hdr;
                 w = t.sl[q].width;
             }
-        if (2 * hp < w) hv = gbl(h)[hp];
+        if (2 * hp < w) hv = ldg<NT>(gbl(h) + hp);
     }
 #pragma unroll
     for (int s = 0; s < SC_MAX_SLICES; ++s) {
@@ -428,18 +439,18 @@ __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance
         row[s] = gbl(reinterpret_cast<const uint32_t *>(I.ring + (int64_t)(on ? row_back(row_n, t.sl[s].delay, I.R) : 0) * I.Wr));
         const void *val = t.sl[s].val;
         raw[s].w[0] = 0u;
-        if (WP != SC_W_UNIT && val && width > 0) raw[s] = wraw4<WP>(val, lane);
+        if (WP != SC_W_UNIT && val && width > 0) raw[s] = wraw4<WP, NT>(val, lane);
         if constexpr (IDX16) {
             pw[s][0] = pw[s][1] = 0xFFFFFFFFu;
             if (width > 0) {
-                const u32x2 x = gbl(static_cast<const u32x2 *>(t.sl[s].col))[lane];
+                const u32x2 x = ldg<NT>(gbl(static_cast<const u32x2 *>(t.sl[s].col)) + lane);
                 pw[s][0] = x.x;
                 pw[s][1] = x.y;
             }
         } else {
             c[s][0] = c[s][1] = c[s][2] = c[s][3] = zb;
             if (width > 0) {
-                const u32x4 x = gbl(static_cast<const u32x4 *>(t.sl[s].col))[lane];
+                const u32x4 x = ldg<NT>(gbl(static_cast<const u32x4 *>(t.sl[s].col)) + lane);
                 c[s][0] = x.x; c[s][1] = x.y; c[s][2] = x.z; c[s][3] = x.w;
             }
         }
@@ -477,7 +488,7 @@ __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance
 #pragma unroll
     for (int s = 0; s < SC_MAX_SLICES; ++s) {
         if (s >= n_sl) break;
-        if (t.sl[s].width > 4) S[s] += slice_tail<WP, IDX16, STAGED>(t.sl[s], row[s], win, zb, 4, lane);
+        if (t.sl[s].width > 4) S[s] += slice_tail<WP, IDX16, STAGED, NT>(t.sl[s], row[s], win, zb, 4, lane);
         // dg[post] += S * w_bar (Simulation_header.cu:98), projections in index order
         const float Sf = (float)(S[s] * (double)t.sl[s].scale);
         const float add = Sf * t.sl[s].W;
@@ -521,7 +532,7 @@ __device__ __forceinline__ TeamHead team_head(const ScTeamTask *T)
     return TeamHead{T->col, T->hdr, T->val, T->inst, T->group, T->bit, T->pop, T->n_in, T->wprec, T->len};
 }
 
-template <int WP, int U, bool IDX16, bool STAGED>
+template <int WP, int U, bool IDX16, bool STAGED, bool NT>
 __device__ __forceinline__ void team_gather(const TeamHead &T, gptr<uint32_t> ring32, const uint32_t *win,
                                             const uint32_t *rowoff, uint32_t zb, int tid, int TT, double *acc)
 {
@@ -533,9 +544,9 @@ __device__ __forceinline__ void team_gather(const TeamHead &T, gptr<uint32_t> ri
         gptr<u32x4> col = gbl(static_cast<const u32x4 *>(T.col));
         gptr<uint32_t> hdr = gbl(T.hdr);
         for (int b = tid; b < len; b += TT) {
-            const u32x4 x = col[b];
-            const uint32_t h = hdr[b];
-            const Raw8<WP> raw = wraw8<WP>(val, b);
+            const u32x4 x = ldg<NT>(col + b);
+            const uint32_t h = ldg<NT>(hdr + b);
+            const Raw8<WP> raw = wraw8<WP, NT>(val, b);
             const uint32_t base = h & SC_BIT_MASK, ro = rowoff[h >> SC_Q_SHIFT];
             uint32_t bit[8], s[8];
 #pragma unroll
@@ -563,9 +574,9 @@ __device__ __forceinline__ void team_gather(const TeamHead &T, gptr<uint32_t> ri
 #pragma unroll
             for (int u = 0; u < V; ++u) {
                 const int v = min(b + u * TT, nv - 1);
-                const u32x4 x = col[v];
+                const u32x4 x = ldg<NT>(col + v);
                 c[u][0] = x.x; c[u][1] = x.y; c[u][2] = x.z; c[u][3] = x.w;
-                raw[u] = wraw4<WP>(val, v);
+                raw[u] = wraw4<WP, NT>(val, v);
             }
 #pragma unroll
             for (int u = 0; u < V; ++u)
@@ -631,23 +642,23 @@ __device__ __forceinline__ int stim_fires(const ScStim &S, int entry, int64_t n,
 }
 
 // One lane task: storage form and multiplier type of its slices.
-template <bool IDX16, bool STAGED, typename Stage>
+template <bool IDX16, bool STAGED, bool NT, typename Stage>
 __device__ __forceinline__ void lane_dispatch(const ScLaneTask &t, const ScInstance &I, const ScPop &P, int row_n,
                                               const Stage &stage, int lane)
 {
     if (IDX16 && t.idx16) {
         switch (t.wp) {
-        case SC_W_F32: lane_group<SC_W_F32, true, STAGED>(t, I, P, row_n, stage, lane); break;
-        case SC_W_BF16: lane_group<SC_W_BF16, true, STAGED>(t, I, P, row_n, stage, lane); break;
-        case SC_W_FP8: lane_group<SC_W_FP8, true, STAGED>(t, I, P, row_n, stage, lane); break;
-        default: lane_group<SC_W_UNIT, true, STAGED>(t, I, P, row_n, stage, lane); break;
+        case SC_W_F32: lane_group<SC_W_F32, true, STAGED, NT>(t, I, P, row_n, stage, lane); break;
+        case SC_W_BF16: lane_group<SC_W_BF16, true, STAGED, NT>(t, I, P, row_n, stage, lane); break;
+        case SC_W_FP8: lane_group<SC_W_FP8, true, STAGED, NT>(t, I, P, row_n, stage, lane); break;
+        default: lane_group<SC_W_UNIT, true, STAGED, NT>(t, I, P, row_n, stage, lane); break;
         }
     } else {
         switch (t.wp) {
-        case SC_W_F32: lane_group<SC_W_F32, false, STAGED>(t, I, P, row_n, stage, lane); break;
-        case SC_W_BF16: lane_group<SC_W_BF16, false, STAGED>(t, I, P, row_n, stage, lane); break;
-        case SC_W_FP8: lane_group<SC_W_FP8, false, STAGED>(t, I, P, row_n, stage, lane); break;
-        default: lane_group<SC_W_UNIT, false, STAGED>(t, I, P, row_n, stage, lane); break;
+        case SC_W_F32: lane_group<SC_W_F32, false, STAGED, NT>(t, I, P, row_n, stage, lane); break;
+        case SC_W_BF16: lane_group<SC_W_BF16, false, STAGED, NT>(t, I, P, row_n, stage, lane); break;
+        case SC_W_FP8: lane_group<SC_W_FP8, false, STAGED, NT>(t, I, P, row_n, stage, lane); break;
+        default: lane_group<SC_W_UNIT, false, STAGED, NT>(t, I, P, row_n, stage, lane); break;
         }
     }
 }
@@ -664,23 +675,23 @@ __device__ __forceinline__ const ScInstance &inst_of(const ScInstance &I0, const
 }
 
 // Gather of one team: storage form and multiplier type of its row.
-template <bool IDX16, bool STAGED>
+template <bool IDX16, bool STAGED, bool NT>
 __device__ __forceinline__ void team_dispatch(const TeamHead &T, gptr<uint32_t> ring32, const uint32_t *win,
                                               const uint32_t *rowoff, uint32_t zb, int tid, int TT, double *acc)
 {
     if (IDX16 && T.hdr) {
         switch (T.wprec) {
-        case SC_W_F32: team_gather<SC_W_F32, 8, true, STAGED>(T, ring32, win, rowoff, zb, tid, TT, acc); break;
-        case SC_W_BF16: team_gather<SC_W_BF16, 8, true, STAGED>(T, ring32, win, rowoff, zb, tid, TT, acc); break;
-        case SC_W_FP8: team_gather<SC_W_FP8, 8, true, STAGED>(T, ring32, win, rowoff, zb, tid, TT, acc); break;
-        default: team_gather<SC_W_UNIT, 8, true, STAGED>(T, ring32, win, rowoff, zb, tid, TT, acc); break;
+        case SC_W_F32: team_gather<SC_W_F32, 8, true, STAGED, NT>(T, ring32, win, rowoff, zb, tid, TT, acc); break;
+        case SC_W_BF16: team_gather<SC_W_BF16, 8, true, STAGED, NT>(T, ring32, win, rowoff, zb, tid, TT, acc); break;
+        case SC_W_FP8: team_gather<SC_W_FP8, 8, true, STAGED, NT>(T, ring32, win, rowoff, zb, tid, TT, acc); break;
+        default: team_gather<SC_W_UNIT, 8, true, STAGED, NT>(T, ring32, win, rowoff, zb, tid, TT, acc); break;
         }
     } else {
         switch (T.wprec) {
-        case SC_W_F32: team_gather<SC_W_F32, 8, false, STAGED>(T, ring32, win, rowoff, zb, tid, TT, acc); break;
-        case SC_W_BF16: team_gather<SC_W_BF16, 8, false, STAGED>(T, ring32, win, rowoff, zb, tid, TT, acc); break;
-        case SC_W_FP8: team_gather<SC_W_FP8, 8, false, STAGED>(T, ring32, win, rowoff, zb, tid, TT, acc); break;
-        default: team_gather<SC_W_UNIT, 8, false, STAGED>(T, ring32, win, rowoff, zb, tid, TT, acc); break;
+        case SC_W_F32: team_gather<SC_W_F32, 8, false, STAGED, NT>(T, ring32, win, rowoff, zb, tid, TT, acc); break;
+        case SC_W_BF16: team_gather<SC_W_BF16, 8, false, STAGED, NT>(T, ring32, win, rowoff, zb, tid, TT, acc); break;
+        case SC_W_FP8: team_gather<SC_W_FP8, 8, false, STAGED, NT>(T, ring32, win, rowoff, zb, tid, TT, acc); break;
+        default: team_gather<SC_W_UNIT, 8, false, STAGED, NT>(T, ring32, win, rowoff, zb, tid, TT, acc); break;
         }
     }
 }
@@ -694,19 +705,19 @@ __device__ __forceinline__ void team_dispatch(const TeamHead &T, gptr<uint32_t> 
 // neuron updates of all its teams side by side (one lane per team) instead of one
 // lane in each of up to 16 wavefronts, which would make the SIMDs issue the long
 // update 16 times.
-template <bool SINGLE, bool IDX16, typename Stage>
+template <bool SINGLE, bool IDX16, bool NT, typename Stage>
 __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *__restrict__ insts, int t, int count,
                                         const TeamHead &h, const ScTeamIn &pin, int row_single, int k,
                                         const Stage &stage)
 {
-    constexpr int NT = SC_BLOCK / SC_WAVE;
-    __shared__ double acc[NT][SC_MAX_IN];
-    __shared__ uint32_t rowoff[NT][SC_MAX_IN];
+    constexpr int NTEAM = SC_BLOCK / SC_WAVE;
+    __shared__ double acc[NTEAM][SC_MAX_IN];
+    __shared__ uint32_t rowoff[NTEAM][SC_MAX_IN];
     constexpr int PW = (SC_POP_WORDS + 1) / 2 * 2;
-    __shared__ alignas(8) uint32_t par[NT][PW];    // population parameters (8-byte aligned rows)
-    __shared__ float fac[NT][3][SC_MAX_IN];  // per projection: scale, W, excitatory flag (as bits)
-    __shared__ float sv[NT][10];  // u, g_exc, g_inh, dge, dgi; refr, device index, ring row, Wr, R (as bits)
-    __shared__ void *ptr[NT][2];  // instance arrays (state, ring) of each team's network
+    __shared__ alignas(8) uint32_t par[NTEAM][PW];    // population parameters (8-byte aligned rows)
+    __shared__ float fac[NTEAM][3][SC_MAX_IN];  // per projection: scale, W, excitatory flag (as bits)
+    __shared__ float sv[NTEAM][10];  // u, g_exc, g_inh, dge, dgi; refr, device index, ring row, Wr, R (as bits)
+    __shared__ void *ptr[NTEAM][2];  // instance arrays (state, ring) of each team's network
     const int wave = uniform(threadIdx.x / SC_WAVE), lane = threadIdx.x & (SC_WAVE - 1);
     const int team = wave / t, tw = wave - team * t;
     const bool active = team < count;
@@ -748,8 +759,8 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
         gptr<uint32_t> ring32 = gbl(reinterpret_cast<const uint32_t *>(I.ring));
         const int tid = tw * SC_WAVE + lane, TT = t * SC_WAVE;
         const uint32_t zb = (uint32_t)(I.Wr - 1) * SC_WAVE;
-        if (win) team_dispatch<IDX16, true>(h, ring32, win, rowoff[team], zb, tid, TT, acc[team]);
-        else team_dispatch<IDX16, false>(h, ring32, win, rowoff[team], zb, tid, TT, acc[team]);
+        if (win) team_dispatch<IDX16, true, NT>(h, ring32, win, rowoff[team], zb, tid, TT, acc[team]);
+        else team_dispatch<IDX16, false, NT>(h, ring32, win, rowoff[team], zb, tid, TT, acc[team]);
     }
     __syncthreads();
     float dge = 0.f, dgi = 0.f;
@@ -861,7 +872,7 @@ __device__ __forceinline__ const uint32_t *stage_windows(uint32_t sd, uint32_t *
 // The block layout is a kernel argument: block b belongs to the first class c with
 // b < cls_end[c]; classes 0..4 are team neurons of width 16, 8, 4, 2, 1
 // wavefronts, class 5 lane groups, class 6 stimulus groups.
-template <bool SINGLE, bool IDX16>
+template <bool SINGLE, bool IDX16, bool NT>
 __device__ __forceinline__ void step_body(const ScInstance &I0, const ScInstance *__restrict__ insts, const ScLayout &L,
                                           const ScLaneTask *__restrict__ lanes, const ScStimTask *__restrict__ stt,
                                           const ScTeamTask *__restrict__ teams, const ScStage *__restrict__ stages,
@@ -899,7 +910,7 @@ __device__ __forceinline__ void step_body(const ScInstance &I0, const ScInstance
         const TeamHead h = team_head(T);
         ScTeamIn pin{};
         if (lane < SC_MAX_IN) pin = T->in[lane];
-        do_team<SINGLE, IDX16>(I0, insts, t, count, h, pin, row_single, k, stage);
+        do_team<SINGLE, IDX16, NT>(I0, insts, t, count, h, pin, row_single, k, stage);
         return;
     }
     const int per_block = c == SC_CLASS_LANE ? SC_LANE_PER_BLOCK : SC_STIM_PER_BLOCK;
@@ -917,8 +928,8 @@ __device__ __forceinline__ void step_body(const ScInstance &I0, const ScInstance
         const ScInstance &I = inst_of<SINGLE>(I0, insts, t.inst);
         const int row_n = SINGLE ? row_single : (I.ctl->row + k) % I.R;
         const ScPop &P = I.pops[t.pop];
-        if (staged) lane_dispatch<IDX16, true>(t, I, P, row_n, stage, lane);
-        else lane_dispatch<IDX16, false>(t, I, P, row_n, stage, lane);
+        if (staged) lane_dispatch<IDX16, true, NT>(t, I, P, row_n, stage, lane);
+        else lane_dispatch<IDX16, false, NT>(t, I, P, row_n, stage, lane);
     } else {  // stimulus blocks (never staged)
         const ScStimTask t = stt[idx];
         const ScInstance &I = inst_of<SINGLE>(I0, insts, t.inst);
@@ -937,26 +948,27 @@ __device__ __forceinline__ void step_body(const ScInstance &I0, const ScInstance
 // One network: 64 VGPRs at most, so two 1024-thread blocks share a CU and one
 // block's serial phases (descriptor loads, barriers, the neuron update) overlap
 // the other's gathers.  IDX16: the network holds 16-bit column offsets (the
-// instantiation without them carries none of their code or registers).
-template <bool IDX16>
+// instantiation without them carries none of their code or registers).  NT: the
+// synapse streams are loaded nontemporal (networks whose streams exceed the L2).
+template <bool IDX16, bool NT>
 __global__ void __launch_bounds__(SC_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_step(ScInstance I0, ScLayout L, const ScLaneTask *__restrict__ lanes, const ScStimTask *__restrict__ stt,
        const ScTeamTask *__restrict__ teams, const ScStage *__restrict__ stages, int k)
 {
-    step_body<true, IDX16>(I0, nullptr, L, lanes, stt, teams, stages, k);
+    step_body<true, IDX16, NT>(I0, nullptr, L, lanes, stt, teams, stages, k);
 }
 
 // Many instances: every task names its instance in the instance table.  Held to
 // 64 VGPRs as well (a few spill to scratch): the doubled occupancy outweighs
 // the spill traffic.
-template <bool IDX16>
+template <bool IDX16, bool NT>
 __global__ void __launch_bounds__(SC_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) k_step_batch(const ScInstance *__restrict__ insts, ScLayout L,
                                                          const ScLaneTask *__restrict__ lanes,
                                                          const ScStimTask *__restrict__ stt,
                                                          const ScTeamTask *__restrict__ teams,
                                                          const ScStage *__restrict__ stages, int k)
 {
-    step_body<false, IDX16>(ScInstance{}, insts, L, lanes, stt, teams, stages, k);
+    step_body<false, IDX16, NT>(ScInstance{}, insts, L, lanes, stt, teams, stages, k);
 }
 
 // Per-population spike counts and synaptic events of K consecutive steps.  The
@@ -1221,17 +1233,24 @@ __global__ void k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, i
 }
 
 // --------------------------------------------------------------------- launchers
+template <bool NT>
+static void launch_step_nt(hipStream_t s, const ScInstance *single, const ScInstance *insts, const DevWork &w, int k)
+{
+    if (single && w.idx16)
+        k_step<true, NT><<<w.n_blocks, SC_BLOCK, 0, s>>>(*single, w.layout, w.lanes, w.stims, w.teams, w.stages, k);
+    else if (single)
+        k_step<false, NT><<<w.n_blocks, SC_BLOCK, 0, s>>>(*single, w.layout, w.lanes, w.stims, w.teams, w.stages, k);
+    else if (w.idx16)
+        k_step_batch<true, NT><<<w.n_blocks, SC_BLOCK, 0, s>>>(insts, w.layout, w.lanes, w.stims, w.teams, w.stages, k);
+    else
+        k_step_batch<false, NT><<<w.n_blocks, SC_BLOCK, 0, s>>>(insts, w.layout, w.lanes, w.stims, w.teams, w.stages, k);
+}
+
 void launch_step(hipStream_t s, const ScInstance *single, const ScInstance *insts, const DevWork &w, int k)
 {
     if (w.n_blocks <= 0) return;
-    if (single && w.idx16)
-        k_step<true><<<w.n_blocks, SC_BLOCK, 0, s>>>(*single, w.layout, w.lanes, w.stims, w.teams, w.stages, k);
-    else if (single)
-        k_step<false><<<w.n_blocks, SC_BLOCK, 0, s>>>(*single, w.layout, w.lanes, w.stims, w.teams, w.stages, k);
-    else if (w.idx16)
-        k_step_batch<true><<<w.n_blocks, SC_BLOCK, 0, s>>>(insts, w.layout, w.lanes, w.stims, w.teams, w.stages, k);
-    else
-        k_step_batch<false><<<w.n_blocks, SC_BLOCK, 0, s>>>(insts, w.layout, w.lanes, w.stims, w.teams, w.stages, k);
+    if (w.nontemporal) launch_step_nt<true>(s, single, insts, w, k);
+    else launch_step_nt<false>(s, single, insts, w, k);
 }
 
 void launch_stats(hipStream_t s, const ScInstance *single, const ScInstance *insts, int n_inst, int K,

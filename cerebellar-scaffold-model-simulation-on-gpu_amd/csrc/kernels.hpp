@@ -21,6 +21,7 @@ struct DevWork {
     const ScStimTask *stims = nullptr;
     const ScTeamTask *teams = nullptr;
     bool idx16 = false;  // some descriptor holds 16-bit column offsets
+    bool nontemporal = false;  // synapse streams loaded as nontemporal (networks beyond the L2)
     const ScStage *stages = nullptr;  // per block: spike windows staged in LDS (nullptr: none)
     int n_staged = 0;                 // blocks with staged windows
     int max_groups = 0;               // most 64-neuron groups of one instance (statistics grid)

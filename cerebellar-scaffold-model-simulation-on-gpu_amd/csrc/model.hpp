@@ -72,6 +72,7 @@ struct Options {
     int graph_steps = 25;
     int reset_g = 0;
     int stage_spikes = 1;
+    int nontemporal = -1;     // synapse streams as nontemporal loads: -1 when they exceed the L2, 0 never, 1 always
 };
 
 // Entry (column k, lane l) of a lane slice that starts at entry `off`: the
@@ -159,6 +160,7 @@ struct Plan {
     int64_t nnz = 0;
     int64_t algo_bytes = 0;
     int stage = 1;  // stage spike windows in LDS where the plan finds it pays
+    bool nontemporal = false;  // load the synapse streams nontemporal
     int64_t syn16 = 0;  // synapses stored as 16-bit column offsets
     int max_delay = 0;
     std::vector<int> out_pops;            // OUTPUT populations

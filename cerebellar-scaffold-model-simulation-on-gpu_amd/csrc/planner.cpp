@@ -665,6 +665,9 @@ int plan_build(scaffold_env *env, Plan &P)
     }
     P.algo_bytes = index_bytes + weight_bytes + (int64_t)ref * 32 + ref / 8;
     P.stage = opt.stage_spikes;
+    // the synapse streams of one step beyond 64 MB (twice the aggregate L2) are
+    // loaded nontemporal, so they do not evict descriptors and spike rows
+    P.nontemporal = opt.nontemporal < 0 ? P.algo_bytes > ((int64_t)64 << 20) : opt.nontemporal != 0;
     for (auto &g : P.groups) g.self_final = 1;
 
     // work list: team neurons (widest teams first), lane groups, stimulus groups

@@ -203,7 +203,11 @@ int scaffold_set_input_stim_ids(scaffold_env *env, int neuron_index,
  *                       (the reference keeps them, GPUScaffoldSimulation.cu:802-854)
  *   "stage_spikes"      1 (default): blocks whose gathers scatter over a bounded
  *                       range of presynaptic neurons first copy those spike words
- *                       into LDS; 0: every gather reads the spike ring in memory */
+ *                       into LDS; 0: every gather reads the spike ring in memory
+ *   "nontemporal"       -1 (default): the synapse streams (columns, multipliers)
+ *                       are loaded nontemporal when one step's exceed 64 MB, so
+ *                       they do not evict descriptors and spike rows from L2;
+ *                       0: never; 1: always.  Results are identical either way */
 int scaffold_set_option(scaffold_env *env, const char *key, int64_t value);
 int scaffold_get_option(scaffold_env *env, const char *key, int64_t *value);
 

@@ -183,8 +183,12 @@ def test_options(pkg):
     assert env.get_option("stage_spikes") == 1
     env.set_option("stage_spikes", 0)
     assert env.get_option("stage_spikes") == 0
+    assert env.get_option("nontemporal") == -1
+    env.set_option("nontemporal", 1)
+    assert env.get_option("nontemporal") == 1
     for k, v, code in [("no_such_option", 1, "ERR_INVALID_ARG"), ("weight_precision", 7, "ERR_INVALID_ARG"),
-                       ("index_bits", 8, "ERR_INVALID_ARG"), ("graph_steps", 0, "ERR_INVALID_ARG")]:
+                       ("index_bits", 8, "ERR_INVALID_ARG"), ("graph_steps", 0, "ERR_INVALID_ARG"),
+                       ("nontemporal", 2, "ERR_INVALID_ARG")]:
         with pytest.raises(pkg.ScaffoldError) as e:
             env.set_option(k, v)
         assert e.value.code == getattr(pkg, code), k

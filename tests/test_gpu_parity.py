@@ -93,6 +93,21 @@ def test_config1_bit_exact(pkg, orc, stage):
     env.finalize()
 
 
+@pytest.mark.parametrize("index_bits", [32, 16])
+def test_nontemporal_streams_bit_exact(pkg, orc, index_bits):
+    """The step kernels that load the synapse streams nontemporal (chosen on its
+    own above 64 MB of streams per step) give the same raster and state."""
+    env = new_env(pkg, 0)
+    env.set_option("nontemporal", 1)
+    env.set_option("index_bits", index_bits)
+    env.initialize(50, 25)
+    ref = orc.OracleSim.from_env(env)
+    ref.initialize()
+    run_and_compare(env, ref, 0, 150, check_counts=True)
+    assert_same_state(env, ref, f"nontemporal, {index_bits}-bit columns")
+    env.finalize()
+
+
 FP64_STEPS = 1000
 V_CHECKPOINTS = (24, 499, 999)
 _FP64_REFERENCE = {}
