@@ -959,8 +959,8 @@ k_step(ScInstance I0, ScLayout L, const ScLaneTask *__restrict__ lanes, const Sc
 }
 
 // Many instances: every task names its instance in the instance table.  Held to
-// 64 VGPRs as well (a few spill to scratch): the doubled occupancy outweighs
-// the spill traffic.
+// 64 VGPRs as well (some registers spill to scratch, more in the 16-bit
+// instantiation): the doubled occupancy outweighs the spill traffic.
 template <bool IDX16, bool NT>
 __global__ void __launch_bounds__(SC_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) k_step_batch(const ScInstance *__restrict__ insts, ScLayout L,
                                                          const ScLaneTask *__restrict__ lanes,
