@@ -185,16 +185,17 @@ struct ScTeamTask {        // one neuron of a team-mode population
 
 // Block layout of the step kernel (a kernel argument).  Blocks are grouped in
 // classes: team neurons of width 16, 8, 4, 2, 1 wavefronts (classes 0..4, with
-// 16/width neurons per block), lane groups (class 5) and stimulus groups (class 6),
-// 16 wavefronts per block.
-#define SC_CLASSES    7
+// 16/width neurons per block), lane groups (class 5), stimulus groups (class 6)
+// and the window counts of the plasticity rules (class 7: one 64-neuron group of
+// a rule's presynaptic population per wavefront), 16 wavefronts per block.
+#define SC_CLASSES    8
 #define SC_CLASS_LANE 5
 #define SC_CLASS_STIM 6
+#define SC_CLASS_HIST 7
 struct ScLayout {
     int32_t cls_end[SC_CLASSES];    // cumulative block count at the end of each class
     int32_t cls_ntask[SC_CLASSES];  // tasks of each class
     int32_t cls_task0[SC_CLASSES];  // first descriptor index of each class
-    int32_t pad_;
 };
 
 // State of one device neuron: one 16-byte record, read and written by one load
@@ -206,10 +207,12 @@ struct alignas(16) ScState {
 
 // One plasticity rule over the synapses of its target projection
 // (Hebb_plasticity / Teacher_plasticity, SynapticPlasticity.cu:38-121), applied
-// after every exchange step in two launches.  k_plastic_hist writes, per neuron of
-// the presynaptic population, one byte `hist`: bits 0..6 count its spikes at
-// n - delay - s, s < window (window <= 127), bit 7 is its spike at n - delay.
-// k_plastic then updates the synapses (source order, grouped by post row) in
+// with every exchange step.  Blocks of the step kernel itself (class
+// SC_CLASS_HIST) write, per neuron of the presynaptic population, one byte
+// `hist`: bits 0..6 count its spikes at n - delay - s, s < window (window <=
+// 127), bit 7 is its spike at n - delay -- rows that are complete before step n
+// starts, as every delay is at least one step.  k_plastic then updates the
+// synapses (source order, grouped by post row) in
 // chunks of at most SC_PLASTIC_CHUNK entries of one row, one wavefront each; a
 // chunk carries its row's trigger: the postsynaptic device bit (Hebb) or the
 // teacher's device bit (Teacher; -1: none).  Per synapse: the presynaptic

@@ -89,10 +89,10 @@ static int check_device()
 static int wbytes_of(int wprec) { return wprec == SC_W_F32 ? 4 : wprec == SC_W_BF16 ? 2 : wprec == SC_W_FP8 ? 1 : 0; }
 
 // Block layout of the step kernel: team neurons by team width (widest first, so
-// the longest rows start earliest), then lane groups, then stimulus groups, 16
-// wavefronts per block.  `team_t` lists the team width of every team descriptor,
+// the longest rows start earliest), then lane groups, then stimulus groups, then
+// the plasticity rules' window-count groups, 16 wavefronts per block.  `team_t` lists the team width of every team descriptor,
 // widest first.
-static ScLayout make_layout(const std::vector<int> &team_t, int n_lanes, int n_stims, int *n_blocks)
+static ScLayout make_layout(const std::vector<int> &team_t, int n_lanes, int n_stims, int n_hist, int *n_blocks)
 {
     ScLayout L{};
     int blocks = 0;
@@ -113,6 +113,9 @@ static ScLayout make_layout(const std::vector<int> &team_t, int n_lanes, int n_s
     L.cls_ntask[SC_CLASS_STIM] = n_stims;
     blocks += (n_stims + SC_STIM_PER_BLOCK - 1) / SC_STIM_PER_BLOCK;
     L.cls_end[SC_CLASS_STIM] = blocks;
+    L.cls_ntask[SC_CLASS_HIST] = n_hist;
+    blocks += (n_hist + SC_BLOCK / SC_WAVE - 1) / (SC_BLOCK / SC_WAVE);
+    L.cls_end[SC_CLASS_HIST] = blocks;
     *n_blocks = blocks;
     return L;
 }
@@ -198,7 +201,7 @@ static std::vector<ScStage> plan_stages(const ScLayout &L, int n_blocks, std::ve
 static int upload_work(std::vector<void *> &allocs, int64_t *bytes, DevWork &w, const std::vector<int> &team_t,
                        std::vector<ScLaneTask> lanes, const std::vector<ScStimTask> &stims,
                        std::vector<ScTeamTask> teams, const std::vector<TaskWin> &lane_win,
-                       const std::vector<TaskWin> &team_win, bool stage)
+                       const std::vector<TaskWin> &team_win, bool stage, int n_hist)
 {
     for (size_t i = 1; i < team_t.size(); ++i)
         if (team_t[i] > team_t[i - 1]) return fail(SCAFFOLD_ERR_STATE, "team descriptors not ordered by width");
@@ -208,7 +211,7 @@ static int upload_work(std::vector<void *> &allocs, int64_t *bytes, DevWork &w, 
     ScLaneTask *l;
     ScStimTask *s;
     ScTeamTask *t;
-    w.layout = make_layout(team_t, (int)lanes.size(), (int)stims.size(), &w.n_blocks);
+    w.layout = make_layout(team_t, (int)lanes.size(), (int)stims.size(), n_hist, &w.n_blocks);
     int n_staged = 0;
     std::vector<ScStage> st;
     if (stage) st = plan_stages(w.layout, w.n_blocks, lanes, lane_win, teams, team_win, &n_staged);
@@ -423,8 +426,10 @@ int engine_create(Engine **out, Plan &P)
             e->h_stims.push_back(s);
         }
     }
+    int n_hist = 0;  // window-count groups of the plasticity rules
+    for (const PlasticPlan &X : P.plastic) n_hist += (P.pops[P.projs[X.rule.target].pre_pop].num + SC_WAVE - 1) / SC_WAVE;
     if ((rc = upload_work(A, B, e->w, e->h_team_t, e->h_lanes, e->h_stims, e->h_teams, e->h_lane_win, e->h_team_win,
-                          P.stage != 0)))
+                          P.stage != 0, n_hist)))
         return bail(rc);
     // plasticity rules: per-synapse arrays; the multipliers they update are the
     // f32 arrays of the target's lane slices or merged rows
@@ -461,6 +466,11 @@ int engine_create(Engine **out, Plan &P)
         for (int c = 0; c < 4; ++c) d.coef[c] = X.rule.coef[c];
         e->w.plastic.push_back(d);
         e->plastic_views.push_back(Engine::PlasticView{X.rule.target, d.val, X.n_val, X.vidx});
+    }
+    if (!e->w.plastic.empty()) {
+        ScPlastic *dp;
+        if ((rc = dupload(A, B, &dp, e->w.plastic))) return bail(rc);
+        e->w.d_plastic = dp;
     }
     e->w.max_groups = P.n_groups;
     e->w.nontemporal = P.nontemporal;
@@ -829,7 +839,7 @@ int batch_build(Batch **out, std::vector<Engine *> &engines, const std::vector<c
     }
     int rc;
     if ((rc = dupload(b->allocs, nullptr, &b->di, insts))) return bail(rc);
-    if ((rc = upload_work(b->allocs, nullptr, b->w, tw, lanes, stims, tt, lane_win, team_win, stage != 0)))
+    if ((rc = upload_work(b->allocs, nullptr, b->w, tw, lanes, stims, tt, lane_win, team_win, stage != 0, 0)))
         return bail(rc);
     for (const ScInstance &x : insts) b->w.max_groups = std::max(b->w.max_groups, x.n_groups);
     b->w.nontemporal = nt_all || algo > ((int64_t)64 << 20);

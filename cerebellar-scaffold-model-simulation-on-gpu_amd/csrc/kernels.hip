@@ -836,6 +836,35 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
     }
 }
 
+// Window counts of the plasticity rules for step n: wavefront `item` of the
+// class takes one 64-neuron group of one rule's presynaptic population, one lane
+// per neuron, and writes its byte of the rule's `hist` (spikes at n - delay - s,
+// s < window, and at n - delay).  Those rows are complete before step n starts
+// (every delay is at least one step).  The group's word of each window row is
+// one wave-uniform load; the loads go out 16 at a time.
+__device__ __forceinline__ void plastic_hist(const ScInstance &I, const ScPlastic *__restrict__ rules, int item,
+                                             int row_n, int lane)
+{
+    const ScPlastic *P = rules;
+    while (item >= P->pre_groups) {  // the rule and group of this item
+        item -= P->pre_groups;
+        ++P;
+    }
+    const int R = I.R, window = P->window;
+    const int pre_t = row_back(row_n, P->delay, R);
+    const uint64_t *col = I.ring + (P->pre_group0 + item);
+    int cnt = 0, s0 = 0;
+    for (int b = 0; b < window; b += 16) {
+        uint64_t w[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) w[u] = b + u < window ? col[(int64_t)row_back(pre_t, b + u, R) * I.Wr] : 0ull;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) cnt += (int)((w[u] >> lane) & 1ull);
+        if (b == 0) s0 = (int)(w[0] >> lane) & 1;
+    }
+    P->hist[(int64_t)item * SC_WAVE + lane] = (uint8_t)(cnt | (s0 << 7));
+}
+
 // Issues the copies of a staged block's windows into LDS (see ScStage); sd holds
 // dword `lane` of the block's window table.  Returns the LDS windows, or nullptr
 // for a block without windows.
@@ -876,7 +905,7 @@ template <bool SINGLE, bool IDX16, bool NT>
 __device__ __forceinline__ void step_body(const ScInstance &I0, const ScInstance *__restrict__ insts, const ScLayout &L,
                                           const ScLaneTask *__restrict__ lanes, const ScStimTask *__restrict__ stt,
                                           const ScTeamTask *__restrict__ teams, const ScStage *__restrict__ stages,
-                                          int k)
+                                          const ScPlastic *__restrict__ plastic, int k)
 {
     // the step counter load is issued first; it overlaps the descriptor loads
     int row_single = 0;
@@ -911,6 +940,11 @@ __device__ __forceinline__ void step_body(const ScInstance &I0, const ScInstance
         ScTeamIn pin{};
         if (lane < SC_MAX_IN) pin = T->in[lane];
         do_team<SINGLE, IDX16, NT>(I0, insts, t, count, h, pin, row_single, k, stage);
+        return;
+    }
+    if (c == SC_CLASS_HIST) {  // one instance (the batch holds no plasticity rules)
+        const int item = uniform(bi * (SC_BLOCK / SC_WAVE) + wave);
+        if (item < L.cls_ntask[c]) plastic_hist(I0, plastic, item, row_single, lane);
         return;
     }
     const int per_block = c == SC_CLASS_LANE ? SC_LANE_PER_BLOCK : SC_STIM_PER_BLOCK;
@@ -953,9 +987,10 @@ __device__ __forceinline__ void step_body(const ScInstance &I0, const ScInstance
 template <bool IDX16, bool NT>
 __global__ void __launch_bounds__(SC_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_step(ScInstance I0, ScLayout L, const ScLaneTask *__restrict__ lanes, const ScStimTask *__restrict__ stt,
-       const ScTeamTask *__restrict__ teams, const ScStage *__restrict__ stages, int k)
+       const ScTeamTask *__restrict__ teams, const ScStage *__restrict__ stages,
+       const ScPlastic *__restrict__ plastic, int k)
 {
-    step_body<true, IDX16, NT>(I0, nullptr, L, lanes, stt, teams, stages, k);
+    step_body<true, IDX16, NT>(I0, nullptr, L, lanes, stt, teams, stages, plastic, k);
 }
 
 // Many instances: every task names its instance in the instance table.  Held to
@@ -968,7 +1003,7 @@ __global__ void __launch_bounds__(SC_BLOCK) __attribute__((amdgpu_waves_per_eu(8
                                                          const ScTeamTask *__restrict__ teams,
                                                          const ScStage *__restrict__ stages, int k)
 {
-    step_body<false, IDX16, NT>(ScInstance{}, insts, L, lanes, stt, teams, stages, k);
+    step_body<false, IDX16, NT>(ScInstance{}, insts, L, lanes, stt, teams, stages, nullptr, k);
 }
 
 // Per-population spike counts and synaptic events of K consecutive steps.  The
@@ -1109,30 +1144,6 @@ __global__ void k_zero_g(ScState *st, int n)
 // Fired-neuron list of one ring row: the spike words are the ballots of spike
 // detection; one block popcounts them, prefix-sums the counts and writes the
 // reference global ids in ascending order.
-// Window counts of one plasticity rule after exchange step n (launch offset k):
-// one wavefront per 64-neuron group of the presynaptic population, one lane per
-// neuron.  The group's word of each window row is one wave-uniform load; the
-// loads go out 16 at a time.
-__global__ void __launch_bounds__(256) k_plastic_hist(ScInstance I, ScPlastic P, int k)
-{
-    const int g = uniform((int)blockIdx.x * 4 + (int)(threadIdx.x / SC_WAVE));
-    const int lane = threadIdx.x & (SC_WAVE - 1);
-    if (g >= P.pre_groups) return;
-    const int R = I.R;
-    const int pre_t = row_back((I.ctl->row + k) % R, P.delay, R);
-    const uint64_t *col = I.ring + (P.pre_group0 + g);
-    int cnt = 0, s0 = 0;
-    for (int b = 0; b < P.window; b += 16) {
-        uint64_t w[16];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) w[u] = b + u < P.window ? col[(int64_t)row_back(pre_t, b + u, R) * I.Wr] : 0ull;
-#pragma unroll
-        for (int u = 0; u < 16; ++u) cnt += (int)((w[u] >> lane) & 1ull);
-        if (b == 0) s0 = (int)(w[0] >> lane) & 1;
-    }
-    P.hist[(int64_t)g * SC_WAVE + lane] = (uint8_t)(cnt | (s0 << 7));
-}
-
 // One plasticity rule after exchange step n (launch offset k): Hebb_plasticity /
 // Teacher_plasticity (SynapticPlasticity.cu:38-121), one wavefront per chunk of
 // synapses of one post row, 4 synapses per lane (their loads in flight
@@ -1237,9 +1248,11 @@ template <bool NT>
 static void launch_step_nt(hipStream_t s, const ScInstance *single, const ScInstance *insts, const DevWork &w, int k)
 {
     if (single && w.idx16)
-        k_step<true, NT><<<w.n_blocks, SC_BLOCK, 0, s>>>(*single, w.layout, w.lanes, w.stims, w.teams, w.stages, k);
+        k_step<true, NT><<<w.n_blocks, SC_BLOCK, 0, s>>>(*single, w.layout, w.lanes, w.stims, w.teams, w.stages,
+                                                         w.d_plastic, k);
     else if (single)
-        k_step<false, NT><<<w.n_blocks, SC_BLOCK, 0, s>>>(*single, w.layout, w.lanes, w.stims, w.teams, w.stages, k);
+        k_step<false, NT><<<w.n_blocks, SC_BLOCK, 0, s>>>(*single, w.layout, w.lanes, w.stims, w.teams, w.stages,
+                                                          w.d_plastic, k);
     else if (w.idx16)
         k_step_batch<true, NT><<<w.n_blocks, SC_BLOCK, 0, s>>>(insts, w.layout, w.lanes, w.stims, w.teams, w.stages, k);
     else
@@ -1297,7 +1310,6 @@ void launch_plastic(hipStream_t s, const ScInstance &I, const DevWork &w, int k)
 {
     for (const ScPlastic &P : w.plastic) {
         if (P.n_chunks <= 0) continue;
-        if (P.pre_groups > 0) k_plastic_hist<<<(unsigned)((P.pre_groups + 3) / 4), 256, 0, s>>>(I, P, k);
         k_plastic<<<(unsigned)((P.n_chunks + 3) / 4), 256, 0, s>>>(I, P, k);
     }
 }

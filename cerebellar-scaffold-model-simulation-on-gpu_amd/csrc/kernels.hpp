@@ -26,6 +26,7 @@ struct DevWork {
     int n_staged = 0;                 // blocks with staged windows
     int max_groups = 0;               // most 64-neuron groups of one instance (statistics grid)
     std::vector<ScPlastic> plastic;   // plasticity rules, applied in order after each step
+    const ScPlastic *d_plastic = nullptr;  // the rules on the device (window-count blocks of the step kernel)
 };
 
 // All launchers are asynchronous on stream s.  `single` != nullptr selects the
