@@ -12,7 +12,10 @@
 //         the LIF update and sets the neuron's bit in the spike word with an
 //         integer atomicOr;
 //   STIM  a wavefront per 64 INPUT neurons: counter-based Philox4x32-10 draws and
-//         the stimulus rule, spike word = ballot.
+//         the stimulus rule, spike word = ballot;
+//   HIST  (with plasticity rules) a wavefront per 64 presynaptic neurons of a
+//         rule: spike counts over the rule's window, for k_plastic, which runs
+//         after the step and updates the rule's multipliers.
 // A step reads ring rows of earlier steps only (every delay >= 1) and writes the
 // row of the current step, so blocks never wait on each other; the kernel
 // boundary orders consecutive steps.  k_stats popcounts the spike words of a run
