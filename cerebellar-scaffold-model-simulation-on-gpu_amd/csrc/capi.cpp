@@ -755,9 +755,9 @@ int scaffold_get_projection_csr(scaffold_env *env, int index, int64_t *row_ptr, 
     if (val) {
         const int64_t nnz = (int64_t)p.col.size();
         if (env->initialized) {
-            rc = env->eng ? engine_read_plastic(env->eng, index, val) : 1;
-            if (rc > 1) return rc;
-            if (rc == 1)  // not plastic: the multipliers as planned
+            rc = env->eng ? engine_read_plastic(env->eng, index, val) : -1;
+            if (rc > 0) return rc;
+            if (rc < 0)  // not plastic: the multipliers as planned
                 for (int64_t e = 0; e < nnz; ++e) val[e] = plan_multiplier(env, index, e);
         } else {
             for (int64_t e = 0; e < nnz; ++e) val[e] = p.val.empty() ? 1.f : p.val[e];

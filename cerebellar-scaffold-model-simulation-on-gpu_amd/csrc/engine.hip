@@ -501,7 +501,7 @@ int engine_read_plastic(Engine *e, int proj, float *out)
         for (size_t i = 0; i < v.vidx.size(); ++i) out[i] = buf[v.vidx[i]];
         return SCAFFOLD_OK;
     }
-    return 1;
+    return -1;
 }
 
 int64_t engine_device_bytes(Engine *e) { return e ? e->bytes : 0; }

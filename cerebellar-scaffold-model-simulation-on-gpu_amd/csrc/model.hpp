@@ -190,7 +190,7 @@ int engine_time(Engine *e, int n_steps, int use_graph, int graph_steps,
 int64_t engine_device_bytes(Engine *e);
 void engine_blocks(Engine *e, int *n_blocks, int *n_staged);
 int engine_copy_probe(int64_t bytes, int iters, double *ms);
-// Current multipliers of a plastic projection in source order; returns 1 if no
+// Current multipliers of a plastic projection in source order; returns -1 if no
 // plasticity rule targets the projection.
 int engine_read_plastic(Engine *e, int proj, float *out);
 
