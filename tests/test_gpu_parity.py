@@ -539,8 +539,9 @@ def test_autotune_then_bit_exact(pkg, orc):
     env.finalize()
 
 
-@pytest.mark.parametrize("index_bits,precision", [(32, 0), (16, 1)])
-def test_plasticity_bit_exact(pkg, orc, index_bits, precision):
+@pytest.mark.parametrize("index_bits,precision,extra", [(32, 0, ()), (16, 1, ()),
+                                                         (32, 0, (("nontemporal", 1), ("graph", 0)))])
+def test_plasticity_bit_exact(pkg, orc, index_bits, precision, extra):
     """Synaptic plasticity on the GPU against the oracle, 200 steps: the
     reference's parallel-fibre-to-Purkinje Teacher rule (A1_J 0, A2_JI -0.01,
     window 50; SynapticPlasticity.cu:10-16) with basket-to-Purkinje as the
@@ -548,7 +549,8 @@ def test_plasticity_bit_exact(pkg, orc, index_bits, precision):
     Golgi-to-granule (lane-mode slices).  Rasters every step, the state and every
     updated multiplier must match bit for bit; 16-bit columns move the team rows'
     multipliers into chunk order, BF16 elsewhere leaves the plastic post
-    populations in f32."""
+    populations in f32; the third case uses the nontemporal step kernels and no
+    graph replays."""
     env = new_env(pkg, 0, plasticities=2)
     idx = {env.projection(i).type: i for i in range(env.info().n_projections)}
     pf_pc, bc_pc, goc_grc = idx[0], idx[11], idx[15]
@@ -560,6 +562,8 @@ def test_plasticity_bit_exact(pkg, orc, index_bits, precision):
         env.set_plasticity(**r)
     env.set_option("index_bits", index_bits)
     env.set_option("weight_precision", pkg.WEIGHTS_BF16 if precision else pkg.WEIGHTS_F32)
+    for k, v in extra:
+        env.set_option(k, v)
     env.initialize(50, 25)
     assert env.projection(pf_pc).lane_mode == 0 and env.projection(goc_grc).lane_mode == 1
     ref = orc.OracleSim.from_env(env)
