@@ -45,7 +45,9 @@ class StimFunc(ctypes.Structure):
                 ("periodic_hz", ctypes.c_float), ("periodic_start", ctypes.c_float),
                 ("periodic_on", ctypes.c_double), ("periodic_off", ctypes.c_double),
                 ("periodic_off_inclusive", ctypes.c_int), ("poisson_hz_from_freq", ctypes.c_int),
-                ("seed", ctypes.c_uint64)]
+                ("seed", ctypes.c_uint64),
+                ("sine_max_hz", ctypes.c_float), ("sine_cycle_s", ctypes.c_float), ("sine_shift", ctypes.c_float),
+                ("sine_on", ctypes.c_double), ("sine_off", ctypes.c_double)]
 
 
 class SynthConfig(ctypes.Structure):

@@ -69,6 +69,11 @@ static ScStim make_dev_stim(const Stimulus &s)
     d.periodic_hz = s.f.periodic_hz;
     d.periodic_start = s.f.periodic_start;
     d.periodic_incl = s.f.periodic_off_inclusive;
+    d.sine_max = s.f.sine_max_hz;
+    d.sine_on = s.f.sine_on;
+    d.sine_off = s.f.sine_off;
+    d.sine_cycle = s.f.sine_cycle_s;
+    d.sine_shift = s.f.sine_shift;
     return d;
 }
 
@@ -113,8 +118,17 @@ int scaffold_stim_func_reference(int func_id, uint64_t seed, scaffold_stim_func 
         f.periodic_on = 300.0;
         f.periodic_off = 351.0;
         break;
+    case 2:  // puff_stim: (0.0 <= time) && PoissonProcess(r, t, SinusoidalOscillation(3, 1.5, 2, 0, t), 0)
+        f.poisson_on = 0.0;
+        f.poisson_off = 0.0;
+        f.sine_max_hz = 3.0f;
+        f.sine_cycle_s = 2.0f;
+        f.sine_shift = 0.0f;
+        f.sine_on = 0.0;
+        f.sine_off = inf;
+        break;
     default:
-        return fail(SCAFFOLD_ERR_INVALID_ARG, "reference stimulus function %d does not exist (0, 1)", func_id);
+        return fail(SCAFFOLD_ERR_INVALID_ARG, "reference stimulus function %d does not exist (0, 1, 2)", func_id);
     }
     *out = f;
     return SCAFFOLD_OK;
@@ -332,6 +346,10 @@ static int add_stim(scaffold_env *env, int neuron_index, std::vector<uint32_t> &
         return fail(SCAFFOLD_ERR_INVALID_ARG, "stimuli drive INPUT populations only");
     if ((int)env->stims.size() >= env->n_inputs)
         return fail(SCAFFOLD_ERR_CAPACITY, "more stimuli than num_of_inputs=%d", env->n_inputs);
+    if (func->sine_max_hz != 0.f &&
+        !(std::isfinite(func->sine_max_hz) && std::isfinite(func->sine_cycle_s) && func->sine_cycle_s != 0.f &&
+          std::isfinite(func->sine_shift)))
+        return fail(SCAFFOLD_ERR_INVALID_ARG, "sinusoidal stimulus needs a finite rate, phase and non-zero period");
     const int num = env->pops[neuron_index].num;
     for (uint32_t id : ids)
         if ((int64_t)id >= num) return fail(SCAFFOLD_ERR_FORMAT, "stimulus id %u out of range [0,%d)", id, num);

@@ -89,7 +89,9 @@ struct ScStim {
     double  periodic_on, periodic_off;
     float   periodic_hz, periodic_start;
     int32_t periodic_incl;
-    int32_t pad_;
+    float   sine_max;            // sinusoidal Poisson peak rate, Hz (0 = off)
+    double  sine_on, sine_off;
+    float   sine_cycle, sine_shift;   // period (s), phase (rad)
 };
 
 struct ScCtl {

@@ -601,6 +601,40 @@ __device__ __forceinline__ void team_gather(const TeamHead &T, gptr<uint32_t> ri
 // Whether one stimulus entry fires in exchange step n (draw index m):
 // InputStimulation (UserInputFunctions.cu:94-118) with the rule of ScStim.  The
 // three Philox blocks of the step run interleaved, round by round.
+// Cosine of a float argument, rounded once to float.  Cody-Waite reduction by
+// pi/2 = HI + LO (HI has 33 significant bits, so k*HI and x - k*HI are exact for
+// |k| < 2^20; pi/2 - HI - LO < 4e-27), then the Taylor series of cos and sin on
+// |r| <= pi/4 (truncation below 1e-17), in double with basic operations only.
+// The evaluation order is fixed, so every IEEE machine gives the same bits.
+__device__ __forceinline__ float stim_cosf(float x)
+{
+    const double HI = 0x1.921fb544p+0, LO = 0x1.0b4611a626331p-34, TWO_OVER_PI = 0x1.45f306dc9c883p-1;
+    const double xd = (double)x;
+    const double k = rint(xd * TWO_OVER_PI);
+    const double r = (xd - k * HI) - k * LO;
+    const double z = r * r;
+    const double c =
+        1.0 + z * (-1.0 / 2 + z * (1.0 / 24 + z * (-1.0 / 720 + z * (1.0 / 40320 + z * (-1.0 / 3628800 +
+              z * (1.0 / 479001600 + z * (-1.0 / 87178291200.0 + z * (1.0 / 20922789888000.0))))))));
+    const double sn =
+        r + r * z * (-1.0 / 6 + z * (1.0 / 120 + z * (-1.0 / 5040 + z * (1.0 / 362880 + z * (-1.0 / 39916800 +
+              z * (1.0 / 6227020800.0 + z * (-1.0 / 1307674368000.0 + z * (1.0 / 355687428096000.0))))))));
+    const int q = (int)((long long)k & 3);
+    const double v = q == 0 ? c : q == 1 ? -sn : q == 2 ? -c : sn;
+    return (float)v;
+}
+
+// Poisson threshold of the sinusoidal component at sub-step time t: the rate of
+// SinusoidalOscillation (UserInputFunctions.cu:12-14), max/2*(1 - cosf(phase)),
+// in float, with the phase 2*pi*(t/sec)/osci + shift evaluated in double and
+// rounded to float for cosf; then PoissonProcess's freq/sec*DT in double.
+__device__ __forceinline__ double stim_sine_thr(float t, float max, float cycle, float shift)
+{
+    const double phase = 0x1.921fb54442d18p+2 * ((double)t / kSec) / (double)cycle + (double)shift;
+    const float rate = max / 2.f * (1.f - stim_cosf((float)phase));
+    return (double)rate / kSec * kDT;
+}
+
 __device__ __forceinline__ int stim_fires(const ScStim &S, int entry, int64_t n, int64_t m)
 {
     const uint64_t blk = (uint64_t)(3 * m);
@@ -639,6 +673,9 @@ __device__ __forceinline__ int stim_fires(const ScStim &S, int entry, int64_t n,
             const int b = (int)((double)fp / kSec);
             if (a != b) f = 1;
         }
+        if (S.sine_max != 0.f && S.sine_on <= td && td < S.sine_off &&
+            rk < stim_sine_thr(t, S.sine_max, S.sine_cycle, S.sine_shift))
+            f = 1;
         spk |= f;
     }
     return spk;

@@ -60,13 +60,20 @@ const char *scaffold_version(void);
 
 /* Stimulus function expressed as data.  It replaces the device function-pointer
  * table d_pInputFunctions[] (UserInputFunctions.cu:83) and its entries
- * background_noise / tone_stim (UserInputFunctions.cu:19-29), built from
- * PoissonProcess / PeriodicFiring (UserInputFunctions.cu:4-10).
+ * background_noise / tone_stim / puff_stim (UserInputFunctions.cu:19-39), built
+ * from PoissonProcess / PeriodicFiring / SinusoidalOscillation
+ * (UserInputFunctions.cu:4-14).
  *
  * For a list entry and sub-step time t (ms, float), with uniform r:
  *   fire = (poisson_on  <= t < poisson_off  && r < poisson_hz/1000*0.1)
  *       || (burst_on    <= t < burst_off    && r < burst_hz/1000*0.1)
  *       || (periodic_on <= t < periodic_off(+inclusive) && PeriodicFiring(t, periodic_hz, periodic_start))
+ *       || (sine_on     <= t < sine_off     && r < sine(t)/1000*0.1)
+ * with the rate of SinusoidalOscillation (UserInputFunctions.cu:12-14), in float:
+ *   sine(t) = sine_max_hz/2 * (1 - cos((float)(2*pi*(t/1000)/sine_cycle_s + sine_shift)))
+ * (the reference's `mean` argument does not enter its formula).  cos is this
+ * library's own float cosine (Cody-Waite reduction by pi/2 and Taylor series in
+ * double, rounded once to float), not CUDA's cosf.
  * A rate of 0 disables a component.  Windows are compared in double precision. */
 typedef struct {
     float  poisson_hz;
@@ -84,10 +91,18 @@ typedef struct {
     /* Philox4x32-10 key of this stimulus: curand_init(seed, subsequence=i, 0)
      * per list entry i (GPUScaffoldSimulation.cu:329, Simulation_header.cu:29-34). */
     uint64_t seed;
+    /* Sinusoidally modulated Poisson component (0 = off); sine_cycle_s is the
+     * period in seconds (finite, non-zero when the component is on), sine_shift
+     * the phase in radians. */
+    float  sine_max_hz;
+    float  sine_cycle_s;
+    float  sine_shift;
+    double sine_on, sine_off;
 } scaffold_stim_func;
 
-/* Fill *out with the reference's table entry func_id:
- * 0 = background_noise (UserInputFunctions.cu:19-23), 1 = tone_stim (:25-29). */
+/* Fill *out with the reference's stimulus function func_id:
+ * 0 = background_noise (UserInputFunctions.cu:19-23), 1 = tone_stim (:25-29),
+ * 2 = puff_stim (:33-39; in the reference's alternative table, :84). */
 int scaffold_stim_func_reference(int func_id, uint64_t seed, scaffold_stim_func *out);
 
 typedef struct scaffold_env   scaffold_env;

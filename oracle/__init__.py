@@ -37,6 +37,8 @@ class OrcStim(ctypes.Structure):
                 ("periodic_hz", ctypes.c_float), ("periodic_start", ctypes.c_float),
                 ("periodic_on", ctypes.c_double), ("periodic_off", ctypes.c_double),
                 ("periodic_off_inclusive", ctypes.c_int), ("poisson_hz_from_freq", ctypes.c_int),
+                ("sine_max_hz", ctypes.c_float), ("sine_cycle_s", ctypes.c_float), ("sine_shift", ctypes.c_float),
+                ("sine_on", ctypes.c_double), ("sine_off", ctypes.c_double),
                 ("seed", ctypes.c_uint64), ("n_entries", ctypes.c_int),
                 ("fanout_ptr", ctypes.POINTER(ctypes.c_int64)), ("targets", ctypes.POINTER(ctypes.c_uint32))]
 
@@ -58,6 +60,8 @@ def lib():
         L.orc_philox_u32.restype = u32
         L.orc_uniform.argtypes = [u32]
         L.orc_uniform.restype = ctypes.c_float
+        L.orc_cosf.argtypes = [ctypes.c_float]
+        L.orc_cosf.restype = ctypes.c_float
         L.orc_create.argtypes = [ctypes.c_int, P(OrcPop), ctypes.c_int, P(OrcProj), ctypes.c_int, P(OrcStim),
                                  ctypes.c_int, u64, ctypes.c_int, ctypes.c_int]
         L.orc_create.restype = ctypes.c_void_p
@@ -120,6 +124,10 @@ def uniform(v):
     return lib().orc_uniform(v)
 
 
+def cosf(x):
+    return lib().orc_cosf(x)
+
+
 def make_pop(num, kind, Cm=0., El=0., dt_ref=0., Ie=0., Vr=0., Vth=0., gL=0., channels=()):
     p = OrcPop()
     p.num, p.kind = num, kind
@@ -133,13 +141,16 @@ def make_pop(num, kind, Cm=0., El=0., dt_ref=0., Ie=0., Vr=0., Vth=0., gL=0., ch
 
 def make_stim(pop, targets, fanout_ptr=None, seed=0, poisson_hz=0., poisson_on=0., poisson_off=float("inf"),
               burst_hz=0., burst_on=0., burst_off=0., periodic_hz=0., periodic_start=0., periodic_on=0.,
-              periodic_off=0., periodic_off_inclusive=0, poisson_hz_from_freq=0):
+              periodic_off=0., periodic_off_inclusive=0, poisson_hz_from_freq=0, sine_max_hz=0., sine_cycle_s=0.,
+              sine_shift=0., sine_on=0., sine_off=0.):
     s = OrcStim()
     s.pop = pop
     s.poisson_hz, s.poisson_on, s.poisson_off = poisson_hz, poisson_on, poisson_off
     s.burst_hz, s.burst_on, s.burst_off = burst_hz, burst_on, burst_off
     s.periodic_hz, s.periodic_start, s.periodic_on, s.periodic_off = periodic_hz, periodic_start, periodic_on, periodic_off
     s.periodic_off_inclusive, s.poisson_hz_from_freq, s.seed = periodic_off_inclusive, poisson_hz_from_freq, seed
+    s.sine_max_hz, s.sine_cycle_s, s.sine_shift = sine_max_hz, sine_cycle_s, sine_shift
+    s.sine_on, s.sine_off = sine_on, sine_off
     s._keep = []
     tg = np.ascontiguousarray(targets, dtype=np.uint32)
     s._keep.append(tg)
@@ -225,7 +236,9 @@ class OracleSim:
                                    burst_off=f.burst_off, periodic_hz=f.periodic_hz, periodic_start=f.periodic_start,
                                    periodic_on=f.periodic_on, periodic_off=f.periodic_off,
                                    periodic_off_inclusive=f.periodic_off_inclusive,
-                                   poisson_hz_from_freq=f.poisson_hz_from_freq))
+                                   poisson_hz_from_freq=f.poisson_hz_from_freq, sine_max_hz=f.sine_max_hz,
+                                   sine_cycle_s=f.sine_cycle_s, sine_shift=f.sine_shift, sine_on=f.sine_on,
+                                   sine_off=f.sine_off))
         seed = env.get_option("neuron_seed") if neuron_seed is None else neuron_seed
         R = ring_rows if ring_rows is not None else info.ring_rows
         return cls(pops, projs, stims, R, seed, precision, sum_mode, keep)

@@ -247,6 +247,40 @@ static inline int orc_periodic(float t, float freq, float start)
     return a != b;
 }
 
+/* The stimulus cosine: Cody-Waite reduction by pi/2 = HI + LO (HI with 33
+ * significant bits, so k*HI and x - k*HI are exact for |k| < 2^20; pi/2 - HI - LO
+ * < 4e-27), Taylor series of cos and sin on |r| <= pi/4 (truncation < 1e-17), in
+ * double with basic operations in a fixed order, rounded once to float.  The
+ * reference calls CUDA's cosf, whose bits are not published: parity with it is
+ * unpinned; this function is pinned to the float rounding of numpy's double cos. */
+float orc_cosf(float x)
+{
+    const double HI = 0x1.921fb544p+0, LO = 0x1.0b4611a626331p-34, TWO_OVER_PI = 0x1.45f306dc9c883p-1;
+    double xd = (double)x;
+    double k = rint(xd * TWO_OVER_PI);
+    double r = (xd - k * HI) - k * LO;
+    double z = r * r;
+    double c = 1.0 + z * (-1.0 / 2 + z * (1.0 / 24 + z * (-1.0 / 720 + z * (1.0 / 40320 +
+               z * (-1.0 / 3628800 + z * (1.0 / 479001600 + z * (-1.0 / 87178291200.0 +
+               z * (1.0 / 20922789888000.0))))))));
+    double s = r + r * z * (-1.0 / 6 + z * (1.0 / 120 + z * (-1.0 / 5040 + z * (1.0 / 362880 +
+               z * (-1.0 / 39916800 + z * (1.0 / 6227020800.0 + z * (-1.0 / 1307674368000.0 +
+               z * (1.0 / 355687428096000.0))))))));
+    int q = (int)((long long)k & 3);
+    double v = q == 0 ? c : q == 1 ? -s : q == 2 ? -c : s;
+    return (float)v;
+}
+
+/* SinusoidalOscillation (UserInputFunctions.cu:12-14):
+ *   max/2 * (1 - cosf(2*M_PI*(time/sec)/osci + shift))
+ * the phase in double (M_PI and sec promote), converted to float for cosf; the
+ * rest in float.  puff_stim (:33-39) passes it to PoissonProcess as freq. */
+static inline float orc_sine_rate(float t, float max, float cycle, float shift)
+{
+    double phase = 0x1.921fb54442d18p+2 * ((double)t / ORC_SEC) / (double)cycle + (double)shift;
+    return max / 2.f * (1.f - orc_cosf((float)phase));
+}
+
 static inline int orc_stim_fn(const orc_stim *st, float rate, float r, float t)
 {
     double td = (double)t;
@@ -256,6 +290,8 @@ static inline int orc_stim_fn(const orc_stim *st, float rate, float r, float t)
     if (st->periodic_hz != 0.f && st->periodic_on <= td &&
         (st->periodic_off_inclusive ? td <= st->periodic_off : td < st->periodic_off) &&
         orc_periodic(t, st->periodic_hz, st->periodic_start)) f = 1;
+    if (st->sine_max_hz != 0.f && st->sine_on <= td && td < st->sine_off &&
+        orc_poisson(r, orc_sine_rate(t, st->sine_max_hz, st->sine_cycle_s, st->sine_shift))) f = 1;
     return f;
 }
 

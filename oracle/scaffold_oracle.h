@@ -31,6 +31,9 @@ uint32_t orc_philox_u32(uint64_t seed, uint64_t subsequence, uint64_t offset);
 /* u32 -> float uniform in (0,1]: 2^-32 + v*2^-32 in float (the rocRAND mapping,
  * stated in place of cuRAND's, which is not restated here). */
 float orc_uniform(uint32_t v);
+/* cos of a float argument rounded to float; the stimulus's own cosine in place
+ * of cosf in SinusoidalOscillation (UserInputFunctions.cu:12-14) */
+float orc_cosf(float x);
 
 /* ----------------------------------------------------------- description */
 enum { ORC_NORMAL = 0, ORC_INPUT = 1, ORC_OUTPUT = 2 };
@@ -61,6 +64,7 @@ typedef struct {
     float  periodic_hz; float periodic_start; double periodic_on, periodic_off;
     int    periodic_off_inclusive;
     int    poisson_hz_from_freq;
+    float  sine_max_hz; float sine_cycle_s; float sine_shift; double sine_on, sine_off;
     uint64_t seed;
     int    n_entries;
     const int64_t  *fanout_ptr;  /* NULL: entry i drives targets[i]             */

@@ -56,8 +56,29 @@ def test_reference_stimulus_functions(pkg):
     assert tone.poisson_hz == 1.0 and tone.poisson_on == -float("inf")
     assert tone.periodic_hz == 140.0 and np.float32(tone.periodic_start) == np.float32(292.8)
     assert tone.periodic_on == 300.0 and tone.periodic_off == 351.0 and tone.periodic_off_inclusive == 0
+    assert bg.sine_max_hz == 0.0 and tone.sine_max_hz == 0.0
+    puff = pkg.stim_func_reference(2, 7)   # puff_stim, UserInputFunctions.cu:33-39
+    assert puff.poisson_hz == 0.0 and puff.burst_hz == 0.0 and puff.periodic_hz == 0.0
+    assert (puff.sine_max_hz, puff.sine_cycle_s, puff.sine_shift) == (3.0, 2.0, 0.0)
+    assert puff.sine_on == 0.0 and puff.sine_off == float("inf") and puff.seed == 7
     with pytest.raises(pkg.ScaffoldError):
-        pkg.stim_func_reference(2, 0)
+        pkg.stim_func_reference(3, 0)
+
+
+@pytest.mark.parametrize("cycle,shift,ok", [(2.0, 0.0, True), (0.0, 0.0, False), (float("inf"), 0.0, False),
+                                            (2.0, float("nan"), False)])
+def test_sinusoidal_stimulus_validation(pkg, cycle, shift, ok):
+    env = pkg.NetworkEnvironment(1, 1, 0, 1, 0)
+    ch = (1., 0., 1.0, 0.5, 1., -85., 1.0, 10.0)
+    g = env.set_neuron_params(0, "glom", 0, 4, 3.0, -74.0, 1.5, 0.0, -84.0, -42.0, 1.5, pkg.INPUT, ch)
+    f = pkg.stim_func_reference(2, 1)
+    f.sine_cycle_s, f.sine_shift = cycle, shift
+    if ok:
+        env.set_input_stim_ids(g, [0, 1], f)
+    else:
+        with pytest.raises(pkg.ScaffoldError) as e:
+            env.set_input_stim_ids(g, [0, 1], f)
+        assert e.value.code == pkg.ERR_INVALID_ARG
 
 
 def test_index_assignment_matches_reference(pkg):
