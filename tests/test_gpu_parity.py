@@ -351,36 +351,68 @@ def test_fp8_every_code_bit_exact(pkg, orc, fanin, bits):
     env.finalize()
 
 
-def test_sinusoidal_stimulus_bit_exact(pkg, orc):
-    """The reference's puff_stim (UserInputFunctions.cu:12-14, 33-39) with its
-    parameters and a fast variant (period 40 ms, phase 0.7, window [12.35, 361.05)
-    ms, peak 3 kHz), plus 1 Hz background: 400 steps cover every quadrant of the
-    phase a dozen times.  The stimulus cosine runs on the GPU; rasters of the
-    inputs and of the post population and the state agree with the oracle bit for
-    bit."""
+def sine_env(pkg, seed=8, cycle=0.04, shift=0.7):
+    """Inputs driven by 1 Hz background, the reference's puff_stim on every second
+    input and a fast variant (period `cycle` s, phase `shift`, window [12.35,
+    361.05) ms, peak 3 kHz) on every third, onto a post population."""
     granule = (3.0, -74.0, 1.5, 0.0, -84.0, -42.0, 1.5, (1., 0., 1.0, 0.5, 1., -85., 1.0, 10.0))
     Cm, El, dt_ref, Ie, Vr, Vth, gL, ch = granule
     n_in, n_post = 3000, 700
     env = pkg.NetworkEnvironment(1, 2, 1, 3, 0)
     post = env.set_neuron_params(0, "post", 0, n_post, Cm, El, dt_ref, Ie, Vr, Vth, gL, pkg.NORMAL, ch)
     inp = env.set_neuron_params(1, "inputs", 0, n_in, Cm, El, dt_ref, Ie, Vr, Vth, gL, pkg.INPUT, ch)
-    rng = np.random.default_rng(8)
+    rng = np.random.default_rng(seed)
     rp = np.arange(n_post + 1, dtype=np.int64) * 40
     col = rng.integers(0, n_in, n_post * 40).astype(np.int32)
     env.set_connectivity_csr(0, n_in, n_post, inp, post, 2e-3, 1.0, rp, col, None)
-    env.set_input_stim_ids(inp, np.arange(n_in), pkg.stim_func_reference(0, 31))
-    puff = pkg.stim_func_reference(2, 32)
-    env.set_input_stim_ids(inp, np.arange(0, n_in, 2), puff)
-    fast = pkg.stim_func_reference(2, 33)
-    fast.sine_max_hz, fast.sine_cycle_s, fast.sine_shift = 3000.0, 0.04, 0.7
+    env.set_input_stim_ids(inp, np.arange(n_in), pkg.stim_func_reference(0, 23 + seed))
+    env.set_input_stim_ids(inp, np.arange(0, n_in, 2), pkg.stim_func_reference(2, 24 + seed))
+    fast = pkg.stim_func_reference(2, 25 + seed)
+    fast.sine_max_hz, fast.sine_cycle_s, fast.sine_shift = 3000.0, cycle, shift
     fast.sine_on, fast.sine_off = 12.35, 361.05
     env.set_input_stim_ids(inp, np.arange(1, n_in, 3), fast)
+    return env
+
+
+def test_sinusoidal_stimulus_bit_exact(pkg, orc):
+    """The reference's puff_stim (UserInputFunctions.cu:12-14, 33-39) and a fast
+    variant: 400 steps cover every quadrant of the phase a dozen times.  The
+    stimulus cosine runs on the GPU; rasters of the inputs and of the post
+    population and the state agree with the oracle bit for bit."""
+    env = sine_env(pkg)
     env.initialize(50, 25)
     ref = orc.OracleSim.from_env(env)
     ref.initialize()
     run_and_compare(env, ref, 0, 400)
     assert_same_state(env, ref, "sinusoidal stimuli")
     env.finalize()
+
+
+def test_sinusoidal_stimulus_in_batch(pkg, orc):
+    """Sinusoidal stimuli in the batched step kernel: two such networks (other
+    seeds, periods and phases) beside a small scaffold, 150 steps, each against
+    its own oracle run."""
+    envs = [sine_env(pkg, 8), sine_env(pkg, 9, cycle=0.025, shift=-1.3),
+            new_env(pkg, 0, scale=0.05, counts=(0,) * 8, seed=140, stim_seed=41, mf_rate_hz=10.0)]
+    for e in envs:
+        e.initialize(50, 25)
+    refs = []
+    for e in envs:
+        r = orc.OracleSim.from_env(e)
+        r.initialize()
+        refs.append(r)
+    b = pkg.Batch(envs)
+    for t in range(0, 150, 25):
+        b.run(25)
+        for i, (e, r) in enumerate(zip(envs, refs)):
+            r.run(t, 25)
+            for s_ in range(t, t + 25):
+                assert np.array_equal(e.spike_row(s_), r.spike_row(s_)), f"instance {i}, step {s_}"
+    for i, (e, r) in enumerate(zip(envs, refs)):
+        assert_same_state(e, r, f"batch instance {i}")
+    b.destroy()
+    for e in envs:
+        e.finalize()
 
 
 @pytest.mark.parametrize("bits", [32, 16])
