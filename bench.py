@@ -1,18 +1,26 @@
 #!/usr/bin/env python3
 """Benchmark of the exchange-step hot path on one MI355X.
 
-Workload (the plain drop-in): the reference-size cerebellar scaffold, 96,737
-cells and ~3.2M synapses from the seeded scaffold generator (88,158 granule,
-7,073 glomeruli fed by 2,400 mossy fibres, 69 Purkinje, 219 Golgi, 603 stellate,
-603 basket, 12 DCN), FP32 state, FP32 synaptic multipliers, 4 Hz mossy background
-plus a 100 Hz burst to a 50-fibre patch at 300-350 ms.  A step is one exchange
-step (1 ms of simulated time = 10 Euler sub-steps of 0.1 ms).
+Headline workload: the reference-size cerebellar scaffold from the seeded
+scaffold generator with the reference connectome's fan-ins -- 96,737 cells
+(88,158 granule, 7,073 glomeruli fed by 2,400 mossy fibres, 69 Purkinje, 219
+Golgi, 603 stellate, 603 basket, 12 DCN) and 4.23M synapses, FP32 state and FP32
+synaptic multipliers, 4 Hz mossy background plus a 100 Hz burst to a 50-fibre
+patch at 300-350 ms.
+
+A bench step is one iteration of the reference's driver loop (main.cu:50-68):
+SetInputFreq, then loop_n_steps over T_cpu_sim = 25 exchange steps (25 ms
+simulated, each 10 Euler sub-steps of 0.1 ms) with the output spike train handed
+over at its end -- the drop-in path a caller of the reference runs.
 
 Prints one JSON line: synaptic events/s (value) and real-time factor of the timed
-steps, the roofline figure of the dominant kernel (the gather kernel) measured
-with HIP events, the CPU baseline (the single-threaded C oracle on a bounded
-sample of the same network), and the batched BF16 form (64 uneven instances
-advanced together).
+iterations, the roofline figure of the step kernel (its duration measured with
+HIP events that the captured graphs record around the step kernels during the
+timed region), the CPU baseline (the single-threaded C oracle on a bounded sample
+of the same network), and further legs: the device loop alone, the rule-text
+density, 16-bit columns, the reference's Purkinje plasticity rule, 64 batched
+uneven instances against sequential runs of the same instances, and the 16x /
+100x tiled scaffolds.
 """
 import argparse
 import json
@@ -29,6 +37,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "synaptic events/s + real-time factor, 96k scaffold; batched BF16 vs plain FP32"
 HBM_PEAK_GBS = 8000.0
+WINDOW = 25          # T_cpu_sim of the reference's main.cu
+T_PRINT = 50
 
 
 def log(*a):
@@ -44,9 +54,56 @@ def make_env(pkg, preset, plasticities=0, **over):
     return env
 
 
-def advance(env, start, n, chunk=25):
-    for s in range(start, start + n, chunk):
-        env.loop_n_steps(s, min(chunk, start + n - s), 0, want_output=False)
+def proj_nnz(env):
+    """Synapses per projection, by the reference's connection type names."""
+    names = ["pf_pc", "pf_goc", "pf_sc", "pf_bc", "aa_goc", "aa_pc", "goc_goc", "sc_sc", "bc_bc", "glom_dcn",
+             "pc_dcn", "bc_pc", "sc_pc", "glom_grc", "glom_goc", "goc_grc"]
+    out = {}
+    for i in range(env.info().n_projections):
+        p = env.projection(i)
+        if p.pre >= 0:
+            out[names[p.type] if 0 <= p.type < len(names) else str(p.type)] = int(p.nnz)
+    return out
+
+
+def drop_in_loop(env, t0, iterations, freq):
+    """`iterations` driver-loop iterations from step t0: SetInputFreq, then
+    loop_n_steps over one window with the output train handed over."""
+    t = t0
+    for _ in range(iterations):
+        env.set_input_freq(freq)
+        env.loop_n_steps(t, WINDOW, 0, want_output=True)
+        t += WINDOW
+    return t
+
+
+def timed_drop_in(env, t, iterations, freq):
+    """Runs `iterations` driver-loop iterations and measures them: wall time (the
+    calls return when their steps are done), synaptic events, per-population
+    spikes, and the step-kernel time from the HIP events the graphs record."""
+    info = env.info()
+    env.step_timer(1)
+    sp0, ev0 = env.totals()
+    w0 = time.perf_counter()
+    t = drop_in_loop(env, t, iterations, freq)
+    wall = time.perf_counter() - w0
+    kms, launches = env.step_timer(0)
+    sp1, ev1 = env.totals()
+    ex_steps = iterations * WINDOW
+    kmean = kms / max(1, launches)
+    algo = info.algorithmic_bytes_per_step
+    rates = {}
+    for i in range(info.n_populations):
+        p = env.population(i)
+        if p.num:
+            rates[p.name.decode()] = round(float(sp1[i] - sp0[i]) / p.num / (ex_steps * 1e-3), 3)
+    return t, {"wall_s": wall, "events": int(ev1 - ev0), "value": (ev1 - ev0) / wall,
+               "ms_per_iteration": wall * 1e3 / iterations, "exchange_steps": ex_steps,
+               "real_time_factor": ex_steps * 1e-3 / wall,
+               "synapses_scanned_per_s": info.total_synapses * ex_steps / wall,
+               "kernel_mean_ms": kmean, "kernel_launches_timed": int(launches),
+               "algorithmic_bytes_per_step": algo, "achieved_GBs": algo / (kmean * 1e-3) / 1e9 if kmean else None,
+               "firing_rates_hz": rates}
 
 
 def outdegree(env):
@@ -84,6 +141,7 @@ def cpu_baseline(env, steps, threads):
         return events, spent
 
     ev1, t1 = timed(1)
+    log(f"cpu baseline: {steps} steps on 1 thread in {t1:.1f} s")
     evn, tn = timed(threads)
     assert ev1 == evn
     model = ""
@@ -92,61 +150,84 @@ def cpu_baseline(env, steps, threads):
             model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
     except OSError:
         pass
-    return {"host_cpu": model, "host_cpus_visible": os.cpu_count(), "value": ev1 / t1, "unit": "synaptic events/s", "cores": 1, "kind": "port",
+    return {"host_cpu": model, "host_cpus_visible": os.cpu_count(), "value": ev1 / t1, "unit": "synaptic events/s",
+            "cores": 1, "kind": "port",
             "sample": f"first {steps} exchange steps ({steps} ms simulated) of the same 96.7k-cell network, "
                       f"{t1:.1f} s on one thread",
-            "real_time_factor": steps * 1e-3 / t1, "ms_per_step": t1 * 1e3 / steps,
+            "real_time_factor": steps * 1e-3 / t1, "ms_per_exchange_step": t1 * 1e3 / steps,
             "all_cores": {"value": evn / tn, "unit": "synaptic events/s", "cores": threads,
-                          "real_time_factor": steps * 1e-3 / tn, "ms_per_step": tn * 1e3 / steps,
+                          "real_time_factor": steps * 1e-3 / tn, "ms_per_exchange_step": tn * 1e3 / steps,
                           "sample": f"same {steps} steps, OpenMP over {threads} threads, {tn:.1f} s"}}
 
 
+def variant(pkg, iterations, warmup, preset=1, plastic=False, connectome=None, **options):
+    """The plain 96.7k scaffold through the drop-in loop with other options:
+    16-bit column indices (same results bit for bit), the rule-text connectome
+    (connectome 0: 3.03M synapses), or the reference's parallel-fibre-to-Purkinje
+    Teacher rule (A1_J 0, A2_JI -0.01, window 50; SynapticPlasticity.cu:10-16),
+    basket-to-Purkinje standing in for the climbing fibre as the teacher."""
+    over = {} if connectome is None else {"connectome": connectome}
+    env = make_env(pkg, preset, plasticities=1 if plastic else 0, **over)
+    for k, v in options.items():
+        env.set_option(k, v)
+    if plastic:
+        idx = {env.projection(i).type: i for i in range(env.info().n_projections)}
+        env.set_plasticity(pkg.STDP_TEACHER, idx[0], idx[11], {pkg.A1_J: 0.0, pkg.A2_JI: -0.01}, 50)
+    env.initialize(T_PRINT, WINDOW)
+    info = env.info()
+    freq = [4.0, 4.0]
+    t = drop_in_loop(env, 0, warmup, freq)
+    _, r = timed_drop_in(env, t, iterations, freq)
+    r.pop("firing_rates_hz")
+    r.update({"options": dict(options, plasticity="teacher pf->pc" if plastic else None,
+                              connectome=connectome if connectome is not None else 1),
+              "neurons": int(info.total_neurons), "synapses": int(info.total_synapses),
+              "synapses_16bit": int(info.synapses_16bit)})
+    env.finalize()
+    return r
+
+
 def batched(pkg, n_inst, steps, warmup, threads):
-    """64 independent instances of uneven size (0.25x..4x of the 96.7k scaffold),
-    each with its own seed and mossy rate U(1, 60) Hz, BF16 multipliers, advanced
-    together by one step kernel per exchange step (hipGraph replays)."""
+    """Config 3: 64 independent instances of uneven size (0.25x..4x of the 96.7k
+    scaffold), each with its own seeds and mossy rate U(1, 60) Hz, BF16
+    multipliers, advanced together by one step kernel per exchange step
+    (hipGraph replays), against sequential plain runs of the same instances: a
+    second, identical copy of every instance advanced alone, one after another.
+    Both copies run the same steps from initialization, so their spike totals
+    must agree exactly."""
     scales = ([0.25] * 16 + [0.5] * 16 + [1.0] * 16 + [2.0] * 8 + [4.0] * 8)[:n_inst]
     rng = np.random.default_rng(1234)
     rates = rng.uniform(1.0, 60.0, len(scales))
-    envs = [None] * len(scales)
+    # autotune the layout on one instance of the largest scale (4x; the 4x
+    # instances hold 42 % of the batch's synapses, and at that size the step is
+    # throughput-bound like the batch) and use it for all
+    probe = make_env(pkg, 1, scale=max(scales), burst_n_mf=max(1, int(round(50 * max(scales)))),
+                     mf_rate_hz=float(np.median(rates)))
+    probe.set_option("weight_precision", pkg.WEIGHTS_BF16)
+    tuned = probe.autotune(100)
+    probe.finalize()
+    log(f"batch: autotuned {tuned}")
+    layout = {k: tuned[k] for k in ("lane_max_fanin", "lane_synapses", "graph_steps", "index_bits", "stage_spikes")}
+    envs = [[None] * len(scales), [None] * len(scales)]
     errs = []
 
-    def work(idx):
+    def work(items):
         try:
-            for i in idx:
-                e = pkg.NetworkEnvironment(1, 7, 16, 2, 0)
-                cfg = pkg.synth_preset(1)
-                cfg.scale = scales[i]
-                cfg.seed = 1000 + i
-                cfg.stim_seed = 5000 + i
-                cfg.mf_rate_hz = float(rates[i])
-                cfg.burst_n_mf = max(1, int(round(50 * scales[i])))
-                e.describe_synthetic(cfg)
+            for c, i in items:
+                e = make_env(pkg, 1, scale=scales[i], seed=1000 + i, stim_seed=5000 + i, mf_rate_hz=float(rates[i]),
+                             burst_n_mf=max(1, int(round(50 * scales[i]))))
                 e.set_option("weight_precision", pkg.WEIGHTS_BF16)
                 e.set_option("neuron_seed", 77 + i)
                 for k, v in layout.items():
                     e.set_option(k, v)
-                e.initialize(50, 25)
-                envs[i] = e
+                e.initialize(T_PRINT, WINDOW)
+                envs[c][i] = e
         except Exception as ex:  # surfaced below
             errs.append(ex)
 
-    # autotune the layout on one instance of the largest scale (4x; the 4x
-    # instances hold 42 % of the batch's synapses, and at that size the step is
-    # throughput-bound like the batch) and use it for all
-    probe = pkg.NetworkEnvironment(1, 7, 16, 2, 0)
-    cfg = pkg.synth_preset(1)
-    cfg.scale = max(scales)
-    cfg.burst_n_mf = max(1, int(round(50 * cfg.scale)))
-    cfg.mf_rate_hz = float(np.median(rates))
-    probe.describe_synthetic(cfg)
-    probe.set_option("weight_precision", pkg.WEIGHTS_BF16)
-    tuned = probe.autotune(100)
-    probe.finalize()
-    layout = {k: tuned[k] for k in ("lane_max_fanin", "lane_synapses", "graph_steps", "index_bits", "stage_spikes")}
-
-    order = sorted(range(len(scales)), key=lambda i: -scales[i])
+    order = sorted(((c, i) for c in (0, 1) for i in range(len(scales))), key=lambda x: -scales[x[1]])
     parts = [order[k::threads] for k in range(threads)]
+    t0 = time.perf_counter()
     ths = [threading.Thread(target=work, args=(p,)) for p in parts]
     for t in ths:
         t.start()
@@ -154,58 +235,50 @@ def batched(pkg, n_inst, steps, warmup, threads):
         t.join()
     if errs:
         raise errs[0]
-    b = pkg.Batch(envs)  # replays graphs of the first instance's graph_steps
+    log(f"batch: built 2 x {len(scales)} instances in {time.perf_counter() - t0:.1f} s")
+    batch_envs, seq_envs = envs
+    # sequential plain runs: each instance alone, one after another
+    seq_wall = 0.0
+    for e in seq_envs:
+        e.time_steps(warmup)
+    for e in seq_envs:
+        w0 = time.perf_counter()
+        e.time_steps(steps)
+        seq_wall += time.perf_counter() - w0
+    log(f"batch: sequential runs {seq_wall * 1e3 / steps:.3f} ms per step")
+    b = pkg.Batch(batch_envs)  # replays graphs of the first instance's graph_steps
     b.run(warmup)
-    before = sum(e.totals()[1] for e in envs)
-    t0 = time.perf_counter()
+    before = sum(e.totals()[1] for e in batch_envs)
+    w0 = time.perf_counter()
     ms = b.time(steps)
-    wall = time.perf_counter() - t0
-    after = sum(e.totals()[1] for e in envs)
-    syn = sum(e.info().total_synapses for e in envs)
-    neurons = sum(e.info().total_neurons for e in envs)
+    wall = time.perf_counter() - w0
+    after = sum(e.totals()[1] for e in batch_envs)
+    same = all(np.array_equal(x.totals()[0], y.totals()[0]) and x.totals()[1] == y.totals()[1]
+               for x, y in zip(batch_envs, seq_envs))
+    syn = sum(e.info().total_synapses for e in batch_envs)
+    neurons = sum(e.info().total_neurons for e in batch_envs)
     b.destroy()
-    for e in envs:
+    for e in batch_envs + seq_envs:
         e.finalize()
-    return {"value": (after - before) / wall, "unit": "synaptic events/s", "instances": len(scales),
+    value = (after - before) / wall
+    # the sequential copies' events of their timed steps equal the batch's (same
+    # instances, same steps), so the ratio of events/s is the ratio of wall times
+    seq_value = (after - before) / seq_wall
+    return {"value": value, "unit": "synaptic events/s", "instances": len(scales),
+            "scales": {str(s): scales.count(s) for s in sorted(set(scales))},
             "synapses_scanned_per_s": syn * steps / wall,
-            "neurons": int(neurons), "synapses": int(syn), "steps": steps, "ms_per_step": wall * 1e3 / steps,
-            "gpu_ms_per_step": ms / steps, "dtype": "bf16 multipliers, f32 state",
-            "aggregate_real_time_factor": len(scales) * steps * 1e-3 / wall, "autotuned": tuned}
-
-
-def plain_variant(pkg, steps, warmup, plastic=False, **options):
-    """The plain FP32 96.7k scaffold with other storage options (e.g. 16-bit
-    column indices: same results bit for bit, other memory traffic), or with the
-    reference's parallel-fibre-to-Purkinje Teacher rule (A1_J 0, A2_JI -0.01,
-    window 50; SynapticPlasticity.cu:10-16), basket-to-Purkinje standing in for
-    the climbing fibre as the teacher projection."""
-    env = make_env(pkg, 1, plasticities=1 if plastic else 0)
-    for k, v in options.items():
-        env.set_option(k, v)
-    if plastic:
-        idx = {env.projection(i).type: i for i in range(env.info().n_projections)}
-        env.set_plasticity(pkg.STDP_TEACHER, idx[0], idx[11], {pkg.A1_J: 0.0, pkg.A2_JI: -0.01}, 50)
-    env.initialize(50, 25)
-    info = env.info()
-    advance(env, 0, warmup)
-    _, ev0 = env.totals()
-    t0 = time.perf_counter()
-    env.time_steps(steps)
-    wall = time.perf_counter() - t0
-    _, ev1 = env.totals()
-    _, kmean = env.time_steps(100, kernel_mean=True)
-    algo = info.algorithmic_bytes_per_step
-    out = {"options": dict(options, plasticity="teacher pf->pc" if plastic else None),
-           "value": (ev1 - ev0) / wall, "ms_per_step": wall * 1e3 / steps,
-           "real_time_factor": steps * 1e-3 / wall, "kernel_mean_ms": kmean, "algorithmic_bytes_per_step": algo,
-           "achieved_GBs": algo / (kmean * 1e-3) / 1e9, "synapses_16bit": info.synapses_16bit}
-    env.finalize()
-    return out
+            "neurons": int(neurons), "synapses": int(syn), "steps": steps, "warmup": warmup,
+            "ms_per_step": wall * 1e3 / steps, "gpu_ms_per_step": ms / steps, "dtype": "bf16 multipliers, f32 state",
+            "aggregate_real_time_factor": len(scales) * steps * 1e-3 / wall, "autotuned": tuned,
+            "sequential": {"value": seq_value, "ms_per_step": seq_wall * 1e3 / steps,
+                           "synapses_scanned_per_s": syn * steps / seq_wall,
+                           "spike_totals_identical_to_batch": bool(same)},
+            "vs_sequential": seq_wall / wall}
 
 
 def scaled(pkg, preset, steps, precisions):
-    """A tiled scaffold (preset 2: 16x in area, 1.55M cells, 50.7M synapses;
-    preset 4: 100x, 9.67M cells, 326M synapses) with the given multiplier storage
+    """A tiled scaffold (preset 2: 16x in area, 1.55M cells, 67M synapses;
+    preset 4: 100x, 9.67M cells, 421M synapses) with the given multiplier storage
     (an entry is a precision, or (precision, index bits)): step time, real-time
     factor, synaptic events/s and the step kernel's algorithmic bandwidth."""
     out = {}
@@ -213,16 +286,19 @@ def scaled(pkg, preset, steps, precisions):
     for entry in precisions:
         prec, bits = entry if isinstance(entry, tuple) else (entry, 32)
         name = names[prec] + ("_idx16" if bits == 16 else "")
+        t0 = time.perf_counter()
         env = make_env(pkg, preset)
         env.set_option("weight_precision", prec)
         env.set_option("index_bits", bits)
-        env.initialize(50, 25)
+        env.initialize(T_PRINT, WINDOW)
         info = env.info()
-        advance(env, 0, 50)
+        log(f"scaffold preset {preset} {name}: built in {time.perf_counter() - t0:.1f} s")
+        env.time_steps(50)
         _, ev0 = env.totals()
-        t0 = time.perf_counter()
+        env.step_timer(1)
+        w0 = time.perf_counter()
         env.time_steps(steps)
-        wall = time.perf_counter() - t0
+        wall = time.perf_counter() - w0
         _, ev1 = env.totals()
         _, kmean = env.time_steps(50, kernel_mean=True)
         algo = info.algorithmic_bytes_per_step
@@ -240,20 +316,22 @@ def scaled(pkg, preset, steps, precisions):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=40,
+                    help="timed driver-loop iterations of 25 exchange steps each (40 = 1 s simulated)")
+    ap.add_argument("--warmup", type=int, default=4, help="untimed driver-loop iterations before them")
     ap.add_argument("--cpu-steps", type=int, default=1000)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
-    ap.add_argument("--no-plastic", action="store_true")
-    ap.add_argument("--no-16x", action="store_true")
+    ap.add_argument("--no-scaled", action="store_true")
+    ap.add_argument("--full", action="store_true",
+                    help="tiled scaffolds in every storage form (default: FP8 with 16-bit columns only)")
     ap.add_argument("--steps-16x", type=int, default=200)
-    ap.add_argument("--no-100x", action="store_true")
     ap.add_argument("--steps-100x", type=int, default=100)
     ap.add_argument("--batch-instances", type=int, default=64)
-    ap.add_argument("--batch-steps", type=int, default=1000)
-    ap.add_argument("--kernel-steps", type=int, default=100)
+    ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--device-steps", type=int, default=1000)
     ap.add_argument("--only-scaled", type=int, default=0, choices=(0, 2, 4),
                     help="only run the tiled scaffold of this preset (2: 16x, 4: 100x) and exit")
     ap.add_argument("--scaled-precision", default="fp32", choices=("fp32", "bf16", "fp8"),
@@ -281,26 +359,17 @@ def main():
     t0 = time.perf_counter()
     env = make_env(pkg, 1)
     env.set_option("graph", 1)
-    env.set_option("graph_steps", 25)
-    env.initialize(50, 25)
+    env.set_option("graph_steps", WINDOW)
+    env.initialize(T_PRINT, WINDOW)
     info = env.info()
     log(f"built scaffold: {info.total_neurons} cells, {info.total_synapses} synapses, "
         f"{info.n_tasks} wavefront tasks, {time.perf_counter() - t0:.1f} s")
-
-    advance(env, 0, args.warmup)
-    # timed region: exactly `steps` exchange steps, synchronized before and after
-    sp0, ev0 = env.totals()
-    t0 = time.perf_counter()
-    gpu_ms = env.time_steps(args.steps)
-    wall = time.perf_counter() - t0
-    sp1, ev1 = env.totals()
-    events = ev1 - ev0
-    value = events / wall
-    rtf = args.steps * 1e-3 / wall
-    # mean duration of the step kernel over runs of back-to-back launches
-    _, kmean = env.time_steps(args.kernel_steps, kernel_mean=True)
-    algo = info.algorithmic_bytes_per_step
-    achieved = algo / (kmean * 1e-3) / 1e9
+    freq = [4.0, 4.0]  # CalcFreq's background rate (main.cu:7); the burst patch keeps its own
+    t = drop_in_loop(env, 0, args.warmup, freq)
+    # timed region: exactly `steps` driver-loop iterations; each call returns when
+    # its exchange steps are done and its output train is in host memory
+    t, r = timed_drop_in(env, t, args.steps, freq)
+    log(f"timed: {args.steps} iterations, {r['ms_per_iteration']:.3f} ms each, kernel mean {r['kernel_mean_ms']:.5f} ms")
     # HBM-side bytes per k_step launch from the FETCH_SIZE / WRITE_SIZE counter
     # passes, calibrated against a copy kernel of known size
     traffic = None
@@ -309,49 +378,66 @@ def main():
         with open(tpath) as f:
             m = re.search(r"^hbm_bytes_per_launch = (\d+)$", f.read(), re.M)
         traffic = int(m.group(1)) if m else None
-    rates = {}
-    for i in range(info.n_populations):
-        p = env.population(i)
-        if p.num:
-            rates[p.name.decode()] = round(float(sp1[i] - sp0[i]) / p.num / (args.steps * 1e-3), 3)
+    kmean = r["kernel_mean_ms"]
+    achieved = r["achieved_GBs"]
     line = {
-        "metric": METRIC, "value": value, "unit": "synaptic events/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True,
+        "metric": METRIC, "value": r["value"], "unit": "synaptic events/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": r["ms_per_iteration"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic: seeded scaffold generator (reference population sizes, rule-based connectivity)",
-        "config": {"workload": "scaffold-96k plain fp32 (1 s = 1000 exchange steps of 10 x 0.1 ms)",
-                   "column_index_bits": 32,
-                   "neurons": info.total_neurons, "synapses": info.total_synapses,
-                   "simulated_ms": args.steps, "input": "4 Hz mossy Poisson + 100 Hz burst to 50 fibres, 300-350 ms"},
-        "real_time_factor": rtf,
-        "synapses_scanned_per_s": info.total_synapses * args.steps / wall,
-        "gpu_ms_per_step": gpu_ms / args.steps,
-        "events_in_timed_region": int(events),
-        "firing_rates_hz": rates,
+        "data": "synthetic: seeded scaffold generator (reference population sizes, reference connectome fan-ins)",
+        "config": {"workload": "scaffold-96k plain fp32, drop-in driver loop",
+                   "step": f"one loop_n_steps call of T_cpu_sim = {WINDOW} exchange steps ({WINDOW} ms simulated, "
+                           f"10 x 0.1 ms Euler sub-steps each) with SetInputFreq before it and the output spike "
+                           f"train handed over after it",
+                   "exchange_steps_timed": r["exchange_steps"], "simulated_ms": r["exchange_steps"],
+                   "first_timed_exchange_step": args.warmup * WINDOW,
+                   "column_index_bits": 32, "neurons": info.total_neurons, "synapses": info.total_synapses,
+                   "synapses_per_projection": proj_nnz(env),
+                   "input": "4 Hz mossy Poisson + 100 Hz burst to 50 fibres, 300-350 ms"},
+        "real_time_factor": r["real_time_factor"],
+        "synapses_scanned_per_s": r["synapses_scanned_per_s"],
+        "events_in_timed_region": r["events"],
+        "firing_rates_hz": r["firing_rates_hz"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_step", "kernel_mean_ms": kmean, "algorithmic_bytes_per_launch": algo},
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
+                     "kernel": "k_step", "kernel_mean_ms": kmean, "launches_timed": r["kernel_launches_timed"],
+                     "algorithmic_bytes_per_launch": r["algorithmic_bytes_per_step"],
+                     "timing": "HIP events recorded by the captured graphs around the step kernels, timed region"},
     }
-    env_for_cpu = env
+    # the device loop alone (graph replays of 25 steps, no output train, no host calls between)
+    env.step_timer(1)
+    w0 = time.perf_counter()
+    env.time_steps(args.device_steps)
+    wall = time.perf_counter() - w0
+    kms, launches = env.step_timer(0)
+    line["device_loop"] = {"exchange_steps": args.device_steps, "ms_per_exchange_step": wall * 1e3 / args.device_steps,
+                           "real_time_factor": args.device_steps * 1e-3 / wall,
+                           "kernel_mean_ms": kms / max(1, launches)}
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(env_for_cpu, args.cpu_steps, args.cpu_threads)
+        line["cpu_baseline"] = cpu_baseline(env, args.cpu_steps, args.cpu_threads)
     env.finalize()
-    if not args.no_batched:  # the same network with 16-bit column offsets (bit-identical results)
-        line["plain_fp32_idx16"] = plain_variant(pkg, args.steps, args.warmup, index_bits=16)
-    if not args.no_plastic:  # the same network with the reference's Purkinje plasticity rule switched on
-        line["plain_fp32_plastic"] = plain_variant(pkg, args.steps, args.warmup, plastic=True)
+    if not args.no_variants:
+        line["plain_fp32_rule_text_density"] = variant(pkg, args.steps, args.warmup, connectome=0)
+        log("variant: rule-text density done")
+        line["plain_fp32_idx16"] = variant(pkg, args.steps, args.warmup, index_bits=16)
+        log("variant: 16-bit columns done")
+        line["plain_fp32_plastic"] = variant(pkg, args.steps, args.warmup, plastic=True)
+        log("variant: plasticity done")
     if not args.no_batched:
         b = batched(pkg, args.batch_instances, args.batch_steps, 50, threads=min(16, os.cpu_count() or 4))
-        b["vs_plain"] = b["value"] / value
+        b["vs_plain"] = b["value"] / line["value"]
         b["vs_plain_synapses_scanned"] = b["synapses_scanned_per_s"] / line["synapses_scanned_per_s"]
         line["batched_bf16"] = b
-    if not args.no_16x:
+        log("batched done")
+    if not args.no_scaled:
+        full = args.full
         line["scaffold16x"] = scaled(pkg, 2, args.steps_16x,
                                      (pkg.WEIGHTS_F32, pkg.WEIGHTS_BF16, pkg.WEIGHTS_FP8, (pkg.WEIGHTS_F32, 16),
-                                      (pkg.WEIGHTS_BF16, 16), (pkg.WEIGHTS_FP8, 16)))
-    if not args.no_100x:
+                                      (pkg.WEIGHTS_BF16, 16), (pkg.WEIGHTS_FP8, 16)) if full
+                                     else ((pkg.WEIGHTS_FP8, 16),))
         line["scaffold100x"] = scaled(pkg, 4, args.steps_100x,
-                                      ((pkg.WEIGHTS_FP8, 16), pkg.WEIGHTS_FP8, (pkg.WEIGHTS_F32, 16), pkg.WEIGHTS_F32))
+                                      ((pkg.WEIGHTS_FP8, 16), pkg.WEIGHTS_FP8, (pkg.WEIGHTS_F32, 16), pkg.WEIGHTS_F32)
+                                      if full else ((pkg.WEIGHTS_FP8, 16),))
     print(json.dumps(line))
 
 

@@ -55,7 +55,7 @@ class SynthConfig(ctypes.Structure):
                 ("weight_jitter", ctypes.c_int32), ("mf_rate_hz", ctypes.c_float), ("burst_rate_hz", ctypes.c_float),
                 ("burst_on_ms", ctypes.c_double), ("burst_off_ms", ctypes.c_double), ("burst_n_mf", ctypes.c_int32),
                 ("burst_fraction", ctypes.c_double), ("stim_seed", ctypes.c_uint64), ("max_delay", ctypes.c_int32),
-                ("reference_delays", ctypes.c_int32)]
+                ("reference_delays", ctypes.c_int32), ("connectome", ctypes.c_int32)]
 
 
 class Info(ctypes.Structure):
@@ -154,6 +154,8 @@ def load_library(path=None):
         "scaffold_get_fired": (ctypes.c_int, [vp, ctypes.c_int, P(i32), i32, P(i32)]),
         "scaffold_get_totals": (ctypes.c_int, [vp, P(u64), P(u64)]),
         "scaffold_write_spike_text": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, cs]),
+        "scaffold_write_output_spike_text": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, cs]),
+        "scaffold_step_timer": (ctypes.c_int, [vp, ctypes.c_int, P(f64), P(i64)]),
         "scaffold_time_steps": (ctypes.c_int, [vp, ctypes.c_int, P(f64), P(f64)]),
         "scaffold_autotune": (ctypes.c_int, [vp, ctypes.c_int, vp]),
         "scaffold_batch_create": (ctypes.c_int, [P(vp)]),
@@ -314,15 +316,18 @@ class NetworkEnvironment:
         _check(self._lib.scaffold_initialize_network_env(self._h, T_print, T_cpu_sim))
 
     def loop_n_steps(self, start_step, n_steps, trial=0, want_output=True):
+        """Advance n_steps from start_step.  With want_output, returns the output
+        spike train (T_cpu_sim x output neurons, uint8) of the last T_cpu_sim window
+        the call completed, or None when it completed none."""
         out = ctypes.c_void_p()
         _check(self._lib.scaffold_loop_n_steps(self._h, start_step, n_steps, trial,
                                                ctypes.byref(out) if want_output else None))
-        if not want_output:
+        if not want_output or not out.value:
             return None
         info = self.info()
         n = info.T_cpu_sim * info.n_output_neurons
-        if n == 0 or not out.value:
-            return np.zeros((info.T_cpu_sim, info.n_output_neurons), dtype=np.uint8)
+        if n == 0:
+            return np.zeros((info.T_cpu_sim, 0), dtype=np.uint8)
         buf = (ctypes.c_uint8 * n).from_address(out.value)
         return np.frombuffer(buf, dtype=np.uint8).copy().reshape(info.T_cpu_sim, info.n_output_neurons)
 
@@ -431,7 +436,20 @@ class NetworkEnvironment:
         return ids[:min(n.value, cap)], n.value
 
     def write_spike_text(self, step0, n, filename):
+        """The reference printer thread's lines (device populations) of n steps."""
         _check(self._lib.scaffold_write_spike_text(self._h, step0, n, str(filename).encode()))
+
+    def write_output_spike_text(self, step0, n, filename):
+        """The reference host thread's lines (OUTPUT populations) of n steps."""
+        _check(self._lib.scaffold_write_output_spike_text(self._h, step0, n, str(filename).encode()))
+
+    def step_timer(self, on=-1):
+        """Step-kernel timer: on=1 clears and starts it, 0 stops it, -1 reads it.
+        Returns (milliseconds measured, step launches covered)."""
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        _check(self._lib.scaffold_step_timer(self._h, on, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
 
     def time_steps(self, n_steps, kernel_mean=False):
         total = ctypes.c_double()

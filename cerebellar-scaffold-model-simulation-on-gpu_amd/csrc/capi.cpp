@@ -273,12 +273,12 @@ int scaffold_set_connectivity_csr(scaffold_env *env, int type, int preNum, int p
         if (row_ptr[j + 1] < row_ptr[j]) return fail(SCAFFOLD_ERR_FORMAT, "row_ptr decreases at row %d", j);
     const int64_t nnz = row_ptr[postNum];
     if (nnz > 0 && !col) return fail(SCAFFOLD_ERR_INVALID_ARG, "null col");
-    if (postNum > post_n && row_ptr[postNum] != row_ptr[post_n])
-        return fail(SCAFFOLD_ERR_FORMAT, "synapses target rows beyond the post population");
     Projection p;
     p.type = type;
     p.preNum = preNum; p.postNum = postNum; p.pre = preType; p.post = postType;
     p.weight = initial_weight; p.delay_ms = delay;
+    // rows beyond the post population are dropped (the reference allocates them
+    // and never reads them, init_params.cu:743)
     const int rows = postNum < post_n ? postNum : post_n;
     p.row_ptr.assign(post_n + 1, 0);
     for (int j = 0; j <= post_n; ++j) p.row_ptr[j] = row_ptr[j <= rows ? j : rows];
@@ -524,40 +524,28 @@ int scaffold_loop_n_steps(scaffold_env *env, int start_step, int n_steps, int tr
     int rc = check_ready(env);
     if (rc) return rc;
     if (start_step < 0 || n_steps < 0) return fail(SCAFFOLD_ERR_INVALID_ARG, "negative step");
-    if (n_steps % env->T_cpu_sim)
-        return fail(SCAFFOLD_ERR_INVALID_ARG, "n_steps (%d) must be a multiple of T_cpu_sim (%d)", n_steps,
-                    env->T_cpu_sim);
-    const int R = env->plan.R;
     if (start_step != env->cur_step) {
         rc = engine_set_ctl(env->eng, start_step, env->draws);
         if (rc) return rc;
         env->cur_step = start_step;
     }
-    rc = engine_run(env->eng, n_steps, env->opt.graph, env->opt.graph_steps);
+    // Hand-off (GPUScaffoldSimulation.cu:768-783): after every step that completes a
+    // window of T_cpu_sim steps (step count n with n % T_cpu_sim == 0; the ring
+    // holds a multiple of T_cpu_sim rows there, so this is its target_row test).
+    // The train of the last window completed in this call is gathered on the
+    // device and copied once; a call that completes no window leaves
+    // *output_spike_train as it was, as the reference does.
+    const int T = env->T_cpu_sim;
+    const int64_t end = env->cur_step + n_steps;
+    const int64_t last = end - end % T;  // last window boundary at or before end
+    const bool handoff = output_spike_train && n_steps > 0 && last > env->cur_step;
+    const uint8_t *train = nullptr;
+    rc = engine_run_output(env->eng, n_steps, env->opt.graph, env->opt.graph_steps, last - T,
+                           handoff ? &train : nullptr);
     if (rc) return rc;
-    env->cur_step += n_steps;
+    env->cur_step = end;
     env->draws += n_steps;
-    if (output_spike_train) {
-        const Plan &P = env->plan;
-        const int T = env->T_cpu_sim;
-        env->out_train.assign((size_t)T * (size_t)(P.n_output > 0 ? P.n_output : 1), 0);
-        env->tmp_words.resize(P.Wr);
-        for (int s = 0; s < T && n_steps > 0; ++s) {
-            int64_t step = env->cur_step - T + s;
-            rc = engine_get_ring_row(env->eng, (int)(step % R), env->tmp_words.data());
-            if (rc) return rc;
-            int col = 0;
-            for (int p : P.out_pops) {
-                const ScPop &sp = P.pops[p];
-                for (int i = 0; i < sp.num; ++i, ++col) {
-                    int d = sp.dev_base + i;
-                    env->out_train[(size_t)s * P.n_output + col] =
-                        (char)((env->tmp_words[d >> 6] >> (d & 63)) & 1u);
-                }
-            }
-        }
-        *output_spike_train = env->out_train.data();
-    }
+    if (handoff) *output_spike_train = reinterpret_cast<const char *>(train);
     return SCAFFOLD_OK;
 }
 
@@ -823,11 +811,14 @@ int scaffold_set_state(scaffold_env *env, const float *V, const float *g, const 
     return engine_set_state(env->eng, V, g, refr);
 }
 
+// Ring row of a queryable step: one of the last R - 1 steps.  (Step n clears the
+// row of step n + 1, which is the row of step n + 1 - R, so after a run ending
+// at cur_step the row of cur_step - R no longer holds that step's spikes.)
 static int step_row(scaffold_env *env, int step, int *row)
 {
     const int R = env->plan.R;
-    if (step < 0 || step >= env->cur_step || step < env->cur_step - R)
-        return fail(SCAFFOLD_ERR_INVALID_ARG, "step %d is not among the last %d steps (next step %lld)", step, R,
+    if (step < 0 || step >= env->cur_step || step <= env->cur_step - R)
+        return fail(SCAFFOLD_ERR_INVALID_ARG, "step %d is not among the last %d steps (next step %lld)", step, R - 1,
                     (long long)env->cur_step);
     *row = step % R;
     return SCAFFOLD_OK;
@@ -893,12 +884,26 @@ int scaffold_get_fired(scaffold_env *env, int step, int32_t *ids, int32_t cap, i
     return engine_get_fired(env->eng, row, ids, cap, n);
 }
 
-int scaffold_write_spike_text(scaffold_env *env, int step0, int n, const char *filename)
+// The spike lines of steps [step0, step0 + n) in the reference's text format.
+// output == 0: the lines print_spike_train writes for the populations simulated
+// on the device (GPUScaffoldSimulation.cu:76-84): "%f\t%d\t%d\n" = (step as ms,
+// population-local id, ReverseNeuronTypeID[internal index]); the printer's table
+// is ReverseNeuronTypeID (:179-183, :429), the inverse of NeuronTypeID, so the
+// last column is the population's NeuronType -- the `type` it was registered
+// with.  OUTPUT populations are skipped: the reference simulates them on the
+// host, and their device bytes stay zero.
+// output == 1: the lines host_update_lif_ch writes for the OUTPUT populations
+// (Simulation_header.cu:456): (step as ms, global id, NeuronType), through the
+// same inverse table (:539).
+static int write_spike_lines(scaffold_env *env, int step0, int n, const char *filename, int output)
 {
-    clear_error();
     int rc = check_ready(env);
     if (rc) return rc;
     if (!filename || n < 0) return fail(SCAFFOLD_ERR_INVALID_ARG, "bad arguments");
+    for (int s = step0; s < step0 + n; ++s) {  // every step queryable before anything is written
+        int row;
+        if ((rc = step_row(env, s, &row))) return rc;
+    }
     FILE *fp = fopen(filename, "a");
     if (!fp) return fail(SCAFFOLD_ERR_IO, "cannot open %s", filename);
     const Plan &P = env->plan;
@@ -908,18 +913,40 @@ int scaffold_write_spike_text(scaffold_env *env, int step0, int n, const char *f
         rc = step_row(env, s, &row);
         if (!rc) rc = engine_get_ring_row(env->eng, row, env->tmp_words.data());
         if (rc) { fclose(fp); return rc; }
-        const float t = (float)s;
-        // print_spike_train: "%f\t%d\t%d\n" = (ms, local id, population), :79-82
+        const float t = (float)s;  // CTYPE t = (n + step), :78
         for (int p = 0; p < (int)P.pops.size(); ++p) {
             const ScPop &sp = P.pops[p];
+            if ((sp.kind == SCAFFOLD_OUTPUT) != (output != 0)) continue;
             for (int i = 0; i < sp.num; ++i) {
                 int d = sp.dev_base + i;
-                if ((env->tmp_words[d >> 6] >> (d & 63)) & 1u) fprintf(fp, "%f\t%d\t%d\n", t, i, env->pops[p].type);
+                if ((env->tmp_words[d >> 6] >> (d & 63)) & 1u)
+                    fprintf(fp, "%f\t%d\t%d\n", t, output ? sp.ref_base + i : i, env->pops[p].type);
             }
         }
     }
     fclose(fp);
     return SCAFFOLD_OK;
+}
+
+int scaffold_write_spike_text(scaffold_env *env, int step0, int n, const char *filename)
+{
+    clear_error();
+    return write_spike_lines(env, step0, n, filename, 0);
+}
+
+int scaffold_write_output_spike_text(scaffold_env *env, int step0, int n, const char *filename)
+{
+    clear_error();
+    return write_spike_lines(env, step0, n, filename, 1);
+}
+
+int scaffold_step_timer(scaffold_env *env, int on, double *ms, int64_t *launches)
+{
+    clear_error();
+    int rc = check_ready(env);
+    if (rc) return rc;
+    if (on < -1 || on > 1) return fail(SCAFFOLD_ERR_INVALID_ARG, "on is -1, 0 or 1");
+    return engine_timer(env->eng, on, ms, launches, nullptr);
 }
 
 int scaffold_time_steps(scaffold_env *env, int n_steps, double *ms_total, double *ms_step_kernel_mean)

@@ -22,6 +22,23 @@ struct TaskWin {
     int64_t segs = 0;
 };
 
+// Optional timing of the step kernels inside the runs: one event pair around the
+// step launches of every run piece (direct launches), and inside a captured graph
+// one pair recorded by the graph itself, read after each call (so of a call that
+// replays the graph several times, the last replay is measured).
+struct KTimer {
+    bool on = false;
+    hipEvent_t g0 = nullptr, g1 = nullptr;   // captured into the graph
+    bool graph_events = false;               // the current graph was captured for a timed run
+    bool g_missing = false;                  // ... but without the events (capture refused them)
+    int graph_pending = 0;                   // step launches of the last replay (0: none)
+    std::vector<hipEvent_t> pool;            // direct pieces: pairs
+    size_t used = 0;
+    std::vector<int> piece_k;
+    double ms = 0.0;
+    int64_t launches = 0;
+};
+
 struct Engine {
     hipStream_t stream = nullptr;
     std::vector<void *> allocs;
@@ -43,6 +60,13 @@ struct Engine {
     int Ndev = 0, total_ref = 0;
     hipGraphExec_t gexec = nullptr;
     int gexec_steps = 0;
+    KTimer kt;
+    // output spike train: device bits of the OUTPUT neurons, the T_out x n_out
+    // byte buffer on the device and its pinned host copy
+    const int32_t *out_bits = nullptr;
+    uint8_t *out_d = nullptr;
+    uint8_t *out_h = nullptr;
+    int n_out = 0, T_out = 0;
     // plastic projections: multiplier array and per-synapse index, for read-back
     struct PlasticView {
         int proj;
@@ -475,6 +499,19 @@ int engine_create(Engine **out, Plan &P)
     e->w.max_groups = P.n_groups;
     e->w.nontemporal = P.nontemporal;
     if ((rc = dalloc(A, B, &e->cnt_buf, 1))) return bail(rc);
+    // output spike train (T_out steps x n_out neurons), copied to pinned host memory
+    e->n_out = P.n_output;
+    e->T_out = std::max(1, P.T_out);
+    {
+        int32_t *ob;
+        if ((rc = dupload(A, B, &ob, P.out_bits))) return bail(rc);
+        e->out_bits = ob;
+        const size_t nb = (size_t)e->T_out * (size_t)std::max(1, e->n_out);
+        if ((rc = dalloc(A, B, &e->out_d, nb))) return bail(rc);
+        if (hipHostMalloc(reinterpret_cast<void **>(&e->out_h), nb, hipHostMallocDefault) != hipSuccess)
+            return bail(fail(SCAFFOLD_ERR_HIP, "hipHostMalloc of the output train failed"));
+        memset(e->out_h, 0, nb);
+    }
     std::vector<ScInstance> one{I};
     if ((rc = dupload(A, B, &e->di, one))) return bail(rc);
     *out = e;
@@ -487,6 +524,10 @@ void engine_destroy(Engine *e)
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
     for (void *p : e->allocs) (void)hipFree(p);
+    if (e->out_h) (void)hipHostFree(e->out_h);
+    if (e->kt.g0) (void)hipEventDestroy(e->kt.g0);
+    if (e->kt.g1) (void)hipEventDestroy(e->kt.g1);
+    for (hipEvent_t x : e->kt.pool) (void)hipEventDestroy(x);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }
@@ -555,44 +596,89 @@ int engine_upload_stims(Engine *e, const std::vector<ScStim> &stims)
 // Steps in runs of at most `chunk` (< ring rows, so the statistics of a run still
 // find every row of the run): the step kernels, then one k_stats over the run,
 // then the step counter advance.  Full runs replay one captured hipGraph.
-static void launch_run(hipStream_t s, const ScInstance *single, const ScInstance *insts, int n_inst,
-                       const DevWork &w, int K)
+static hipError_t launch_run(hipStream_t s, const ScInstance *single, const ScInstance *insts, int n_inst,
+                              const DevWork &w, int K, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr)
 {
+    hipError_t err = hipSuccess;
+    if (ev0) err = hipEventRecord(ev0, s);
     for (int k = 0; k < K; ++k) {
         launch_step(s, single, insts, w, k);
         if (single) launch_plastic(s, *single, w, k);
     }
+    if (ev1 && err == hipSuccess) err = hipEventRecord(ev1, s);
     launch_stats(s, single, insts, n_inst, K, w.max_groups);
     launch_advance(s, insts, n_inst, K);
+    return err;
+}
+
+// One run of `chunk` steps captured as a graph (with the timer's events around the
+// step kernels when ev0/ev1 are given).
+static hipError_t capture_run(hipStream_t s, hipGraphExec_t *gexec, const ScInstance *single, const ScInstance *insts,
+                              int n_inst, const DevWork &w, int chunk, hipEvent_t ev0, hipEvent_t ev1)
+{
+    hipGraph_t g = nullptr;
+    hipError_t err = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (err != hipSuccess) return err;
+    hipError_t err_run = launch_run(s, single, insts, n_inst, w, chunk, ev0, ev1);
+    err = hipStreamEndCapture(s, &g);
+    if (err_run != hipSuccess) err = err_run;
+    if (err == hipSuccess) err = hipGraphInstantiate(gexec, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    if (err != hipSuccess) {
+        (void)hipGetLastError();
+        *gexec = nullptr;
+    }
+    return err;
 }
 
 static int run_steps(hipStream_t s, hipGraphExec_t *gexec, int *gexec_steps, const ScInstance *single,
-                     const ScInstance *insts, int n_inst, const DevWork &w, int n_steps, int use_graph, int chunk)
+                     const ScInstance *insts, int n_inst, const DevWork &w, int n_steps, int use_graph, int chunk,
+                     KTimer *kt = nullptr)
 {
     int done = 0;
+    const bool timed = kt && kt->on;
     if (use_graph && n_steps >= chunk) {
-        if (!*gexec || *gexec_steps != chunk) {
+        if (!*gexec || *gexec_steps != chunk || (kt && kt->graph_events != timed)) {
             if (*gexec) {
                 (void)hipGraphExecDestroy(*gexec);
                 *gexec = nullptr;
             }
-            hipGraph_t g;
-            HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            launch_run(s, single, insts, n_inst, w, chunk);
-            HIP_TRY(hipStreamEndCapture(s, &g));
-            hipError_t err = hipGraphInstantiate(gexec, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (err != hipSuccess) return fail(SCAFFOLD_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(err));
+            hipError_t err = hipErrorUnknown;
+            if (timed) {
+                if (!kt->g0) HIP_TRY(hipEventCreate(&kt->g0));
+                if (!kt->g1) HIP_TRY(hipEventCreate(&kt->g1));
+                err = capture_run(s, gexec, single, insts, n_inst, w, chunk, kt->g0, kt->g1);
+            }
+            if (kt) kt->graph_events = timed && err == hipSuccess;
+            if (err != hipSuccess) {  // untimed, or event nodes could not be captured
+                err = capture_run(s, gexec, single, insts, n_inst, w, chunk, nullptr, nullptr);
+                if (kt && timed) kt->graph_events = true;  // do not retry the capture every call
+                if (kt && timed) kt->g_missing = true;
+            }
+            if (err != hipSuccess) return fail(SCAFFOLD_ERR_HIP, "graph capture: %s", hipGetErrorString(err));
             *gexec_steps = chunk;
         }
         while (n_steps - done >= chunk) {
             HIP_TRY(hipGraphLaunch(*gexec, s));
             done += chunk;
+            if (timed && !kt->g_missing) kt->graph_pending = chunk;
         }
     }
     while (done < n_steps) {
         const int K = std::min(chunk, n_steps - done);
-        launch_run(s, single, insts, n_inst, w, K);
+        hipEvent_t a = nullptr, b = nullptr;
+        if (timed) {
+            while (kt->used + 2 > kt->pool.size()) {
+                hipEvent_t x;
+                HIP_TRY(hipEventCreate(&x));
+                kt->pool.push_back(x);
+            }
+            a = kt->pool[kt->used];
+            b = kt->pool[kt->used + 1];
+            kt->used += 2;
+            kt->piece_k.push_back(K);
+        }
+        HIP_TRY(launch_run(s, single, insts, n_inst, w, K, a, b));
         HIP_TRY(hipGetLastError());
         done += K;
     }
@@ -601,12 +687,64 @@ static int run_steps(hipStream_t s, hipGraphExec_t *gexec, int *gexec_steps, con
 
 static int chunk_for(int graph_steps, int R) { return std::max(1, std::min(graph_steps, R - 1)); }
 
+// Waits for the engine's stream, then adds what the step-kernel timer measured.
+static int engine_finish(Engine *e)
+{
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    KTimer &kt = e->kt;
+    float x = 0.f;
+    if (kt.graph_pending) {
+        HIP_TRY(hipEventElapsedTime(&x, kt.g0, kt.g1));
+        kt.ms += x;
+        kt.launches += kt.graph_pending;
+        kt.graph_pending = 0;
+    }
+    for (size_t i = 0; i < kt.piece_k.size(); ++i) {
+        HIP_TRY(hipEventElapsedTime(&x, kt.pool[2 * i], kt.pool[2 * i + 1]));
+        kt.ms += x;
+        kt.launches += kt.piece_k[i];
+    }
+    kt.piece_k.clear();
+    kt.used = 0;
+    return SCAFFOLD_OK;
+}
+
 int engine_run(Engine *e, int n_steps, int use_graph, int graph_steps)
 {
     int rc = run_steps(e->stream, &e->gexec, &e->gexec_steps, &e->hi, e->di, 1, e->w, n_steps, use_graph,
-                       chunk_for(graph_steps, e->hi.R));
+                       chunk_for(graph_steps, e->hi.R), &e->kt);
     if (rc) return rc;
-    return engine_sync(e);
+    return engine_finish(e);
+}
+
+int engine_run_output(Engine *e, int n_steps, int use_graph, int graph_steps, int64_t first_out_step,
+                      const uint8_t **train)
+{
+    int rc = run_steps(e->stream, &e->gexec, &e->gexec_steps, &e->hi, e->di, 1, e->w, n_steps, use_graph,
+                       chunk_for(graph_steps, e->hi.R), &e->kt);
+    if (rc) return rc;
+    if (train) {
+        const int R = e->hi.R;
+        const int row0 = (int)(((first_out_step % R) + R) % R);
+        const size_t nb = (size_t)e->T_out * (size_t)e->n_out;
+        launch_out_train(e->stream, e->hi.ring, e->hi.Wr, R, row0, e->T_out, e->out_bits, e->n_out, e->out_d);
+        HIP_TRY(hipGetLastError());
+        if (nb) HIP_TRY(hipMemcpyAsync(e->out_h, e->out_d, nb, hipMemcpyDeviceToHost, e->stream));
+        *train = e->out_h;
+    }
+    return engine_finish(e);
+}
+
+int engine_timer(Engine *e, int on, double *ms, int64_t *launches, int *graph_timed)
+{
+    if (on >= 0) {
+        e->kt.on = on != 0;
+        if (on) { e->kt.ms = 0.0; e->kt.launches = 0; }
+    }
+    if (ms) *ms = e->kt.ms;
+    if (launches) *launches = e->kt.launches;
+    if (graph_timed) *graph_timed = e->kt.g_missing ? 0 : 1;
+    return SCAFFOLD_OK;
 }
 
 int engine_time(Engine *e, int n_steps, int use_graph, int graph_steps, double *ms_total, double *ms_kernel_mean)

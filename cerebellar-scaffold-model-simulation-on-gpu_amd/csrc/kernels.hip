@@ -1277,6 +1277,22 @@ __global__ void __launch_bounds__(1024) k_compact(const uint64_t *__restrict__ r
     }
 }
 
+// Output spike train of T consecutive steps, the first in ring row row0: byte
+// (s, c) is the spike of output neuron c (device bit bits[c]) at step s, row-major
+// by step, as loop_n_steps hands it over (GPUScaffoldSimulation.cu:777).
+__global__ void __launch_bounds__(256) k_out_train(const uint64_t *__restrict__ ring, int Wr, int R, int row0, int T,
+                                                   const int32_t *__restrict__ bits, int n_out,
+                                                   uint8_t *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= T * n_out) return;
+    const int s = i / n_out, c = i - s * n_out;
+    int row = row0 + s;
+    if (row >= R) row -= R;
+    const int b = bits[c];
+    out[i] = (uint8_t)((ring[(int64_t)row * Wr + (b >> 6)] >> (b & 63)) & 1ull);
+}
+
 __global__ void k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, int64_t n)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -1352,6 +1368,13 @@ void launch_plastic(hipStream_t s, const ScInstance &I, const DevWork &w, int k)
         if (P.n_chunks <= 0) continue;
         k_plastic<<<(unsigned)((P.n_chunks + 3) / 4), 256, 0, s>>>(I, P, k);
     }
+}
+
+void launch_out_train(hipStream_t s, const uint64_t *ring, int Wr, int R, int row0, int T, const int32_t *bits,
+                      int n_out, uint8_t *out)
+{
+    const int n = T * n_out;
+    if (n > 0) k_out_train<<<(n + 255) / 256, 256, 0, s>>>(ring, Wr, R, row0, T, bits, n_out, out);
 }
 
 void launch_copy(hipStream_t s, const void *src, void *dst, int64_t n16, int grid)

@@ -40,6 +40,10 @@ void launch_init_params(hipStream_t s, const ScInstance *di, int n_threads, uint
 void launch_compact(hipStream_t s, const uint64_t *row, int n_groups, const ScGroup *groups, const ScPop *pops,
                     int32_t *out, int cap, int32_t *count);
 void launch_copy(hipStream_t s, const void *src, void *dst, int64_t n16, int grid);
+// Output spike train: T steps from ring row row0 (wrapping at R), one byte per
+// (step, output neuron), output neurons at device bits bits[0..n_out).
+void launch_out_train(hipStream_t s, const uint64_t *ring, int Wr, int R, int row0, int T, const int32_t *bits,
+                      int n_out, uint8_t *out);
 void launch_zero_g(hipStream_t s, ScState *st, int n);
 // The plasticity rules of one network after step k of a run (single instance).
 void launch_plastic(hipStream_t s, const ScInstance &I, const DevWork &w, int k);

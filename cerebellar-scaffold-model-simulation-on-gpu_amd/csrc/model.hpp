@@ -165,6 +165,8 @@ struct Plan {
     int max_delay = 0;
     std::vector<int> out_pops;            // OUTPUT populations
     int n_output = 0;
+    std::vector<int32_t> out_bits;        // device bit of each OUTPUT neuron, in output-train order
+    int T_out = 0;                        // steps per output-train hand-off (T_cpu_sim)
     std::vector<PlasticPlan> plastic;     // plasticity rules in table order
 };
 
@@ -178,6 +180,15 @@ int engine_zero_g(Engine *e);
 int engine_set_ctl(Engine *e, int64_t n, int64_t m);
 int engine_upload_stims(Engine *e, const std::vector<ScStim> &stims);
 int engine_run(Engine *e, int n_steps, int use_graph, int graph_steps);
+// engine_run, then (train != nullptr) the output spike train of the T_out steps
+// from first_out_step on, gathered on the device and copied with one D2H copy into
+// the engine's pinned buffer (*train), valid until the next call.
+int engine_run_output(Engine *e, int n_steps, int use_graph, int graph_steps, int64_t first_out_step,
+                      const uint8_t **train);
+// Step-kernel timer: on 1 starts (clears) it, 0 stops it, -1 leaves it; returns
+// the measured milliseconds, the step launches they cover and whether graph
+// replays are measured (1) or only direct launches (0).
+int engine_timer(Engine *e, int on, double *ms, int64_t *launches, int *graph_timed);
 int engine_sync(Engine *e);
 int engine_get_state(Engine *e, float *V, float *g, int32_t *refr);
 int engine_set_state(Engine *e, const float *V, const float *g, const int32_t *refr);
@@ -220,7 +231,6 @@ struct scaffold_env {
     int64_t cur_step = 0;   // trial-relative step of the next launch
     sc::Plan plan;
     sc::Engine *eng = nullptr;
-    std::vector<char> out_train;  // output_spike_train buffer
     std::vector<uint64_t> tmp_words;
     int in_batches = 0;           // batches referencing this environment's device state
 };

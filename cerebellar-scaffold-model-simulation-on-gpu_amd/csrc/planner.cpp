@@ -15,9 +15,10 @@ namespace sc {
 // ------------------------------------------------------------------ .dat loader
 // GetMaxConv + LoadConnectivityFile_ELL (init_params.cu:13-67,96-128): text lines
 // "pre post" (population-local ids).  Rows keep the file order of their entries
-// (h_col[post]++ at :43-48).  Unlike the reference, which reads out of bounds for
-// such lines, ids outside [0,preNum) x [0,postNum) and posts beyond the post
-// population are rejected.
+// (h_col[post]++ at :43-48).  Unlike the reference, which writes out of bounds
+// for such lines, ids outside [0,preNum) x [0,postNum) are rejected; pairs whose
+// post id lies in [post population size, postNum) are dropped, as the reference
+// allocates those ELL rows but never reads them.
 int load_dat(const char *file, int preNum, int postNum, int postPopNum, Projection &p)
 {
     FILE *fp = fopen(file, "r");
@@ -42,11 +43,7 @@ int load_dat(const char *file, int preNum, int postNum, int postPopNum, Projecti
             return fail(SCAFFOLD_ERR_FORMAT, "%s:%d: pair (%ld, %ld) outside [0,%d) x [0,%d)", file, lineno, a, b,
                         preNum, postNum);
         }
-        if (b >= postPopNum) {
-            fclose(fp);
-            return fail(SCAFFOLD_ERR_FORMAT, "%s:%d: post id %ld beyond the post population (%d)", file, lineno,
-                        b, postPopNum);
-        }
+        if (b >= postPopNum) continue;  // an ELL row the kernels never read (init_params.cu:743)
         pre.push_back((int32_t)a);
         post.push_back((int32_t)b);
     }
@@ -177,7 +174,12 @@ int plan_build(scaffold_env *env, Plan &P)
     P.Wr = (P.n_groups + 2) & ~1;  // at least one always-zero word; even, so rows are 16-byte aligned
     P.total_ref = ref;
     P.n_output = 0;
-    for (int p : P.out_pops) P.n_output += P.pops[p].num;
+    P.out_bits.clear();
+    for (int p : P.out_pops) {
+        P.n_output += P.pops[p].num;
+        for (int i = 0; i < P.pops[p].num; ++i) P.out_bits.push_back(P.pops[p].dev_base + i);
+    }
+    P.T_out = env->T_cpu_sim > 0 ? env->T_cpu_sim : 1;
     if ((int64_t)dev + SC_WAVE > (int64_t)SC_BIT_MASK)
         return fail(SCAFFOLD_ERR_CAPACITY, "more than 2^29 neurons in one instance");
     const uint32_t sentinel = (uint32_t)(P.Wr - 1) * SC_WAVE;  // a bit of the row's last, always-zero word
@@ -204,6 +206,22 @@ int plan_build(scaffold_env *env, Plan &P)
 
     // projections (index order fixes the accumulation order per post population)
     const int nproj = env->n_conn;
+    // Spike ring rows.  Step n writes row n and clears row n + 1 (its team-mode
+    // words are assembled by atomicOr during step n + 1), so every row a step reads
+    // -- n - delay, and for plasticity n - delay - s (s < window) and n - t_delay --
+    // must differ from both: R >= delay + 2 and R >= delay + window + 2.  The
+    // reference's 2 x T_print rows (GPUScaffoldSimulation.cu:361) are grown where
+    // they fall short (T_print = 1 with one-step delays gives 2 rows, which the
+    // reference can use because it never clears a row ahead).
+    for (int pi = 0; pi < nproj; ++pi)
+        if (env->projs[pi].set) P.R = std::max(P.R, env->projs[pi].delay() + 2);
+    for (const Plasticity &r : env->plast) {
+        if (r.window > SC_PLASTIC_WINDOW_MAX)
+            return fail(SCAFFOLD_ERR_CAPACITY, "plasticity window %d above %d steps", r.window, SC_PLASTIC_WINDOW_MAX);
+        if (r.window < 1) return fail(SCAFFOLD_ERR_INVALID_ARG, "plasticity window must be at least one step");
+        P.R = std::max(P.R, env->projs[r.target].delay() + r.window + 2);
+        if (r.rule == SC_STDP_TEACHER) P.R = std::max(P.R, env->projs[r.teacher].delay() + 2);
+    }
     P.projs.assign(nproj, ScProj{});
     P.parr.assign(nproj, HostProjArrays{});
     P.outdeg.assign(dev, 0);
@@ -266,7 +284,7 @@ int plan_build(scaffold_env *env, Plan &P)
         sp.W = src.weight > 0 ? src.weight : src.weight * (-1.f); // :733
         sp.pre_pop = src.pre;
         sp.post_pop = src.post;
-        if (sp.delay >= P.R) return fail(SCAFFOLD_ERR_INVALID_ARG, "delay beyond the spike ring");
+        if (sp.delay + 2 > P.R) return fail(SCAFFOLD_ERR_STATE, "delay %d beyond the spike ring", sp.delay);
         P.max_delay = std::max(P.max_delay, sp.delay);
         const int64_t nnz = (int64_t)src.col.size();
         P.nnz += nnz;
@@ -566,17 +584,14 @@ int plan_build(scaffold_env *env, Plan &P)
         X.rule = r;
         X.delay = sp.delay;
         // the A2_JI window reads rows n - delay - s, s < window; the row of step
-        // n + 1 is cleared during step n, so it must lie outside the window
-        if (r.window > SC_PLASTIC_WINDOW_MAX)
-            return fail(SCAFFOLD_ERR_CAPACITY, "plasticity window %d above %d steps", r.window, SC_PLASTIC_WINDOW_MAX);
-        if (r.window < 1 || sp.delay + r.window + 2 > P.R)
-            return fail(SCAFFOLD_ERR_INVALID_ARG, "plasticity on projection %d: delay %d + window %d needs a spike ring "
-                        "of at least %d rows (T_print %d gives %d)", r.target, sp.delay, r.window,
-                        sp.delay + r.window + 2, P.R / 2, P.R);
+        // n + 1 is cleared during step n, so it lies outside the window (the ring
+        // was grown above to delay + window + 2 rows)
+        if (sp.delay + r.window + 2 > P.R)
+            return fail(SCAFFOLD_ERR_STATE, "plasticity on projection %d: window beyond the spike ring", r.target);
         const Projection *tp = r.rule == SC_STDP_TEACHER ? &env->projs[r.teacher] : nullptr;
         if (tp) {
             X.t_delay = tp->delay();
-            if (X.t_delay >= P.R) return fail(SCAFFOLD_ERR_INVALID_ARG, "teacher delay beyond the spike ring");
+            if (X.t_delay + 2 > P.R) return fail(SCAFFOLD_ERR_STATE, "teacher delay beyond the spike ring");
         }
         // Row sums of the projection stay exact in f64 while every multiplier is a
         // multiple of 2^grid_exp and every row sums to less than 2^(grid_exp + 53):

@@ -319,6 +319,7 @@ int synth_describe(scaffold_env *env, const scaffold_synth_config *cfg)
     if (env->n_types < 7 || env->n_conn < 16 || env->n_inputs < 2)
         return fail(SCAFFOLD_ERR_CAPACITY, "the scaffold needs 7 populations, 16 projections, 2 inputs");
     if (!(cfg->scale > 0)) return fail(SCAFFOLD_ERR_INVALID_ARG, "scale must be positive");
+    if (cfg->connectome != 0 && cfg->connectome != 1) return fail(SCAFFOLD_ERR_INVALID_ARG, "connectome is 0 or 1");
     const double side = 400.0 * std::sqrt(cfg->scale);
     const double X = side, Z = side;
     int counts[8];
@@ -349,6 +350,26 @@ int synth_describe(scaffold_env *env, const scaffold_synth_config *cfg)
     const double band_area = 2 * half_band * 2 * std::min(half_len, X / 2);
     const double band_grc = dens_grc * std::min(band_area, X * Z);
     auto p_for = [&](double target) { return std::min(1.0, target / std::max(1.0, band_grc)); };
+    // connectome 1: the mean fan-ins of the reference's own connectome (the sizes
+    // of its data/dat_backup/latest connection files): 28.3k parallel fibres per
+    // Purkinje cell (the rule's p = 0.8 on a band widened to hold them), 1.6k per
+    // Golgi, 1.03k per stellate and 994 per basket cell, 26 Purkinje cells per
+    // DCN cell.  The band is cut at the sheet's z edges (it wraps along x only),
+    // so a band of half-width h covers 2h - h^2/Z of its length on average over
+    // the sheet; the widths and probabilities below are solved for the targets.
+    // connectome 0: the rule text (p = 0.8 in a 100 um band, 400 per Golgi and
+    // 1000 per stellate / basket cell, Purkinje -> DCN all to all).
+    const bool ref_stats = cfg->connectome == 1;
+    const double L2 = 2 * std::min(half_len, X / 2);  // band length
+    double pc_half_band = half_band;
+    if (ref_stats) {  // 0.8 * dens * L2 * (2h - h^2/Z) = 28,300
+        const double c = 28300.0 / (0.8 * std::max(1e-12, dens_grc) * L2);
+        pc_half_band = c >= Z ? Z : Z * (1.0 - std::sqrt(1.0 - c / Z));
+    }
+    const double edge = ref_stats ? 1.0 - std::min(half_band, Z) / (2 * Z) : 1.0;  // mean covered fraction
+    auto p_band = [&](double target) { return p_for(target / edge); };
+    const double goc_target = ref_stats ? 1600.0 : 400.0;
+    const double sc_target = ref_stats ? 1030.0 : 1000.0, bc_target = ref_stats ? 994.0 : 1000.0;
     const int J = cfg->weight_jitter;
     const int maxd = cfg->max_delay > 0 ? cfg->max_delay : 4;
     const double aa_r = 50.0;
@@ -377,15 +398,19 @@ int synth_describe(scaffold_env *env, const scaffold_synth_config *cfg)
         Edges E;
         switch (rp.type) {
         case 0: E = build_rows(counts[post], ps, J, [&](int j, Rng &r, std::vector<int32_t> &o) {
-                    band_pick(cell[GRC], g_grc, cell[post].x[j], cell[post].z[j], half_band, half_len, 0.8, r, o); });
+                    band_pick(cell[GRC], g_grc, cell[post].x[j], cell[post].z[j], pc_half_band, half_len, 0.8, r, o); });
             break;
         case 1: E = build_rows(counts[post], ps, J, [&](int j, Rng &r, std::vector<int32_t> &o) {
-                    band_pick(cell[GRC], g_grc, cell[post].x[j], cell[post].z[j], half_band, half_len, p_for(400), r, o); });
+                    band_pick(cell[GRC], g_grc, cell[post].x[j], cell[post].z[j], half_band, half_len,
+                              p_band(goc_target), r, o); });
             break;
         case 2:
-        case 3: E = build_rows(counts[post], ps, J, [&](int j, Rng &r, std::vector<int32_t> &o) {
-                    band_pick(cell[GRC], g_grc, cell[post].x[j], cell[post].z[j], half_band, half_len, p_for(1000), r, o); });
+        case 3: {
+            const double pb = p_band(rp.type == 2 ? sc_target : bc_target);
+            E = build_rows(counts[post], ps, J, [&](int j, Rng &r, std::vector<int32_t> &o) {
+                    band_pick(cell[GRC], g_grc, cell[post].x[j], cell[post].z[j], half_band, half_len, pb, r, o); });
             break;
+        }
         case 4: E = build_rows(counts[post], ps, J, aa_pick(post, 361)); break;
         case 5: E = build_rows(counts[post], ps, J, aa_pick(post, 250)); break;
         case 6: E = build_rows(counts[post], ps, J, [&](int j, Rng &r, std::vector<int32_t> &o) {
@@ -401,8 +426,18 @@ int synth_describe(scaffold_env *env, const scaffold_synth_config *cfg)
                     const int k = std::min(counts[GLOM], 375);
                     for (int t = 0; t < k; ++t) o.push_back((int32_t)r.below(counts[GLOM])); });
             break;
-        case 10: E = build_rows(counts[post], ps, J, [&](int, Rng &, std::vector<int32_t> &o) {
-                     for (int t = 0; t < counts[PC]; ++t) o.push_back(t); });
+        case 10: E = build_rows(counts[post], ps, J, [&](int, Rng &r, std::vector<int32_t> &o) {
+                     if (!ref_stats) {
+                         for (int t = 0; t < counts[PC]; ++t) o.push_back(t);
+                         return;
+                     }
+                     std::vector<int32_t> all(counts[PC]);  // 26 distinct Purkinje cells
+                     for (int t = 0; t < counts[PC]; ++t) all[t] = t;
+                     const int k = std::min(counts[PC], 26);
+                     for (int t = 0; t < k; ++t) {
+                         std::swap(all[t], all[t + (int)r.below(all.size() - t)]);
+                         o.push_back(all[t]);
+                     } });
             break;
         case 11: E = build_rows(counts[post], ps, J, [&](int j, Rng &r, std::vector<int32_t> &o) {
                      nearest_pick(cell[BC], g_bc, cell[post].x[j], cell[post].z[j], 20, 40, r, o); });
@@ -511,9 +546,11 @@ extern "C" int scaffold_synth_config_preset(int which, scaffold_synth_config *cf
         c.burst_rate_hz = 0.f;
         break;
     }
-    case 1: c.scale = 1.0; c.burst_n_mf = 50; break;
-    case 2: c.scale = 16.0; c.burst_n_mf = 800; break;
-    case 4: c.scale = 100.0; c.burst_n_mf = 0; c.burst_fraction = 0.05; break;
+    // the reference-size and tiled networks carry the reference connectome's
+    // fan-ins (4.23M synapses at reference size, 67M at 16x, 421M at 100x)
+    case 1: c.scale = 1.0; c.burst_n_mf = 50; c.connectome = 1; break;
+    case 2: c.scale = 16.0; c.burst_n_mf = 800; c.connectome = 1; break;
+    case 4: c.scale = 100.0; c.burst_n_mf = 0; c.burst_fraction = 0.05; c.connectome = 1; break;
     default: return sc::fail(SCAFFOLD_ERR_INVALID_ARG, "preset %d: use 0, 1, 2 or 4", which);
     }
     *cfg = c;

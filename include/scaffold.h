@@ -135,9 +135,11 @@ int scaffold_set_neuron_params(scaffold_env *env, int type, const char *name,
  * inhibitory one with |w| (GPUScaffoldSimulation.cu:732-733).  delay is in ms
  * (whole exchange steps, truncated as (int)delay, :734).  Every stored synapse
  * has weight multiplier 1.0 as in the reference (init_params.cu:46).
- * postNum may exceed the post population size (rows beyond it are ignored,
- * as in init_params.cu:743); pairs with pre >= preNum or post >= postNum are a
- * format error here (the reference reads out of bounds). */
+ * postNum may exceed the post population size: pairs whose post id lies in
+ * [post population size, postNum) are dropped, as the reference's kernels never
+ * read those ELL rows (init_params.cu:743 passes granule_cell_num for the Golgi
+ * cells); pairs with pre >= preNum or post >= postNum are a format error here
+ * (the reference writes out of bounds). */
 int scaffold_set_connectivity_params(scaffold_env *env, int type, const char *filename,
                                      int preNum, int postNum, int preType, int postType,
                                      float initial_weight, float delay,
@@ -230,17 +232,25 @@ int scaffold_get_option(scaffold_env *env, const char *key, int64_t *value);
  * layout, uploads it, draws the initial state twice (Initialization :381 and the
  * ResetNetworkEnv(env,0) at :689), so trial 0 uses each neuron's second Philox
  * output.  T_print is raised to the maximum delay; T_print must be a multiple of
- * T_cpu_sim (:674-684).  The spike ring has 2*T_print rows (:361). */
+ * T_cpu_sim (:674-684).  The spike ring has 2*T_print rows (:361), or more
+ * where a step reads further back: at least max_delay + 2 rows, and
+ * delay + time_window + 2 for a plasticity rule (each step clears the row of the
+ * step after it); scaffold_info.ring_rows gives the count. */
 int scaffold_initialize_network_env(scaffold_env *env, int T_print, int T_cpu_sim);
 
-/* loop_n_steps (GPUScaffoldSimulation.cu:693-791).  Advances n_steps exchange
- * steps (1 ms = 10 Euler sub-steps of 0.1 ms each) starting at trial-relative
- * step start_step.  n_steps must be a multiple of T_cpu_sim.  If
- * output_spike_train is not NULL it receives a library-owned host buffer of
- * T_cpu_sim x (number of OUTPUT neurons) bytes: the OUTPUT-population spikes of
- * the last T_cpu_sim steps of this call (row-major by step), valid until the
- * next call.  (The reference hands over the buffer of its lagging host thread,
- * :777.) */
+/* loop_n_steps (GPUScaffoldSimulation.cu:693-791).  Advances n_steps >= 0
+ * exchange steps (1 ms = 10 Euler sub-steps of 0.1 ms each) starting at
+ * trial-relative step start_step, and returns when they are done.  As in the
+ * reference (:768-783), the output spike train is handed over after every step
+ * that completes a window of T_cpu_sim steps (step count n with
+ * n % T_cpu_sim == 0): if output_spike_train is not NULL and this call completes
+ * at least one window, *output_spike_train receives a library-owned host buffer
+ * (pinned) of T_cpu_sim x (number of OUTPUT neurons) bytes holding the
+ * OUTPUT-population spikes of the last window completed (row-major by step),
+ * gathered on the GPU and copied with one transfer, valid until the next call; a
+ * call that completes no window leaves it unchanged.  (The reference hands over
+ * the buffer of its host thread that simulates the OUTPUT cells, :777; its rows
+ * are T_cpu_sim x all neurons wide.) */
 int scaffold_loop_n_steps(scaffold_env *env, int start_step, int n_steps, int trial,
                           const char **output_spike_train);
 
@@ -283,10 +293,16 @@ typedef struct {
     uint64_t stim_seed;
     int32_t  max_delay;          /* delays are clipped to [1, max_delay] steps           */
     int32_t  reference_delays;   /* 1: use the reference's delays (1..5 ms)              */
+    int32_t  connectome;         /* 0: the benchmark rules as written (parallel fibres at
+                                    p = 0.8 in a 100 um band, 400 per Golgi cell; 3.03M
+                                    synapses at reference size); 1: the mean fan-ins of
+                                    the reference's connectome files (28.3k parallel fibres
+                                    per Purkinje cell, 1.6k per Golgi cell; 4.07M)        */
 } scaffold_synth_config;
 
-/* Presets: which = 0 (small, ~9.9k cells), 1 (reference size, ~96.7k cells),
- * 2 (16x area), 4 (100x area).  Other fields get the benchmark defaults. */
+/* Presets: which = 0 (small, ~9.9k cells, connectome 0), 1 (reference size,
+ * ~96.7k cells), 2 (16x area), 4 (100x area) -- 1, 2 and 4 with connectome 1.
+ * Other fields get the benchmark defaults. */
 int scaffold_synth_config_preset(int which, scaffold_synth_config *cfg);
 int scaffold_describe_synthetic(scaffold_env *env, const scaffold_synth_config *cfg);
 
@@ -364,7 +380,10 @@ int scaffold_get_state(scaffold_env *env, float *V, float *g, int32_t *refr);
 int scaffold_set_state(scaffold_env *env, const float *V, const float *g, const int32_t *refr);
 
 /* Spike row of trial-relative step `step` (one byte per neuron, global ids);
- * the step must be among the last ring_rows steps. */
+ * the step must be among the last ring_rows - 1 steps (each step clears the ring
+ * row of the step after it, which holds the step ring_rows earlier).  The same
+ * window applies to scaffold_get_counts, scaffold_get_fired and the text
+ * writers. */
 int scaffold_get_spike_row(scaffold_env *env, int step, uint8_t *spikes);
 /* Per-population spike counts (n_populations) and synaptic events
  * (sum over fired neurons of their out-degree) of step `step`. */
@@ -375,16 +394,35 @@ int scaffold_get_totals(scaffold_env *env, uint64_t *spikes, uint64_t *events);
 /* Compacted fired-neuron list of step `step` (global ids, ascending), built on
  * the GPU with wavefront ballot/popcount and a prefix sum. */
 int scaffold_get_fired(scaffold_env *env, int step, int32_t *ids, int32_t cap, int32_t *n);
-/* Write the spikes of trial-relative steps [step0, step0+n) in the reference's
- * text format "%f\t%d\t%d\n" = (ms, population-local id, population type)
- * (GPUScaffoldSimulation.cu:81), appending to filename. */
+/* Append the spikes of trial-relative steps [step0, step0+n) to filename as the
+ * reference's printer thread print_spike_train writes them
+ * (GPUScaffoldSimulation.cu:76-84): "%f\t%d\t%d\n" = (step as ms,
+ * population-local id, population type), step by step, neurons in global-id
+ * order, for every population except the OUTPUT ones (which the reference
+ * simulates on the host and prints there).  The last column is the NeuronType
+ * the population was registered with: the printer's table is
+ * ReverseNeuronTypeID (:179-183, :429). */
 int scaffold_write_spike_text(scaffold_env *env, int step0, int n, const char *filename);
+/* The OUTPUT-population lines of the same steps as host_update_lif_ch writes
+ * them (Simulation_header.cu:456): (step as ms, global id, population type).
+ * The reference keeps them in a file of their own (spike_host*.dat) and appends
+ * it to the device file at the end of a trial (GPUScaffoldSimulation.cu:826). */
+int scaffold_write_output_spike_text(scaffold_env *env, int step0, int n, const char *filename);
 
 /* Timing helper for benchmarks: advance n_steps (continuing the trial) and
  * return the device time measured with HIP events on the engine stream, and the
  * mean duration of one step kernel measured around individual launches. */
 int scaffold_time_steps(scaffold_env *env, int n_steps, double *ms_total,
                         double *ms_step_kernel_mean);
+
+/* Step-kernel timer (no counterpart in the reference): on = 1 clears and starts
+ * it, 0 stops it, -1 only reads it.  While it runs, every run of steps
+ * (scaffold_loop_n_steps, scaffold_time_steps) records HIP events on the
+ * engine's stream around its step kernels -- inside a captured graph the graph
+ * records them, and of a call that replays the graph several times the last
+ * replay is measured.  *ms receives the measured time, *launches the step
+ * kernels it covers (either may be NULL). */
+int scaffold_step_timer(scaffold_env *env, int on, double *ms, int64_t *launches);
 
 /* Autotuner (no counterpart in the reference, whose layout is fixed at compile
  * time).  Before scaffold_initialize_network_env: builds private trial engines of

@@ -142,6 +142,42 @@ def test_dat_loader_errors(pkg, tmp_path, text, code):
         assert "bad.dat:1" in str(e.value)
 
 
+def test_rows_beyond_the_post_population_are_dropped(pkg, tmp_path):
+    """postNum larger than the post population (the reference passes
+    granule_cell_num for the Golgi cells, init_params.cu:743): pairs whose post id
+    lies beyond the population are dropped, as the reference never reads those
+    ELL rows; ids at or beyond postNum stay a format error."""
+    env, a, b = _two_pop_env(pkg)
+    f = tmp_path / "wide.dat"
+    f.write_text("1\t0\n2\t3\n4\t5\n0\t9\n")       # posts 5 and 9 lie beyond the 4 post cells
+    idx = env.set_connectivity_params(0, f, 5, 10, a, b, 0.25, 1.0)
+    rp, col, _ = env.projection_csr(idx)
+    assert list(rp) == [0, 1, 1, 1, 2] and list(col) == [1, 2]
+    idx = env.set_connectivity_csr(0, 5, 6, a, b, 0.25, 1.0, [0, 1, 1, 1, 2, 3, 4], [1, 2, 4, 0])
+    rp, col, _ = env.projection_csr(idx)
+    assert list(rp) == [0, 1, 1, 1, 2] and list(col) == [1, 2]
+    env2, a2, b2 = _two_pop_env(pkg)
+    f.write_text("1\t0\n0\t10\n")
+    with pytest.raises(pkg.ScaffoldError) as e:
+        env2.set_connectivity_params(0, f, 5, 10, a2, b2, 0.25, 1.0)
+    assert e.value.code == pkg.ERR_FORMAT
+
+
+def test_short_ring_with_one_step_delays_is_planned(pkg):
+    """T_print = T_cpu_sim = 1 gives the reference a ring of 2 rows; with
+    one-step delays a step would read the row the step before it clears, so the
+    planner grows the ring to delay + 2 rows instead of accepting the race.  On
+    a machine without a GPU the call gets past the planner and stops at the
+    device."""
+    env, a, b = _two_pop_env(pkg)
+    env.set_connectivity_csr(0, 5, 4, a, b, 0.1, 1.0, [0, 1, 2, 3, 4], [0, 1, 2, 3])
+    if HAS_GPU_NODE:
+        pytest.skip("a GPU driver node is present (the GPU test checks the ring)")
+    with pytest.raises(pkg.ScaffoldError) as e:
+        env.initialize(1, 1)
+    assert e.value.code in (pkg.ERR_NO_DEVICE, pkg.ERR_HIP)
+
+
 def test_projection_validation(pkg):
     env, a, b = _two_pop_env(pkg)
     with pytest.raises(pkg.ScaffoldError) as e:          # sub-step delays are not exchange steps
@@ -352,15 +388,15 @@ def test_plasticity_registration(pkg):
 
 def test_plasticity_plan_checks(pkg):
     """The planner's plasticity checks run at initialization, before any device
-    work: the spike ring must hold delay + window + 2 rows, the window fits the
-    one-byte counts, and a Teacher rule must not potentiate."""
+    work: the window fits the one-byte counts and a Teacher rule must not
+    potentiate.  A ring shorter than delay + window + 2 rows (T_print 20 gives
+    40 for a window of 50 behind a delay of 5) is grown, not rejected."""
     def env_with(**rule):
         env = new_env(pkg, 0, plasticities=1)
         idx = {env.projection(i).type: i for i in range(env.info().n_projections)}
         env.set_plasticity(pkg.STDP_TEACHER, idx[0], idx[11], **rule)
         return env
-    cases = [(dict(coefficients={pkg.A2_JI: -0.01}, time_window=50), 20, pkg.ERR_INVALID_ARG),  # ring 40 < 57
-             (dict(coefficients={pkg.A2_JI: -0.01}, time_window=128), 100, pkg.ERR_CAPACITY),
+    cases = [(dict(coefficients={pkg.A2_JI: -0.01}, time_window=128), 100, pkg.ERR_CAPACITY),
              (dict(coefficients={pkg.A1_J: 0.01}, time_window=50), 50, pkg.ERR_INVALID_ARG)]
     for rule, t_print, code in cases:
         env = env_with(**rule)
@@ -368,9 +404,10 @@ def test_plasticity_plan_checks(pkg):
             env.initialize(t_print, 10 if t_print == 20 else 25)
         assert e.value.code == code, (rule, t_print, str(e.value))
         env.finalize()
-    if not HAS_GPU_NODE:  # a valid rule passes the planner and stops at the device
-        env = env_with(coefficients={pkg.A1_J: 0.0, pkg.A2_JI: -0.01}, time_window=50)
-        with pytest.raises(pkg.ScaffoldError) as e:
-            env.initialize(50, 25)
-        assert e.value.code in (pkg.ERR_NO_DEVICE, pkg.ERR_HIP)
-        env.finalize()
+    if not HAS_GPU_NODE:  # valid rules pass the planner and stop at the device
+        for t_print in (50, 20):
+            env = env_with(coefficients={pkg.A1_J: 0.0, pkg.A2_JI: -0.01}, time_window=50)
+            with pytest.raises(pkg.ScaffoldError) as e:
+                env.initialize(t_print, 10 if t_print == 20 else 25)
+            assert e.value.code in (pkg.ERR_NO_DEVICE, pkg.ERR_HIP)
+            env.finalize()

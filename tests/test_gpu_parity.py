@@ -4,6 +4,8 @@ Integer and byte outputs (spike rasters, refractory counters, counts, events,
 fired lists) must match exactly; V and g are compared bit for bit as well,
 because the engine evaluates the reference's FP32 expressions in the same order
 without contraction (tolerance 0 ulp)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -79,17 +81,21 @@ def test_config0_bit_exact(pkg, orc, graph):
 
 @pytest.mark.parametrize("stage", [1, 0])
 def test_config1_bit_exact(pkg, orc, stage):
-    """The 96.7k scaffold, with blocks staging their spike windows in LDS (the
-    default) and with every gather reading the ring in memory."""
+    """The 96.7k scaffold (4.23M synapses, the reference connectome's fan-ins),
+    with blocks staging their spike windows in LDS (the default; 375 steps, past
+    the 100 Hz burst at 300-350 ms) and with every gather reading the ring in
+    memory (150 steps)."""
     env = new_env(pkg, 1)
     env.set_option("stage_spikes", stage)
     env.initialize(50, 25)
     info = env.info()
     assert (info.n_staged_blocks > 0) if stage else (info.n_staged_blocks == 0)
     ref = orc.OracleSim.from_env(env)
+    ref.set_threads(min(16, os.cpu_count() or 1))
     ref.initialize()
-    run_and_compare(env, ref, 0, 150, check_counts=True)
-    assert_same_state(env, ref, f"config1 150 steps, stage_spikes {stage}")
+    steps = 375 if stage else 150   # the staged (default) layout runs through the 300-350 ms burst
+    run_and_compare(env, ref, 0, steps, check_counts=True)
+    assert_same_state(env, ref, f"config1 {steps} steps, stage_spikes {stage}")
     env.finalize()
 
 
@@ -205,8 +211,10 @@ def test_rates_vs_fp64_oracle(pkg, orc, preset, precision, name):
 
 
 def test_subtile_rates_vs_fp64_oracle_100x(pkg, orc):
-    """The scaffold tiled 100x in area (9.67M cells, 326M synapses) on the GPU with
-    FP8-e4m3 multipliers and 16-bit column offsets, 1 s.  A 1x sub-tile -- the
+    """The scaffold tiled 100x in area (9.67M cells, 421M synapses) on the GPU with
+    FP8-e4m3 multipliers and 16-bit column offsets, 1 s.  First, 25 steps of the
+    whole network against the FP32 oracle on the same stored multipliers: rasters
+    every step and the final state, bit for bit.  Then a 1x sub-tile -- the
     400 x 400 um square at the centre of the sheet, inside the burst patch -- is
     re-simulated in the FP64 oracle with the unquantized multipliers, driven by the
     GPU's spikes of every neuron outside it.  Per-population firing rates of the
@@ -214,14 +222,17 @@ def test_subtile_rates_vs_fp64_oracle_100x(pkg, orc):
     the sub-tile); the first step at which the sub-tile's raster differs is
     reported."""
     import json
-    import os
     env = new_env(pkg, 4)
     env.set_option("weight_precision", pkg.WEIGHTS_FP8)
     env.set_option("index_bits", 16)
     env.initialize(50, 25)
     info = env.info()
-    assert info.synapses_16bit == info.total_synapses
+    assert info.synapses_16bit == info.total_synapses and info.total_synapses > 4e8
     pops = [env.population(i) for i in range(info.n_populations)]
+    exact = orc.OracleSim.from_env(env)   # FP32 reference semantics, stored multipliers
+    exact.set_threads(min(16, os.cpu_count() or 1))
+    exact.initialize()
+    print("100x: bit-exact phase, 25 steps", flush=True)
     desc = new_env(pkg, 4)  # same network with the generated (unquantized) multipliers
     ref = orc.OracleSim.from_env(desc, precision=1, ring_rows=100)
     desc.finalize()
@@ -243,8 +254,12 @@ def test_subtile_rates_vs_fp64_oracle_100x(pkg, orc):
     diverged = None
     for s0 in range(0, FP64_STEPS, 25):
         env.loop_n_steps(s0, 25, 0, want_output=False)
+        if exact is not None:
+            exact.run(s0, 25)
         for s in range(s0, s0 + 25):
             row = env.spike_row(s)
+            if exact is not None:
+                assert np.array_equal(row, exact.spike_row(s)), f"100x: raster differs from the FP32 oracle at step {s}"
             ref.step_driven(s, row)
             mine = ref.spike_row(s)
             for i in ids:
@@ -252,6 +267,10 @@ def test_subtile_rates_vs_fp64_oracle_100x(pkg, orc):
                 orc_n[i] += int(mine[ids[i]].sum())
             if diverged is None and not np.array_equal(row[on], mine[on]):
                 diverged = s
+        if exact is not None:
+            assert_same_state(env, exact, "100x, 25 steps")
+            exact = None   # frees the FP32 oracle's copy of the network
+            print("100x: 25 steps bit-exact; FP64 sub-tile phase", flush=True)
     env.finalize()
     report = {"network": "scaffold 100x, fp8, 16-bit offsets", "subtile_neurons": {pops[i].name.decode(): int(ids[i].size)
                                                                                  for i in ids},
@@ -501,30 +520,183 @@ def test_trials_reset_and_freq(pkg, orc):
     env.finalize()
 
 
+def reference_text_lines(env, ref, steps, output):
+    """The spike lines the reference writes for `steps`, rendered here from the
+    oracle's raster: print_spike_train (GPUScaffoldSimulation.cu:76-84) prints
+    (float step, population-local id, ReverseNeuronTypeID[internal index]) for the
+    device populations, host_update_lif_ch (Simulation_header.cu:456) prints
+    (float step, global id, ReverseNeuronTypeID[...]) for the OUTPUT population;
+    ReverseNeuronTypeID inverts NeuronTypeID (:179-183), so the last column is the
+    NeuronType enum the population was registered with (init_params.cu:377-511)."""
+    info = env.info()
+    pops = [env.population(i) for i in range(info.n_populations)]
+    reverse = {i: p.type for i, p in enumerate(pops)}     # internal index -> NeuronType
+    lines = []
+    for s in steps:
+        row = ref.spike_row(s)
+        for i, p in enumerate(pops):
+            if (p.device_type == 2) != output:
+                continue
+            for j in np.nonzero(row[p.base_id:p.base_id + p.num])[0]:
+                lines.append("%f\t%d\t%d" % (float(np.float32(s)), p.base_id + j if output else j, reverse[i]))
+    return lines
+
+
 def test_output_train_fired_and_text(pkg, orc, tmp_path):
+    """loop_n_steps with any n_steps: the output train is handed over after every
+    step that completes a T_cpu_sim window (GPUScaffoldSimulation.cu:768-783) and
+    holds the last completed window; calls that complete none return no train.
+    The fired list, the oldest queryable ring row and both text files (device
+    populations, OUTPUT population) agree with the oracle."""
     env = new_env(pkg, 0)
     env.initialize(50, 25)
     ref = orc.OracleSim.from_env(env)
     ref.initialize()
     info = env.info()
+    R = info.ring_rows
+    assert info.T_cpu_sim == 25 and R == 100
     dcn = [i for i in range(info.n_populations) if env.population(i).device_type == pkg.OUTPUT]
     base = env.population(dcn[0]).base_id
     num = env.population(dcn[0]).num
-    for t in range(0, 100, 25):
-        out = env.loop_n_steps(t, 25, 0)
-        ref.run(t, 25)
-        want = np.stack([ref.spike_row(s)[base:base + num] for s in range(t, t + 25)])
-        assert np.array_equal(out, want)
-    for s in range(50, 100):
+    assert info.n_output_neurons == num
+    t = 0
+    for n in (7, 11, 7, 30, 3, 1, 48, 25, 50, 5, 0, 13):
+        out = env.loop_n_steps(t, n, 0)
+        ref.run(t, n)
+        end = t + n
+        last = end - end % 25
+        if last > t:
+            assert out is not None, f"call ({t}, {n}) completes the window ending at {last}"
+            want = np.stack([ref.spike_row(s)[base:base + num] for s in range(last - 25, last)])
+            assert np.array_equal(out, want), f"output train of the window ending at {last}"
+        else:
+            assert out is None, f"call ({t}, {n}) completes no window"
+        t = end
+    cur = env.info().current_step
+    assert cur == t
+    for s in range(cur - R + 1, cur):
+        assert np.array_equal(env.spike_row(s), ref.spike_row(s)), f"step {s}"
+    for s in (cur - R, cur):   # the row of cur - R was cleared during the last step
+        with pytest.raises(pkg.ScaffoldError) as e:
+            env.spike_row(s)
+        assert e.value.code == pkg.ERR_INVALID_ARG
+    oldest = cur - R + 1
+    ids, n = env.fired(oldest)
+    want = np.nonzero(ref.spike_row(oldest))[0]
+    assert n == want.size and np.array_equal(ids, want)
+    for s in range(cur - 50, cur):
         ids, n = env.fired(s)
         want = np.nonzero(ref.spike_row(s))[0]
         assert n == want.size and np.array_equal(ids, want), f"fired list at step {s}"
-    path = tmp_path / "spikes.txt"
-    env.write_spike_text(75, 25, path)
-    lines = open(path).read().split("\n")[:-1]
-    nz = sum(int(ref.spike_row(s).sum()) for s in range(75, 100))
-    assert len(lines) == nz
+    steps = range(oldest, cur)
+    dev_file, host_file = tmp_path / "spike_dev.txt", tmp_path / "spike_host.txt"
+    env.write_spike_text(oldest, len(steps), dev_file)
+    env.write_output_spike_text(oldest, len(steps), host_file)
+    got = dev_file.read_text().splitlines()
+    want = reference_text_lines(env, ref, steps, output=False)
+    assert len(want) > 1000 and got == want
+    got = host_file.read_text().splitlines()
+    want = reference_text_lines(env, ref, steps, output=True)
+    assert got == want
+    assert all(ln.split("\t")[2] == str(env.population(dcn[0]).type) for ln in got)   # dcn_cell = 6
+    with pytest.raises(pkg.ScaffoldError):
+        env.write_spike_text(cur - R, 2, dev_file)
     env.finalize()
+
+
+def test_short_ring_one_step_delays(pkg, orc):
+    """T_print = T_cpu_sim = 1 and every delay one step: the reference's ring would
+    have 2 rows; the planner grows it to 3 (a step never reads the row the step
+    before it cleared).  One-step calls each hand over a one-step output train;
+    rasters and state agree with the oracle over 300 steps."""
+    env = new_env(pkg, 0, max_delay=1)
+    env.initialize(1, 1)
+    info = env.info()
+    assert info.max_delay == 1 and info.T_print == 1 and info.ring_rows == 3
+    ref = orc.OracleSim.from_env(env)
+    ref.initialize()
+    dcn = [i for i in range(info.n_populations) if env.population(i).device_type == pkg.OUTPUT][0]
+    base, num = env.population(dcn).base_id, env.population(dcn).num
+    for s in range(300):
+        out = env.loop_n_steps(s, 1, 0)
+        ref.step(s)
+        assert np.array_equal(env.spike_row(s), ref.spike_row(s)), f"step {s}"
+        assert out.shape == (1, num) and np.array_equal(out[0], ref.spike_row(s)[base:base + num])
+    assert_same_state(env, ref, "ring of 3 rows")
+    env.finalize()
+
+
+def test_config3_batch_real_scales(pkg, orc):
+    """The batched form at the instance sizes of the 64-instance workload: one
+    instance each at 0.25x, 0.5x, 1x, 2x and 4x of the 96.7k scaffold (33M
+    synapses in all), each with its own seeds and mossy rate (1 to 60 Hz), BF16
+    multipliers, 16- and 32-bit columns mixed, advanced together by the batched
+    step kernel.  Every instance matches its own FP32 oracle run bit for bit for
+    75 steps; then the batch runs on to 1 s and the 60 Hz instance's firing rates
+    agree with the FP64 oracle (unquantized multipliers) within 1 % per population
+    with at least 1000 spikes."""
+    specs = [(0.25, 17.0), (0.5, 4.0), (1.0, 60.0), (2.0, 33.0), (4.0, 1.0)]   # (scale, mossy rate Hz)
+    threads = min(16, os.cpu_count() or 1)
+
+    def cfg_of(i, scale, rate):
+        return dict(scale=scale, seed=1000 + i, stim_seed=5000 + i, mf_rate_hz=rate,
+                    burst_n_mf=max(1, int(round(50 * scale))))
+    envs = []
+    for i, (scale, rate) in enumerate(specs):
+        e = new_env(pkg, 1, **cfg_of(i, scale, rate))
+        e.set_option("neuron_seed", 77 + i)
+        e.set_option("weight_precision", pkg.WEIGHTS_BF16)
+        e.set_option("index_bits", 16 if i % 2 == 0 else 32)
+        e.initialize(50, 25)
+        envs.append(e)
+    assert sum(e.info().total_synapses for e in envs) > 3e7
+    refs = []
+    for e in envs:
+        r = orc.OracleSim.from_env(e)
+        r.set_threads(threads)
+        r.initialize()
+        refs.append(r)
+    hot = 2   # the 1x instance at 60 Hz
+    desc = new_env(pkg, 1, **cfg_of(hot, *specs[hot]))
+    f64 = orc.OracleSim.from_env(desc, precision=1, ring_rows=100, neuron_seed=77 + hot)
+    desc.finalize()
+    f64.set_threads(threads)
+    f64.initialize()
+    pops = [envs[hot].population(i) for i in range(envs[hot].info().n_populations)]
+    f64_n = np.zeros(len(pops), np.int64)
+    b = pkg.Batch(envs)
+    for t in range(0, FP64_STEPS, 25):
+        b.run(25)
+        if t < 75:
+            for i, (e, r) in enumerate(zip(envs, refs)):
+                r.run(t, 25)
+                for s_ in range(t, t + 25):
+                    a, w = e.spike_row(s_), r.spike_row(s_)
+                    if not np.array_equal(a, w):
+                        d = np.nonzero(a != w)[0]
+                        raise AssertionError(f"instance {i} (scale {specs[i][0]}), step {s_}: ids {d[:10]}")
+            if t == 50:
+                for i, (e, r) in enumerate(zip(envs, refs)):
+                    assert_same_state(e, r, f"instance {i} (scale {specs[i][0]}) after 75 steps")
+                refs = []
+                print("config 3: 75 steps bit-exact for every instance", flush=True)
+        for s_ in range(t, t + 25):
+            f64.step(s_)
+            row = f64.spike_row(s_)
+            for i, p in enumerate(pops):
+                f64_n[i] += int(row[p.base_id:p.base_id + p.num].sum())
+    gpu_n = envs[hot].totals()[0].astype(np.int64)
+    b.destroy()
+    for e in envs:
+        e.finalize()
+    worst, report = 0.0, {}
+    for i, p in enumerate(pops):
+        err = abs(int(gpu_n[i]) - int(f64_n[i])) / max(int(f64_n[i]), 1) * 100
+        report[p.name.decode()] = (int(gpu_n[i]), int(f64_n[i]), round(err, 3))
+        if f64_n[i] >= 1000:
+            worst = max(worst, err)
+    print("60 Hz instance, spikes gpu / fp64 / error %:", report)
+    assert worst <= 1.0, report
 
 
 @pytest.mark.parametrize("index_bits", [32, 16])
