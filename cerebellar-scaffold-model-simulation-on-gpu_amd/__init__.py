@@ -21,7 +21,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libscaffold_mi355x.so")
+# SCAFFOLD_LIB selects another build of the same library (e.g. the host-sanitizer
+# build of `make asan`)
+LIB_PATH = os.environ.get("SCAFFOLD_LIB") or os.path.join(_HERE, "lib", "libscaffold_mi355x.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "scaffold.h")
 
 OK, ERR_INVALID_ARG, ERR_STATE, ERR_IO, ERR_FORMAT, ERR_UNSUPPORTED, ERR_HIP, ERR_NO_DEVICE, \
@@ -165,6 +167,7 @@ def load_library(path=None):
         "scaffold_batch_time": (ctypes.c_int, [vp, ctypes.c_int, P(f64)]),
         "scaffold_batch_destroy": (ctypes.c_int, [vp]),
         "scaffold_copy_probe": (ctypes.c_int, [i64, ctypes.c_int, P(f64)]),
+        "scaffold_encode_multipliers": (ctypes.c_int, [ctypes.c_int, P(f32), i64, P(f32), P(f32), P(f32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -203,6 +206,17 @@ def synth_preset(which):
     c = SynthConfig()
     _check(load_library().scaffold_synth_config_preset(which, ctypes.byref(c)))
     return c
+
+
+def encode_multipliers(precision, values):
+    """(stored multipliers dequantized, FP8 scale, row-sum factor) of one
+    projection's multipliers as the engine stores them with `precision`."""
+    v = np.ascontiguousarray(values, dtype=np.float32)
+    out = np.zeros_like(v)
+    sc, ss = ctypes.c_float(), ctypes.c_float()
+    _check(load_library().scaffold_encode_multipliers(precision, _ptr(v, ctypes.c_float), v.size,
+                                                      _ptr(out, ctypes.c_float), ctypes.byref(sc), ctypes.byref(ss)))
+    return out, sc.value, ss.value
 
 
 def copy_probe(nbytes, iters=10):

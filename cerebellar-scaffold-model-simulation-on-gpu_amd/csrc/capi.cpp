@@ -246,7 +246,7 @@ int scaffold_set_connectivity_params(scaffold_env *env, int type, const char *fi
                     env->pops[preType].num);
     rc = load_dat(filename, preNum, postNum, env->pops[postType].num, p);
     if (rc) return rc;
-    int target;
+    int target = -1;
     rc = assign_projection(env, postType, &target);
     if (rc) return rc;
     p.set = true;
@@ -292,7 +292,7 @@ int scaffold_set_connectivity_csr(scaffold_env *env, int type, int preNum, int p
         int len = (int)(p.row_ptr[j + 1] - p.row_ptr[j]);
         if (len > p.max_conv) p.max_conv = len;
     }
-    int target;
+    int target = -1;
     rc = assign_projection(env, postType, &target);
     if (rc) return rc;
     p.set = true;
@@ -321,7 +321,10 @@ int scaffold_set_plasticity(scaffold_env *env, int rule, int target, int teacher
     }
     if (present && !coefficients) return fail(SCAFFOLD_ERR_INVALID_ARG, "null coefficients");
     if (present >> 5) return fail(SCAFFOLD_ERR_INVALID_ARG, "present: bits 0-4 only");
-    if (time_window < 1) return fail(SCAFFOLD_ERR_INVALID_ARG, "time_window must be >= 1");
+    if (time_window < 0) return fail(SCAFFOLD_ERR_INVALID_ARG, "time_window must be >= 0");
+    // 0: the window the reference's kernels use whatever the table says,
+    // s_upper_bound = 50 (SynapticPlasticity.cu:45,86)
+    if (time_window == 0) time_window = 50;
     Plasticity r;
     r.rule = rule;
     r.target = target;
@@ -830,7 +833,7 @@ int scaffold_get_spike_row(scaffold_env *env, int step, uint8_t *spikes)
     int rc = check_ready(env);
     if (rc) return rc;
     if (!spikes) return fail(SCAFFOLD_ERR_INVALID_ARG, "null output");
-    int row;
+    int row = 0;
     rc = step_row(env, step, &row);
     if (rc) return rc;
     const Plan &P = env->plan;
@@ -849,7 +852,7 @@ int scaffold_get_counts(scaffold_env *env, int step, int32_t *counts, uint64_t *
     clear_error();
     int rc = check_ready(env);
     if (rc) return rc;
-    int row;
+    int row = 0;
     rc = step_row(env, step, &row);
     if (rc) return rc;
     unsigned long long ev = 0;
@@ -878,7 +881,7 @@ int scaffold_get_fired(scaffold_env *env, int step, int32_t *ids, int32_t cap, i
     int rc = check_ready(env);
     if (rc) return rc;
     if (!n || (cap > 0 && !ids)) return fail(SCAFFOLD_ERR_INVALID_ARG, "null output");
-    int row;
+    int row = 0;
     rc = step_row(env, step, &row);
     if (rc) return rc;
     return engine_get_fired(env->eng, row, ids, cap, n);
@@ -901,7 +904,7 @@ static int write_spike_lines(scaffold_env *env, int step0, int n, const char *fi
     if (rc) return rc;
     if (!filename || n < 0) return fail(SCAFFOLD_ERR_INVALID_ARG, "bad arguments");
     for (int s = step0; s < step0 + n; ++s) {  // every step queryable before anything is written
-        int row;
+        int row = 0;
         if ((rc = step_row(env, s, &row))) return rc;
     }
     FILE *fp = fopen(filename, "a");
@@ -909,7 +912,7 @@ static int write_spike_lines(scaffold_env *env, int step0, int n, const char *fi
     const Plan &P = env->plan;
     env->tmp_words.resize(P.Wr);
     for (int s = step0; s < step0 + n; ++s) {
-        int row;
+        int row = 0;
         rc = step_row(env, s, &row);
         if (!rc) rc = engine_get_ring_row(env->eng, row, env->tmp_words.data());
         if (rc) { fclose(fp); return rc; }
@@ -946,7 +949,7 @@ int scaffold_step_timer(scaffold_env *env, int on, double *ms, int64_t *launches
     int rc = check_ready(env);
     if (rc) return rc;
     if (on < -1 || on > 1) return fail(SCAFFOLD_ERR_INVALID_ARG, "on is -1, 0 or 1");
-    return engine_timer(env->eng, on, ms, launches, nullptr);
+    return engine_timer(env->eng, on, ms, launches);
 }
 
 int scaffold_time_steps(scaffold_env *env, int n_steps, double *ms_total, double *ms_step_kernel_mean)
@@ -1044,6 +1047,24 @@ int scaffold_autotune(scaffold_env *env, int n_steps, scaffold_tune_result *out)
     env->opt.index_bits = r.index_bits;
     env->opt.stage_spikes = r.stage_spikes;
     *out = r;
+    return SCAFFOLD_OK;
+}
+
+int scaffold_encode_multipliers(int precision, const float *in, int64_t n, float *out, float *scale,
+                                float *sum_scale)
+{
+    clear_error();
+    if (precision < SCAFFOLD_WEIGHTS_F32 || precision > SCAFFOLD_WEIGHTS_FP8)
+        return fail(SCAFFOLD_ERR_INVALID_ARG, "precision %d: use SCAFFOLD_WEIGHTS_*", precision);
+    if (n < 0 || (n > 0 && (!in || !out))) return fail(SCAFFOLD_ERR_INVALID_ARG, "bad arguments");
+    for (int64_t e = 0; e < n; ++e)
+        if (!std::isfinite(in[e])) return fail(SCAFFOLD_ERR_INVALID_ARG, "multiplier %lld is not finite", (long long)e);
+    if (n > 0 && out != in) memmove(out, in, sizeof(float) * (size_t)n);
+    const int wp = precision == SCAFFOLD_WEIGHTS_BF16 ? SC_W_BF16 : precision == SCAFFOLD_WEIGHTS_FP8 ? SC_W_FP8 : SC_W_F32;
+    float sc = 1.f, ss = 1.f;
+    encode_values(wp, out, n, &sc, &ss);
+    if (scale) *scale = sc;
+    if (sum_scale) *sum_scale = ss;
     return SCAFFOLD_OK;
 }
 

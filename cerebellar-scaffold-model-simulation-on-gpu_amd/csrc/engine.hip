@@ -22,19 +22,15 @@ struct TaskWin {
     int64_t segs = 0;
 };
 
-// Optional timing of the step kernels inside the runs: one event pair around the
-// step launches of every run piece (direct launches), and inside a captured graph
-// one pair recorded by the graph itself, read after each call (so of a call that
-// replays the graph several times, the last replay is measured).
+// Optional timing of the step kernels inside the runs: a pair of events recorded
+// on the engine's stream around the step kernels of every run piece (direct
+// launches, or one replay of a graph that holds the step kernels only, the
+// statistics and counter advance of the piece following it as direct launches).
 struct KTimer {
     bool on = false;
-    hipEvent_t g0 = nullptr, g1 = nullptr;   // captured into the graph
-    bool graph_events = false;               // the current graph was captured for a timed run
-    bool g_missing = false;                  // ... but without the events (capture refused them)
-    int graph_pending = 0;                   // step launches of the last replay (0: none)
-    std::vector<hipEvent_t> pool;            // direct pieces: pairs
+    std::vector<hipEvent_t> pool;  // pairs
     size_t used = 0;
-    std::vector<int> piece_k;
+    std::vector<int> piece_k;      // step launches of each measured piece
     double ms = 0.0;
     int64_t launches = 0;
 };
@@ -60,6 +56,8 @@ struct Engine {
     int Ndev = 0, total_ref = 0;
     hipGraphExec_t gexec = nullptr;
     int gexec_steps = 0;
+    hipGraphExec_t gexec_k = nullptr;  // timed runs: the step kernels of a run only
+    int gexec_k_steps = 0;
     KTimer kt;
     // output spike train: device bits of the OUTPUT neurons, the T_out x n_out
     // byte buffer on the device and its pinned host copy
@@ -523,10 +521,9 @@ void engine_destroy(Engine *e)
     if (!e) return;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    if (e->gexec_k) (void)hipGraphExecDestroy(e->gexec_k);
     for (void *p : e->allocs) (void)hipFree(p);
     if (e->out_h) (void)hipHostFree(e->out_h);
-    if (e->kt.g0) (void)hipEventDestroy(e->kt.g0);
-    if (e->kt.g1) (void)hipEventDestroy(e->kt.g1);
     for (hipEvent_t x : e->kt.pool) (void)hipEventDestroy(x);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
@@ -596,89 +593,105 @@ int engine_upload_stims(Engine *e, const std::vector<ScStim> &stims)
 // Steps in runs of at most `chunk` (< ring rows, so the statistics of a run still
 // find every row of the run): the step kernels, then one k_stats over the run,
 // then the step counter advance.  Full runs replay one captured hipGraph.
-static hipError_t launch_run(hipStream_t s, const ScInstance *single, const ScInstance *insts, int n_inst,
-                              const DevWork &w, int K, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr)
+// The step kernels of a run of K steps (and the plasticity kernels after each).
+static void launch_step_kernels(hipStream_t s, const ScInstance *single, const ScInstance *insts, const DevWork &w,
+                                int K)
 {
-    hipError_t err = hipSuccess;
-    if (ev0) err = hipEventRecord(ev0, s);
     for (int k = 0; k < K; ++k) {
         launch_step(s, single, insts, w, k);
         if (single) launch_plastic(s, *single, w, k);
     }
-    if (ev1 && err == hipSuccess) err = hipEventRecord(ev1, s);
-    launch_stats(s, single, insts, n_inst, K, w.max_groups);
-    launch_advance(s, insts, n_inst, K);
-    return err;
 }
 
-// One run of `chunk` steps captured as a graph (with the timer's events around the
-// step kernels when ev0/ev1 are given).
+// The statistics of a run and the advance of the step counters.
+static void launch_run_tail(hipStream_t s, const ScInstance *single, const ScInstance *insts, int n_inst,
+                            const DevWork &w, int K)
+{
+    launch_stats(s, single, insts, n_inst, K, w.max_groups);
+    launch_advance(s, insts, n_inst, K);
+}
+
+// Captures `chunk` steps as a graph: the step kernels, and with `tail` the
+// statistics and the counter advance as well.
 static hipError_t capture_run(hipStream_t s, hipGraphExec_t *gexec, const ScInstance *single, const ScInstance *insts,
-                              int n_inst, const DevWork &w, int chunk, hipEvent_t ev0, hipEvent_t ev1)
+                              int n_inst, const DevWork &w, int chunk, bool tail)
 {
     hipGraph_t g = nullptr;
     hipError_t err = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
     if (err != hipSuccess) return err;
-    hipError_t err_run = launch_run(s, single, insts, n_inst, w, chunk, ev0, ev1);
+    launch_step_kernels(s, single, insts, w, chunk);
+    if (tail) launch_run_tail(s, single, insts, n_inst, w, chunk);
     err = hipStreamEndCapture(s, &g);
-    if (err_run != hipSuccess) err = err_run;
     if (err == hipSuccess) err = hipGraphInstantiate(gexec, g, nullptr, nullptr, 0);
     if (g) (void)hipGraphDestroy(g);
-    if (err != hipSuccess) {
-        (void)hipGetLastError();
-        *gexec = nullptr;
-    }
+    if (err != hipSuccess) *gexec = nullptr;
     return err;
 }
 
+// The next pair of timer events (created on first use).
+static int timer_pair(KTimer *kt, int K, hipEvent_t *a, hipEvent_t *b)
+{
+    while (kt->used + 2 > kt->pool.size()) {
+        hipEvent_t x;
+        HIP_TRY(hipEventCreate(&x));
+        kt->pool.push_back(x);
+    }
+    *a = kt->pool[kt->used];
+    *b = kt->pool[kt->used + 1];
+    kt->used += 2;
+    kt->piece_k.push_back(K);
+    return SCAFFOLD_OK;
+}
+
+// Graph replays of `chunk` steps, then direct launches for the rest.  With the
+// timer on, each replay is of the graph of the step kernels alone, between two
+// events, its statistics and counter advance launched after it; otherwise of the
+// graph of the whole run.
 static int run_steps(hipStream_t s, hipGraphExec_t *gexec, int *gexec_steps, const ScInstance *single,
                      const ScInstance *insts, int n_inst, const DevWork &w, int n_steps, int use_graph, int chunk,
-                     KTimer *kt = nullptr)
+                     KTimer *kt = nullptr, hipGraphExec_t *gexec_k = nullptr, int *gexec_k_steps = nullptr)
 {
     int done = 0;
-    const bool timed = kt && kt->on;
+    const bool timed = kt && kt->on && gexec_k;
+    hipGraphExec_t *ge = timed ? gexec_k : gexec;
+    int *ge_steps = timed ? gexec_k_steps : gexec_steps;
     if (use_graph && n_steps >= chunk) {
-        if (!*gexec || *gexec_steps != chunk || (kt && kt->graph_events != timed)) {
-            if (*gexec) {
-                (void)hipGraphExecDestroy(*gexec);
-                *gexec = nullptr;
+        if (!*ge || *ge_steps != chunk) {
+            if (*ge) {
+                (void)hipGraphExecDestroy(*ge);
+                *ge = nullptr;
             }
-            hipError_t err = hipErrorUnknown;
-            if (timed) {
-                if (!kt->g0) HIP_TRY(hipEventCreate(&kt->g0));
-                if (!kt->g1) HIP_TRY(hipEventCreate(&kt->g1));
-                err = capture_run(s, gexec, single, insts, n_inst, w, chunk, kt->g0, kt->g1);
-            }
-            if (kt) kt->graph_events = timed && err == hipSuccess;
-            if (err != hipSuccess) {  // untimed, or event nodes could not be captured
-                err = capture_run(s, gexec, single, insts, n_inst, w, chunk, nullptr, nullptr);
-                if (kt && timed) kt->graph_events = true;  // do not retry the capture every call
-                if (kt && timed) kt->g_missing = true;
-            }
+            hipError_t err = capture_run(s, ge, single, insts, n_inst, w, chunk, !timed);
             if (err != hipSuccess) return fail(SCAFFOLD_ERR_HIP, "graph capture: %s", hipGetErrorString(err));
-            *gexec_steps = chunk;
+            *ge_steps = chunk;
         }
         while (n_steps - done >= chunk) {
-            HIP_TRY(hipGraphLaunch(*gexec, s));
+            hipEvent_t a = nullptr, b = nullptr;
+            if (timed) {
+                int rc = timer_pair(kt, chunk, &a, &b);
+                if (rc) return rc;
+                HIP_TRY(hipEventRecord(a, s));
+            }
+            HIP_TRY(hipGraphLaunch(*ge, s));
+            if (timed) {
+                HIP_TRY(hipEventRecord(b, s));
+                launch_run_tail(s, single, insts, n_inst, w, chunk);
+                HIP_TRY(hipGetLastError());
+            }
             done += chunk;
-            if (timed && !kt->g_missing) kt->graph_pending = chunk;
         }
     }
     while (done < n_steps) {
         const int K = std::min(chunk, n_steps - done);
         hipEvent_t a = nullptr, b = nullptr;
         if (timed) {
-            while (kt->used + 2 > kt->pool.size()) {
-                hipEvent_t x;
-                HIP_TRY(hipEventCreate(&x));
-                kt->pool.push_back(x);
-            }
-            a = kt->pool[kt->used];
-            b = kt->pool[kt->used + 1];
-            kt->used += 2;
-            kt->piece_k.push_back(K);
+            int rc = timer_pair(kt, K, &a, &b);
+            if (rc) return rc;
+            HIP_TRY(hipEventRecord(a, s));
         }
-        HIP_TRY(launch_run(s, single, insts, n_inst, w, K, a, b));
+        launch_step_kernels(s, single, insts, w, K);
+        if (timed) HIP_TRY(hipEventRecord(b, s));
+        launch_run_tail(s, single, insts, n_inst, w, K);
         HIP_TRY(hipGetLastError());
         done += K;
     }
@@ -693,12 +706,6 @@ static int engine_finish(Engine *e)
     HIP_TRY(hipStreamSynchronize(e->stream));
     KTimer &kt = e->kt;
     float x = 0.f;
-    if (kt.graph_pending) {
-        HIP_TRY(hipEventElapsedTime(&x, kt.g0, kt.g1));
-        kt.ms += x;
-        kt.launches += kt.graph_pending;
-        kt.graph_pending = 0;
-    }
     for (size_t i = 0; i < kt.piece_k.size(); ++i) {
         HIP_TRY(hipEventElapsedTime(&x, kt.pool[2 * i], kt.pool[2 * i + 1]));
         kt.ms += x;
@@ -712,7 +719,7 @@ static int engine_finish(Engine *e)
 int engine_run(Engine *e, int n_steps, int use_graph, int graph_steps)
 {
     int rc = run_steps(e->stream, &e->gexec, &e->gexec_steps, &e->hi, e->di, 1, e->w, n_steps, use_graph,
-                       chunk_for(graph_steps, e->hi.R), &e->kt);
+                       chunk_for(graph_steps, e->hi.R), &e->kt, &e->gexec_k, &e->gexec_k_steps);
     if (rc) return rc;
     return engine_finish(e);
 }
@@ -721,7 +728,7 @@ int engine_run_output(Engine *e, int n_steps, int use_graph, int graph_steps, in
                       const uint8_t **train)
 {
     int rc = run_steps(e->stream, &e->gexec, &e->gexec_steps, &e->hi, e->di, 1, e->w, n_steps, use_graph,
-                       chunk_for(graph_steps, e->hi.R), &e->kt);
+                       chunk_for(graph_steps, e->hi.R), &e->kt, &e->gexec_k, &e->gexec_k_steps);
     if (rc) return rc;
     if (train) {
         const int R = e->hi.R;
@@ -735,7 +742,7 @@ int engine_run_output(Engine *e, int n_steps, int use_graph, int graph_steps, in
     return engine_finish(e);
 }
 
-int engine_timer(Engine *e, int on, double *ms, int64_t *launches, int *graph_timed)
+int engine_timer(Engine *e, int on, double *ms, int64_t *launches)
 {
     if (on >= 0) {
         e->kt.on = on != 0;
@@ -743,7 +750,6 @@ int engine_timer(Engine *e, int on, double *ms, int64_t *launches, int *graph_ti
     }
     if (ms) *ms = e->kt.ms;
     if (launches) *launches = e->kt.launches;
-    if (graph_timed) *graph_timed = e->kt.g_missing ? 0 : 1;
     return SCAFFOLD_OK;
 }
 

@@ -186,9 +186,8 @@ int engine_run(Engine *e, int n_steps, int use_graph, int graph_steps);
 int engine_run_output(Engine *e, int n_steps, int use_graph, int graph_steps, int64_t first_out_step,
                       const uint8_t **train);
 // Step-kernel timer: on 1 starts (clears) it, 0 stops it, -1 leaves it; returns
-// the measured milliseconds, the step launches they cover and whether graph
-// replays are measured (1) or only direct launches (0).
-int engine_timer(Engine *e, int on, double *ms, int64_t *launches, int *graph_timed);
+// the measured milliseconds and the step launches they cover.
+int engine_timer(Engine *e, int on, double *ms, int64_t *launches);
 int engine_sync(Engine *e);
 int engine_get_state(Engine *e, float *V, float *g, int32_t *refr);
 int engine_set_state(Engine *e, const float *V, const float *g, const int32_t *refr);
@@ -244,4 +243,5 @@ int describe_reference(scaffold_env *env, const char *dir, uint64_t stim_seed);
 // capi.cpp / planner.cpp
 std::vector<ScStim> dev_stims(scaffold_env *env);
 float plan_multiplier(const scaffold_env *env, int proj, int64_t e);
+void encode_values(int wprec, float *v, int64_t n, float *scale, float *sum_scale);
 }  // namespace sc

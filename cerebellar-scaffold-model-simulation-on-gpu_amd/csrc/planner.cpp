@@ -113,6 +113,33 @@ float plan_multiplier(const scaffold_env *env, int proj, int64_t e)
     }
 }
 
+// Multipliers as storage `wprec` holds them, in place (dequantized): BF16 rounds
+// to nearest even; FP8-e4m3 stores x / scale with the projection's power-of-two
+// scale 2^ceil(log2(max|x| / 448)), so every stored value lies within the
+// format's 448.  *sum_scale (BF16 / FP8) is the sum of the given values over the
+// sum of the stored ones: round to nearest biases the mean of a distribution
+// that straddles a binade (U(0.8, 1.2) loses 0.3 % in FP8), and the engine
+// multiplies each row sum of the projection by this factor.
+void encode_values(int wprec, float *v, int64_t n, float *scale, float *sum_scale)
+{
+    *scale = 1.f;
+    *sum_scale = 1.f;
+    double gen_sum = 0.0, q_sum = 0.0;
+    for (int64_t e = 0; e < n; ++e) gen_sum += v[e];
+    if (wprec == SC_W_FP8) {
+        float mx = 0.f;
+        for (int64_t e = 0; e < n; ++e) mx = std::max(mx, std::fabs(v[e]));
+        *scale = mx > 0.f ? std::ldexp(1.f, (int)std::ceil(std::log2((double)mx / 448.0))) : 1.f;
+        for (int64_t e = 0; e < n; ++e) v[e] = e4m3_to_f32(f32_to_e4m3(v[e] / *scale)) * *scale;
+    } else if (wprec == SC_W_BF16) {
+        for (int64_t e = 0; e < n; ++e) v[e] = bf16_to_f32(f32_to_bf16(v[e]));
+    }
+    if (wprec == SC_W_FP8 || wprec == SC_W_BF16) {
+        for (int64_t e = 0; e < n; ++e) q_sum += v[e];
+        if (q_sum > 0.0 && gen_sum > 0.0) *sum_scale = (float)(gen_sum / q_sum);
+    }
+}
+
 // Multipliers are summed in f64.  The sum of a row is exact when every value is a
 // multiple of 2^q (q from the smallest magnitude) and the row's absolute sum stays
 // below 2^(q+52); then the order of accumulation cannot change the result.
@@ -295,23 +322,7 @@ int plan_build(scaffold_env *env, Plan &P)
         if (sp.wprec != SC_W_UNIT) {
             ev.resize(nnz);
             for (int64_t e = 0; e < nnz; ++e) ev[e] = src.val.empty() ? 1.f : src.val[e];
-            double gen_sum = 0.0, q_sum = 0.0;
-            for (float x : ev) gen_sum += x;
-            if (sp.wprec == SC_W_FP8) {
-                float mx = 0.f;
-                for (float x : ev) mx = std::max(mx, std::fabs(x));
-                sp.scale = mx > 0.f ? std::ldexp(1.f, (int)std::ceil(std::log2((double)mx / 448.0))) : 1.f;
-                for (auto &x : ev) x = e4m3_to_f32(f32_to_e4m3(x / sp.scale)) * sp.scale;
-            } else if (sp.wprec == SC_W_BF16) {
-                for (auto &x : ev) x = bf16_to_f32(f32_to_bf16(x));
-            }
-            // round to nearest biases the mean of a multiplier distribution that
-            // straddles a binade (U(0.8, 1.2) loses 0.3 % in FP8); the row sums of
-            // a quantized projection are scaled back to the generated mean
-            if (sp.wprec == SC_W_FP8 || sp.wprec == SC_W_BF16) {
-                for (float x : ev) q_sum += x;
-                if (q_sum > 0.0 && gen_sum > 0.0) sp.sum_scale = (float)(gen_sum / q_sum);
-            }
+            encode_values(sp.wprec, ev.data(), nnz, &sp.scale, &sp.sum_scale);
             if (!row_sums_exact(ev, src.row_ptr))
                 return fail(SCAFFOLD_ERR_WEIGHT_RANGE,
                             "projection %d: multipliers span too wide a range for exact f64 row sums", pi);
@@ -640,7 +651,7 @@ int plan_build(scaffold_env *env, Plan &P)
         for (float x : ev)
             quiet_keeps = quiet_keeps && !std::signbit(x) && (r.rule != SC_STDP_HEBB || x <= 1.f);
         X.skip_quiet = quiet_keeps;
-        const ScPop &pre = P.pops[src.pre], &post = P.pops[src.post];
+        const ScPop &post = P.pops[src.post];
         for (size_t j = 0; j + 1 < src.row_ptr.size(); ++j) {
             int32_t pb = (int32_t)(post.dev_base + (int)j);  // the row's trigger
             if (tp) {  // the teacher: the first entry of the teacher projection's row (SynapticPlasticity.cu:93)

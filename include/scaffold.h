@@ -165,8 +165,13 @@ int scaffold_set_connectivity_params(scaffold_env *env, int type, const char *fi
  * updated multipliers are rounded to multiples of 2^grid_exp (see
  * scaffold_projection_info), which keeps every row sum exact; a Teacher rule
  * must not potentiate (its multipliers would have no bound).  The spike ring
- * must hold delay + time_window + 2 rows (2 x T_print).  Environments with
- * plasticity cannot join a batch.  At most num_of_plasticities rules. */
+ * is grown to delay + time_window + 2 rows where 2 x T_print is shorter.
+ * time_window is the number of steps the A2_JI sum covers, as the reference's
+ * table field intends ("should be time window"); its kernels use a fixed
+ * s_upper_bound of 50 steps instead (SynapticPlasticity.cu:45,86), which
+ * time_window = 0 selects (the reference's own table sets 50 as well).
+ * Environments with plasticity cannot join a batch.  At most num_of_plasticities
+ * rules. */
 #define SCAFFOLD_STDP_HEBB    0  /* STDP_RULE (struct_enum_def.h:121) */
 #define SCAFFOLD_STDP_TEACHER 1
 #define SCAFFOLD_STDP_A0    0    /* STDP_COEFFICIENTS (struct_enum_def.h:120) */
@@ -418,9 +423,10 @@ int scaffold_time_steps(scaffold_env *env, int n_steps, double *ms_total,
 /* Step-kernel timer (no counterpart in the reference): on = 1 clears and starts
  * it, 0 stops it, -1 only reads it.  While it runs, every run of steps
  * (scaffold_loop_n_steps, scaffold_time_steps) records HIP events on the
- * engine's stream around its step kernels -- inside a captured graph the graph
- * records them, and of a call that replays the graph several times the last
- * replay is measured.  *ms receives the measured time, *launches the step
+ * engine's stream around its step kernels: around each replay of a graph that
+ * holds the run's step kernels alone (the run's statistics and counter advance
+ * follow it as two direct launches), or around the step kernels of a run
+ * launched directly.  *ms receives the measured time, *launches the step
  * kernels it covers (either may be NULL). */
 int scaffold_step_timer(scaffold_env *env, int on, double *ms, int64_t *launches);
 
@@ -454,6 +460,16 @@ int scaffold_batch_time(scaffold_batch *batch, int n_steps, double *ms_total);
 int scaffold_batch_destroy(scaffold_batch *batch);
 
 /* ------------------------------------------------------------ diagnostics */
+/* The multipliers of one projection as the engine stores them with `precision`
+ * (SCAFFOLD_WEIGHTS_*), dequantized into out (which may alias in): BF16 rounded to
+ * nearest even; FP8-e4m3 (OCP) as in / scale rounded to nearest even, saturating
+ * at 448, with the projection's power-of-two scale 2^ceil(log2(max|in| / 448))
+ * (*scale).  *sum_scale: sum(in) / sum(out) for BF16 and FP8 (the factor each
+ * row sum is multiplied by), else 1.  The planner runs exactly this step on every
+ * projection; no counterpart in the reference, whose ELL values are all 1.0. */
+int scaffold_encode_multipliers(int precision, const float *in, int64_t n, float *out, float *scale,
+                                float *sum_scale);
+
 /* Device copy of `bytes` bytes between two library-allocated scratch buffers;
  * used to calibrate HBM counters against a transfer of known size. */
 int scaffold_copy_probe(int64_t bytes, int iters, double *ms);

@@ -372,7 +372,7 @@ def test_plasticity_registration(pkg):
     bad = [dict(rule=2, target=pf_pc), dict(rule=pkg.STDP_HEBB, target=99),
            dict(rule=pkg.STDP_TEACHER, target=pf_pc, teacher=-1),
            dict(rule=pkg.STDP_TEACHER, target=pf_pc, teacher=goc_grc),   # another post population
-           dict(rule=pkg.STDP_HEBB, target=pf_pc, time_window=0),
+           dict(rule=pkg.STDP_HEBB, target=pf_pc, time_window=-1),
            dict(rule=pkg.STDP_HEBB, target=pf_pc, coefficients={pkg.A0: float("nan")})]
     for kw in bad:
         with pytest.raises(pkg.ScaffoldError) as e:
