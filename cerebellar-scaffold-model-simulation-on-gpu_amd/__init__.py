@@ -154,6 +154,7 @@ def load_library(path=None):
         "scaffold_get_spike_row": (ctypes.c_int, [vp, ctypes.c_int, P(ctypes.c_uint8)]),
         "scaffold_get_counts": (ctypes.c_int, [vp, ctypes.c_int, P(i32), P(u64)]),
         "scaffold_get_fired": (ctypes.c_int, [vp, ctypes.c_int, P(i32), i32, P(i32)]),
+        "scaffold_get_fired_noalloc": (ctypes.c_int, [vp, ctypes.c_int, P(i32), i32, P(i32)]),
         "scaffold_get_totals": (ctypes.c_int, [vp, P(u64), P(u64)]),
         "scaffold_write_spike_text": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, cs]),
         "scaffold_write_output_spike_text": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, cs]),
@@ -447,6 +448,14 @@ class NetworkEnvironment:
         ids = np.zeros(max(cap, 1), np.int32)
         n = ctypes.c_int32()
         _check(self._lib.scaffold_get_fired(self._h, step, _ptr(ids, ctypes.c_int32), cap, ctypes.byref(n)))
+        return ids[:min(n.value, cap)], n.value
+
+    def fired_noalloc(self, step, cap=None):
+        """fired() through the library's static scratch (no device allocation)."""
+        cap = self.info().total_neurons if cap is None else cap
+        ids = np.zeros(max(cap, 1), np.int32)
+        n = ctypes.c_int32()
+        _check(self._lib.scaffold_get_fired_noalloc(self._h, step, _ptr(ids, ctypes.c_int32), cap, ctypes.byref(n)))
         return ids[:min(n.value, cap)], n.value
 
     def write_spike_text(self, step0, n, filename):

@@ -887,6 +887,18 @@ int scaffold_get_fired(scaffold_env *env, int step, int32_t *ids, int32_t cap, i
     return engine_get_fired(env->eng, row, ids, cap, n);
 }
 
+int scaffold_get_fired_noalloc(scaffold_env *env, int step, int32_t *ids, int32_t cap, int32_t *n)
+{
+    clear_error();
+    int rc = check_ready(env);
+    if (rc) return rc;
+    if (!n || (cap > 0 && !ids)) return fail(SCAFFOLD_ERR_INVALID_ARG, "null output");
+    int row = 0;
+    rc = step_row(env, step, &row);
+    if (rc) return rc;
+    return engine_get_fired_noalloc(env->eng, row, ids, cap, n);
+}
+
 // The spike lines of steps [step0, step0 + n) in the reference's text format.
 // output == 0: the lines print_spike_train writes for the populations simulated
 // on the device (GPUScaffoldSimulation.cu:76-84): "%f\t%d\t%d\n" = (step as ms,

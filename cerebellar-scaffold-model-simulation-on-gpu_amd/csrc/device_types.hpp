@@ -243,6 +243,12 @@ struct ScPlastic {
     float coef[4];             // A0, A1_I, A1_J, A2_JI
 };
 
+// Fired-list queries without allocation: ids at most, and the word offset of the
+// ids in the static scratch (word 0 holds the count; 64 words keep the ids
+// 256-byte aligned).
+#define SC_FIRED_NOALLOC_MAX (1 << 22)
+#define SC_SCRATCH_IDS       64
+
 struct ScInstance {
     ScState  *st;              // per device index
     uint64_t *ring;

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "kernels.hpp"
@@ -52,6 +53,7 @@ struct Engine {
     int n_stims = 0;
     int n_pops = 0;
     int32_t *cnt_buf = nullptr;  // compaction count
+    int32_t *scratch = nullptr;  // the static scratch on this engine's device (no-allocation queries)
     std::vector<int32_t> dev_to_ref;
     int Ndev = 0, total_ref = 0;
     hipGraphExec_t gexec = nullptr;
@@ -74,6 +76,31 @@ struct Engine {
     };
     std::vector<PlasticView> plastic_views;
 };
+
+// Static scratch of scaffold_get_fired_noalloc: word 0 receives the fired count,
+// words SC_SCRATCH_IDS .. the ids (256-byte aligned).  One copy per GPU, held
+// until the process ends; its device address is looked up once per device with
+// hipGetSymbolAddress (in host code the array's name is not a device address).
+static __device__ __align__(256) int32_t scratch[SC_SCRATCH_IDS + SC_FIRED_NOALLOC_MAX];
+static constexpr int kMaxDevices = 64;
+static int32_t *g_scratch[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+// The scratch address on the current device (looked up on first use per device).
+static int scratch_on_current_device(int32_t **out)
+{
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(SCAFFOLD_ERR_CAPACITY, "device %d beyond the scratch table", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch[dev] = static_cast<int32_t *>(p);
+    }
+    *out = g_scratch[dev];
+    return SCAFFOLD_OK;
+}
 
 template <typename T>
 static int dalloc(std::vector<void *> &allocs, int64_t *bytes, T **p, size_t count)
@@ -497,6 +524,7 @@ int engine_create(Engine **out, Plan &P)
     e->w.max_groups = P.n_groups;
     e->w.nontemporal = P.nontemporal;
     if ((rc = dalloc(A, B, &e->cnt_buf, 1))) return bail(rc);
+    if ((rc = scratch_on_current_device(&e->scratch))) return bail(rc);
     // output spike train (T_out steps x n_out neurons), copied to pinned host memory
     e->n_out = P.n_output;
     e->T_out = std::max(1, P.T_out);
@@ -878,6 +906,23 @@ int engine_get_fired(Engine *e, int row, int32_t *ids, int32_t cap, int32_t *n)
         err = hipMemcpy(ids, dout, sizeof(int32_t) * std::min(cap, *n), hipMemcpyDeviceToHost);
     (void)hipFree(dout);
     if (err != hipSuccess) return fail(SCAFFOLD_ERR_HIP, "fired-list compaction: %s", hipGetErrorString(err));
+    return SCAFFOLD_OK;
+}
+
+// engine_get_fired without allocating: the compaction writes its count and ids
+// into the static scratch of the engine's device, then both are copied out.
+int engine_get_fired_noalloc(Engine *e, int row, int32_t *ids, int32_t cap, int32_t *n)
+{
+    if (cap < 0 || cap > SC_FIRED_NOALLOC_MAX)
+        return fail(SCAFFOLD_ERR_CAPACITY, "cap %d: the static scratch holds at most %d ids", cap, SC_FIRED_NOALLOC_MAX);
+    int32_t *count = e->scratch, *out = e->scratch + SC_SCRATCH_IDS;
+    launch_compact(e->stream, e->hi.ring + (size_t)row * e->hi.Wr, e->hi.n_groups, e->hi.groups, e->hi.pops, out, cap,
+                   count);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    HIP_TRY(hipMemcpy(n, count, sizeof(int32_t), hipMemcpyDeviceToHost));
+    const int32_t k = std::min(cap, *n);
+    if (k > 0) HIP_TRY(hipMemcpy(ids, out, sizeof(int32_t) * (size_t)k, hipMemcpyDeviceToHost));
     return SCAFFOLD_OK;
 }
 

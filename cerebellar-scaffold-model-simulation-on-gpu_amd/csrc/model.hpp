@@ -194,6 +194,8 @@ int engine_set_state(Engine *e, const float *V, const float *g, const int32_t *r
 int engine_get_ring_row(Engine *e, int row, uint64_t *words);
 int engine_get_counts(Engine *e, int row, int32_t *counts, unsigned long long *events);
 int engine_get_fired(Engine *e, int row, int32_t *ids, int32_t cap, int32_t *n);
+// engine_get_fired through the static scratch of the engine's device (no allocation).
+int engine_get_fired_noalloc(Engine *e, int row, int32_t *ids, int32_t cap, int32_t *n);
 int engine_get_totals(Engine *e, unsigned long long *totals);
 int engine_time(Engine *e, int n_steps, int use_graph, int graph_steps,
                 double *ms_total, double *ms_kernel_mean);

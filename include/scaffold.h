@@ -399,6 +399,14 @@ int scaffold_get_totals(scaffold_env *env, uint64_t *spikes, uint64_t *events);
 /* Compacted fired-neuron list of step `step` (global ids, ascending), built on
  * the GPU with wavefront ballot/popcount and a prefix sum. */
 int scaffold_get_fired(scaffold_env *env, int step, int32_t *ids, int32_t cap, int32_t *n);
+/* scaffold_get_fired without allocating device memory (no counterpart in the
+ * reference): for callers that cannot allocate -- code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc.  The compaction's count and
+ * ids go through a static scratch array of each GPU (16 MiB, held until the
+ * process ends), so cap may be at most 4,194,304 ids (SCAFFOLD_ERR_CAPACITY
+ * otherwise).  Every call on one GPU shares that array: calls that overlap on
+ * one GPU (from several threads or environments at once) race. */
+int scaffold_get_fired_noalloc(scaffold_env *env, int step, int32_t *ids, int32_t cap, int32_t *n);
 /* Append the spikes of trial-relative steps [step0, step0+n) to filename as the
  * reference's printer thread print_spike_train writes them
  * (GPUScaffoldSimulation.cu:76-84): "%f\t%d\t%d\n" = (step as ms,
