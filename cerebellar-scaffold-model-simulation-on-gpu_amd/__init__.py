@@ -68,7 +68,8 @@ class Info(ctypes.Structure):
                 ("current_step", ctypes.c_int32), ("n_output_neurons", ctypes.c_int32),
                 ("device_bytes", ctypes.c_int64), ("algorithmic_bytes_per_step", ctypes.c_int64),
                 ("n_tasks", ctypes.c_int32), ("n_groups", ctypes.c_int32), ("synapses_16bit", ctypes.c_int64),
-                ("n_blocks", ctypes.c_int32), ("n_staged_blocks", ctypes.c_int32)]
+                ("n_blocks", ctypes.c_int32), ("n_staged_blocks", ctypes.c_int32),
+                ("lane_synapses", ctypes.c_int32), ("stage_ratio", ctypes.c_int32)]
 
 
 class PopulationInfo(ctypes.Structure):
@@ -158,9 +159,11 @@ def load_library(path=None):
         "scaffold_get_totals": (ctypes.c_int, [vp, P(u64), P(u64)]),
         "scaffold_write_spike_text": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, cs]),
         "scaffold_write_output_spike_text": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, cs]),
+        "scaffold_record_spikes": (ctypes.c_int, [vp, cs]),
         "scaffold_step_timer": (ctypes.c_int, [vp, ctypes.c_int, P(f64), P(i64)]),
         "scaffold_time_steps": (ctypes.c_int, [vp, ctypes.c_int, P(f64), P(f64)]),
         "scaffold_autotune": (ctypes.c_int, [vp, ctypes.c_int, vp]),
+        "scaffold_time_layout": (ctypes.c_int, [vp, ctypes.c_int, P(f64), P(f64)]),
         "scaffold_batch_create": (ctypes.c_int, [P(vp)]),
         "scaffold_batch_add": (ctypes.c_int, [vp, vp]),
         "scaffold_batch_build": (ctypes.c_int, [vp]),
@@ -323,6 +326,13 @@ class NetworkEnvironment:
         _check(self._lib.scaffold_autotune(self._h, n_steps, ctypes.byref(r)))
         return {f: getattr(r, f) for f, _ in TuneResult._fields_}
 
+    def time_layout(self, n_steps=100):
+        """Time a private engine of the network under the current options (before
+        initialize): (device ms per step, mean step-kernel ms)."""
+        ms, km = ctypes.c_double(), ctypes.c_double()
+        _check(self._lib.scaffold_time_layout(self._h, n_steps, ctypes.byref(ms), ctypes.byref(km)))
+        return ms.value, km.value
+
     def write_dat(self, index, filename):
         _check(self._lib.scaffold_write_dat(self._h, index, str(filename).encode()))
 
@@ -465,6 +475,12 @@ class NetworkEnvironment:
     def write_output_spike_text(self, step0, n, filename):
         """The reference host thread's lines (OUTPUT populations) of n steps."""
         _check(self._lib.scaffold_write_output_spike_text(self._h, step0, n, str(filename).encode()))
+
+    def record_spikes(self, directory):
+        """The reference's PRINT mode: loop_n_steps writes spike_dev0_{0,1}.dat and
+        spike_host{0,1}.dat into `directory` as it runs; reset() merges them into
+        spike_dev0.dat and spike_result_trial<trial>.dat.  None stops recording."""
+        _check(self._lib.scaffold_record_spikes(self._h, None if directory is None else str(directory).encode()))
 
     def step_timer(self, on=-1):
         """Step-kernel timer: on=1 clears and starts it, 0 stops it, -1 reads it.

@@ -9,6 +9,7 @@
 #include <new>
 
 #include "model.hpp"
+#include "recorder.hpp"
 
 namespace sc {
 
@@ -443,7 +444,7 @@ int scaffold_set_option(scaffold_env *env, const char *key, int64_t value)
         o.lane_max_fanin = (int)value;
     } else if (!strcmp(key, "lane_synapses")) {
         if (frozen) return frozen_err();
-        if (value < 1 || value > 4096) return fail(SCAFFOLD_ERR_INVALID_ARG, "lane_synapses in [1, 4096]");
+        if (value < 0 || value > 4096) return fail(SCAFFOLD_ERR_INVALID_ARG, "lane_synapses in [0, 4096]");
         o.lane_synapses = (int)value;
     } else if (!strcmp(key, "graph")) o.graph = value ? 1 : 0;
     else if (!strcmp(key, "graph_steps")) {
@@ -453,6 +454,10 @@ int scaffold_set_option(scaffold_env *env, const char *key, int64_t value)
     else if (!strcmp(key, "stage_spikes")) {
         if (frozen) return frozen_err();
         o.stage_spikes = value ? 1 : 0;
+    } else if (!strcmp(key, "stage_ratio")) {
+        if (frozen) return frozen_err();
+        if (value < 0 || value > 1000000) return fail(SCAFFOLD_ERR_INVALID_ARG, "stage_ratio in [0, 1000000]");
+        o.stage_ratio = (int)value;
     } else if (!strcmp(key, "nontemporal")) {
         if (frozen) return frozen_err();
         if (value < -1 || value > 1) return fail(SCAFFOLD_ERR_INVALID_ARG, "nontemporal is -1, 0 or 1");
@@ -477,6 +482,7 @@ int scaffold_get_option(scaffold_env *env, const char *key, int64_t *value)
     else if (!strcmp(key, "graph_steps")) *value = o.graph_steps;
     else if (!strcmp(key, "reset_g")) *value = o.reset_g;
     else if (!strcmp(key, "stage_spikes")) *value = o.stage_spikes;
+    else if (!strcmp(key, "stage_ratio")) *value = o.stage_ratio;
     else if (!strcmp(key, "nontemporal")) *value = o.nontemporal;
     else return fail(SCAFFOLD_ERR_INVALID_ARG, "unknown option %s", key);
     return SCAFFOLD_OK;
@@ -528,6 +534,8 @@ int scaffold_loop_n_steps(scaffold_env *env, int start_step, int n_steps, int tr
     if (rc) return rc;
     if (start_step < 0 || n_steps < 0) return fail(SCAFFOLD_ERR_INVALID_ARG, "negative step");
     if (start_step != env->cur_step) {
+        if (env->rec && ((rc = engine_record_sync(env->eng, env->rec)) || (rc = recorder_restart(env->rec))))
+            return rc;
         rc = engine_set_ctl(env->eng, start_step, env->draws);
         if (rc) return rc;
         env->cur_step = start_step;
@@ -544,7 +552,7 @@ int scaffold_loop_n_steps(scaffold_env *env, int start_step, int n_steps, int tr
     const bool handoff = output_spike_train && n_steps > 0 && last > env->cur_step;
     const uint8_t *train = nullptr;
     rc = engine_run_output(env->eng, n_steps, env->opt.graph, env->opt.graph_steps, last - T,
-                           handoff ? &train : nullptr);
+                           handoff ? &train : nullptr, env->rec, T, env->cur_step);
     if (rc) return rc;
     env->cur_step = end;
     env->draws += n_steps;
@@ -576,10 +584,13 @@ int scaffold_set_input_freq(scaffold_env *env, const float *freq_list)
 
 int scaffold_reset_network_env(scaffold_env *env, int trial)
 {
-    (void)trial;
     clear_error();
     int rc = check_ready(env);
     if (rc) return rc;
+    // PRINT mode (GPUScaffoldSimulation.cu:809-846): the trial's spike files are
+    // merged before the state is re-initialized.
+    if (env->rec && ((rc = engine_record_sync(env->eng, env->rec)) || (rc = recorder_merge(env->rec, trial))))
+        return rc;
     rc = engine_init_params(env->eng, env->opt.neuron_seed, env->init_calls);
     if (rc) return rc;
     env->init_calls++;
@@ -596,6 +607,15 @@ int scaffold_finalize_network_env(scaffold_env *env)
     if (!env) return fail(SCAFFOLD_ERR_INVALID_ARG, "null environment");
     if (env->in_batches > 0)
         return fail(SCAFFOLD_ERR_STATE, "environment is part of a batch; destroy the batch first");
+    if (env->rec) {  // the last windows are written; the files are left as they are
+        if (env->eng) {
+            (void)engine_record_sync(env->eng, env->rec);
+            (void)engine_sync(env->eng);
+        }
+        (void)recorder_finish(env->rec);
+        recorder_destroy(env->rec);
+        env->rec = nullptr;
+    }
     if (env->eng) engine_destroy(env->eng);
     env->eng = nullptr;
     delete env;
@@ -677,6 +697,8 @@ int scaffold_get_info(scaffold_env *env, scaffold_info *out)
         i.n_groups = env->plan.n_groups;
         i.synapses_16bit = env->plan.syn16;
         engine_blocks(env->eng, &i.n_blocks, &i.n_staged_blocks);
+        i.lane_synapses = env->plan.lane_synapses;
+        i.stage_ratio = env->plan.stage_ratio;
     }
     *out = i;
     return SCAFFOLD_OK;
@@ -955,6 +977,27 @@ int scaffold_write_output_spike_text(scaffold_env *env, int step0, int n, const 
     return write_spike_lines(env, step0, n, filename, 1);
 }
 
+int scaffold_record_spikes(scaffold_env *env, const char *dir)
+{
+    clear_error();
+    int rc = check_ready(env);
+    if (rc) return rc;
+    if (env->in_batches > 0) return fail(SCAFFOLD_ERR_STATE, "environment is part of a batch");
+    if (env->rec) {  // stop the current recording: its windows are written first
+        rc = engine_record_sync(env->eng, env->rec);
+        if (!rc) rc = engine_sync(env->eng);
+        const int frc = recorder_finish(env->rec);
+        recorder_destroy(env->rec);
+        env->rec = nullptr;
+        if (rc || frc) return rc ? rc : frc;
+    }
+    if (!dir) return SCAFFOLD_OK;
+    const Plan &P = env->plan;
+    std::vector<int32_t> types;
+    for (size_t p = 0; p < P.pops.size(); ++p) types.push_back(env->pops[p].type);
+    return recorder_create(&env->rec, dir, env->T_cpu_sim, env->T_print, P.R, P.Wr, P.pops, types);
+}
+
 int scaffold_step_timer(scaffold_env *env, int on, double *ms, int64_t *launches)
 {
     clear_error();
@@ -970,6 +1013,7 @@ int scaffold_time_steps(scaffold_env *env, int n_steps, double *ms_total, double
     int rc = check_ready(env);
     if (rc) return rc;
     if (n_steps <= 0) return fail(SCAFFOLD_ERR_INVALID_ARG, "n_steps must be positive");
+    if (env->rec && (rc = engine_record_sync(env->eng, env->rec))) return rc;  // these steps are not recorded
     rc = engine_time(env->eng, n_steps, env->opt.graph, env->opt.graph_steps, ms_total, ms_step_kernel_mean);
     if (rc) return rc;
     env->cur_step += n_steps;
@@ -978,8 +1022,9 @@ int scaffold_time_steps(scaffold_env *env, int n_steps, double *ms_total, double
 }
 
 // One trial engine of the network under the options in env->opt: built,
-// initialized like InitializeNetworkEnv, warmed up, timed, destroyed.
-static int trial_time(scaffold_env *env, int T_print, int n_steps, double *ms)
+// initialized like InitializeNetworkEnv, warmed up, timed (and its step kernels
+// timed with the step-kernel timer when kernel_ms is given), destroyed.
+static int trial_time(scaffold_env *env, int T_print, int n_steps, double *ms, double *kernel_ms = nullptr)
 {
     Plan P;
     P.R = 2 * T_print;
@@ -993,10 +1038,35 @@ static int trial_time(scaffold_env *env, int T_print, int n_steps, double *ms)
     if (!rc) rc = engine_set_ctl(e, 0, 0);
     if (!rc) rc = engine_run(e, 50, env->opt.graph, env->opt.graph_steps);
     double total = 0.0;
+    if (!rc && kernel_ms) rc = engine_timer(e, 1, nullptr, nullptr);
     if (!rc) rc = engine_time(e, n_steps, env->opt.graph, env->opt.graph_steps, &total, nullptr);
+    if (!rc && kernel_ms) {
+        double kms = 0.0;
+        int64_t launches = 0;
+        rc = engine_timer(e, 0, &kms, &launches);
+        *kernel_ms = kms / (double)std::max<int64_t>(1, launches);
+    }
     engine_destroy(e);
     if (!rc) *ms = total / n_steps;
     return rc;
+}
+
+// The spike ring of a trial engine: T_print of at least 50 steps and the longest delay.
+static int trial_T_print(scaffold_env *env)
+{
+    float max_delay = 0.f;
+    for (auto &p : env->projs)
+        if (p.set && p.delay_ms > max_delay) max_delay = p.delay_ms;
+    return std::max(50, (int)max_delay);
+}
+
+int scaffold_time_layout(scaffold_env *env, int n_steps, double *ms_per_step, double *kernel_ms)
+{
+    clear_error();
+    int rc = check_described(env);
+    if (rc) return rc;
+    if (n_steps <= 0 || !ms_per_step) return fail(SCAFFOLD_ERR_INVALID_ARG, "bad arguments");
+    return trial_time(env, trial_T_print(env), n_steps, ms_per_step, kernel_ms);
 }
 
 int scaffold_autotune(scaffold_env *env, int n_steps, scaffold_tune_result *out)
@@ -1005,15 +1075,16 @@ int scaffold_autotune(scaffold_env *env, int n_steps, scaffold_tune_result *out)
     int rc = check_described(env);
     if (rc) return rc;
     if (!out || n_steps <= 0) return fail(SCAFFOLD_ERR_INVALID_ARG, "bad arguments");
-    float max_delay = 0.f;
-    for (auto &p : env->projs)
-        if (p.set && p.delay_ms > max_delay) max_delay = p.delay_ms;
-    const int T_print = std::max(50, (int)max_delay);
+    const int T_print = trial_T_print(env);
     const Options saved = env->opt;
     scaffold_tune_result r{};
     r.ms_per_step = 1e300;
     r.ms_default = -1.0;
     const int fanins[] = {16, 48, 128}, synapses[] = {8, 16, 32};
+    // the environment's own options are the first candidate
+    r.lane_max_fanin = saved.lane_max_fanin;
+    r.lane_synapses = saved.lane_synapses;
+    r.graph_steps = saved.graph_steps;
     r.index_bits = saved.index_bits;
     r.stage_spikes = saved.stage_spikes;
     // the winner so far as options, with one field changed
@@ -1030,12 +1101,13 @@ int scaffold_autotune(scaffold_env *env, int n_steps, scaffold_tune_result *out)
         if (!trc) ++r.n_tried;
         return trc;
     };
+    if ((rc = trial([](Options &) {}, &r.ms_default))) return rc;
+    r.ms_per_step = r.ms_default;
     for (int lf : fanins)
         for (int ls : synapses) {
             double ms = 0.0;
             if ((rc = trial([&](Options &o) { o.lane_max_fanin = lf; o.lane_synapses = ls; o.graph_steps = 25; }, &ms)))
                 return rc;
-            if (lf == Options().lane_max_fanin && ls == Options().lane_synapses) r.ms_default = ms;
             if (ms < r.ms_per_step) {
                 r.ms_per_step = ms;
                 r.lane_max_fanin = lf;
@@ -1117,6 +1189,7 @@ int scaffold_batch_add(scaffold_batch *batch, scaffold_env *env)
         if (e == env) return fail(SCAFFOLD_ERR_INVALID_ARG, "environment added twice");
     if (!env->plast.empty())
         return fail(SCAFFOLD_ERR_UNSUPPORTED, "environments with plasticity rules cannot join a batch");
+    if (env->rec) return fail(SCAFFOLD_ERR_STATE, "a recording environment cannot join a batch");
     batch->envs.push_back(env);
     env->in_batches++;
     return SCAFFOLD_OK;

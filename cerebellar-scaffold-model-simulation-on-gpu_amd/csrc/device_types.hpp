@@ -153,10 +153,15 @@ struct ScStimTask {        // 64 neurons of an INPUT population
     int32_t inst, group, pop, nvalid;
 };
 
+// ScTeamIn.lds of a projection that a staged block reads from the ring in
+// memory (a block of 16-bit team rows whose windows did not all fit in LDS).
+#define SC_LDS_GLOBAL ((int32_t)0x80000000)
+
 struct ScTeamIn {          // one incoming projection of a team-mode neuron
     int32_t end;           // end of its segment in the merged row (32-bit form; informational)
     int32_t delay, exc;
-    int32_t lds;           // staged blocks: LDS word that stands for word 0 of the ring row
+    int32_t lds;           // staged blocks: LDS word that stands for word 0 of the ring row,
+                           // or SC_LDS_GLOBAL
     float W, scale;
 };
 

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "kernels.hpp"
+#include "recorder.hpp"
 
 namespace sc {
 
@@ -21,6 +22,8 @@ struct TaskWin {
     int32_t delay[SC_MAX_IN] = {}, pre[SC_MAX_IN] = {};
     uint32_t lo[SC_MAX_IN] = {}, hi[SC_MAX_IN] = {};
     int64_t segs = 0;
+    int64_t qsegs[SC_MAX_IN] = {};  // segments of each slice / projection
+    bool partial = false;           // a team in the 16-bit form: its projections can be staged one by one
 };
 
 // Optional timing of the step kernels inside the runs: a pair of events recorded
@@ -171,15 +174,20 @@ static ScLayout make_layout(const std::vector<int> &team_t, int n_lanes, int n_s
 
 // Staging plan: for every block of lane groups or team neurons, the windows of
 // ring-row words its tasks read, one per (instance, delay, presynaptic
-// population) spanning the union of their bit ranges.  A block is staged when
-// the windows fit the LDS budget and copying them touches at most half the
-// 64-byte segments its gathers would (window words / 16 <= segments / 2); the
-// slices and projections of a staged block then carry the LDS word that stands
-// for word 0 of their ring row.
+// population) spanning the union of their bit ranges.  A block stages all its
+// windows when they fit the LDS budget and hold at most `ratio` words per
+// 64-byte segment its gathers would touch (ratio 8: the copy touches at most
+// half the segments the gathers would, words / 16 <= segments / 2).  When they
+// do not, a block of team neurons in the 16-bit form (whose chunks name their
+// projection) stages the windows that pay most -- most segments per word, each
+// within the ratio -- while they fit, and its other projections read the ring
+// in memory (their ScTeamIn.lds is SC_LDS_GLOBAL).  The slices and projections
+// of a staged window carry the LDS word that stands for word 0 of its ring row.
 static std::vector<ScStage> plan_stages(const ScLayout &L, int n_blocks, std::vector<ScLaneTask> &lanes,
                                         const std::vector<TaskWin> &lane_win, std::vector<ScTeamTask> &teams,
-                                        const std::vector<TaskWin> &team_win, int *n_staged)
+                                        const std::vector<TaskWin> &team_win, int ratio, int *n_staged)
 {
+    constexpr int MAXK = 4 * SC_MAX_WIN;  // candidate windows per block
     std::vector<ScStage> st((size_t)n_blocks);
     *n_staged = 0;
     for (int c = 0; c <= SC_CLASS_LANE; ++c) {
@@ -189,11 +197,12 @@ static std::vector<ScStage> plan_stages(const ScLayout &L, int n_blocks, std::ve
             const int bi = b - b0, per = lane ? SC_LANE_PER_BLOCK : 1 << c;
             const int first = (lane ? 0 : L.cls_task0[c]) + bi * per;
             const int count = std::min(per, L.cls_ntask[c] - bi * per);
-            ScWin wins[SC_MAX_WIN];
-            int32_t pre_of[SC_MAX_WIN];
-            uint32_t lo_of[SC_MAX_WIN], hi_of[SC_MAX_WIN];
+            ScWin wins[MAXK];
+            int32_t pre_of[MAXK];
+            uint32_t lo_of[MAXK], hi_of[MAXK];
+            int64_t seg_of[MAXK];
             int nw = 0;
-            bool ok = true;
+            bool ok = true, partial = !lane;
             int64_t segs = 0;
             auto key_of = [&](int inst, int delay, int pre) {
                 for (int i = 0; i < nw; ++i)
@@ -204,39 +213,66 @@ static std::vector<ScStage> plan_stages(const ScLayout &L, int n_blocks, std::ve
                 const TaskWin &W = lane ? lane_win[first + i] : team_win[first + i];
                 const int inst = lane ? lanes[first + i].inst : teams[first + i].inst;
                 segs += W.segs;
+                partial = partial && W.partial;
                 for (int r = 0; r < W.n && ok; ++r) {
                     if (W.lo[r] > W.hi[r]) continue;
                     int kx = key_of(inst, W.delay[r], W.pre[r]);
                     if (kx < 0) {
-                        if (nw == SC_MAX_WIN) { ok = false; break; }
+                        if (nw == MAXK) { ok = false; break; }
                         kx = nw++;
                         wins[kx] = ScWin{inst, W.delay[r], 0, 0, 0, 0};
                         pre_of[kx] = W.pre[r];
                         lo_of[kx] = W.lo[r];
                         hi_of[kx] = W.hi[r];
+                        seg_of[kx] = 0;
                     }
                     lo_of[kx] = std::min(lo_of[kx], W.lo[r]);
                     hi_of[kx] = std::max(hi_of[kx], W.hi[r]);
+                    seg_of[kx] += W.qsegs[r];
                 }
             }
             if (!ok || nw == 0) continue;
-            int64_t words = 4;  // LDS words 0..3: zero (padding entries read word 0)
+            int64_t total = 4;  // LDS words 0..3: zero (padding entries read word 0)
             for (int i = 0; i < nw; ++i) {  // whole 16-byte vectors: rows are 16-byte aligned
                 wins[i].w0 = (int32_t)((lo_of[i] >> 5) & ~3u);
                 wins[i].nw = (int32_t)((((hi_of[i] >> 5) + 4) & ~3u) - (uint32_t)wins[i].w0);
-                wins[i].off = (int32_t)words;
-                words += wins[i].nw;
+                total += wins[i].nw;
             }
-            if (words > SC_WIN_WORDS || words > 8 * segs) continue;
+            bool chosen[MAXK] = {};
+            int n_chosen = 0;
+            if (nw <= SC_MAX_WIN && total <= SC_WIN_WORDS && total <= (int64_t)ratio * segs) {
+                for (int i = 0; i < nw; ++i) chosen[i] = true;
+                n_chosen = nw;
+            } else if (partial) {
+                int order[MAXK];
+                for (int i = 0; i < nw; ++i) order[i] = i;
+                std::sort(order, order + nw, [&](int x, int y) {  // most segments per word first
+                    return seg_of[x] * wins[y].nw > seg_of[y] * wins[x].nw;
+                });
+                int64_t words = 4;
+                for (int k = 0; k < nw && n_chosen < SC_MAX_WIN; ++k) {
+                    const int i = order[k];
+                    if (wins[i].nw > (int64_t)ratio * seg_of[i] || words + wins[i].nw > SC_WIN_WORDS) continue;
+                    chosen[i] = true;
+                    words += wins[i].nw;
+                    ++n_chosen;
+                }
+            }
+            if (n_chosen == 0) continue;
             ScStage &S = st[(size_t)b];
-            S.n = nw;
-            for (int i = 0; i < nw; ++i) S.w[i] = wins[i];
+            int64_t off = 4;
+            for (int i = 0; i < nw; ++i) {
+                if (!chosen[i]) continue;
+                wins[i].off = (int32_t)off;
+                off += wins[i].nw;
+                S.w[S.n++] = wins[i];
+            }
             for (int i = 0; i < count; ++i) {
                 const TaskWin &W = lane ? lane_win[first + i] : team_win[first + i];
                 const int inst = lane ? lanes[first + i].inst : teams[first + i].inst;
                 for (int r = 0; r < W.n; ++r) {
                     const int kx = W.lo[r] > W.hi[r] ? -1 : key_of(inst, W.delay[r], W.pre[r]);
-                    const int32_t lds = kx < 0 ? 0 : wins[kx].off - wins[kx].w0;
+                    const int32_t lds = kx < 0 ? 0 : chosen[kx] ? wins[kx].off - wins[kx].w0 : SC_LDS_GLOBAL;
                     if (lane) lanes[first + i].sl[r].lds = lds;
                     else teams[first + i].in[r].lds = lds;
                 }
@@ -250,7 +286,7 @@ static std::vector<ScStage> plan_stages(const ScLayout &L, int n_blocks, std::ve
 static int upload_work(std::vector<void *> &allocs, int64_t *bytes, DevWork &w, const std::vector<int> &team_t,
                        std::vector<ScLaneTask> lanes, const std::vector<ScStimTask> &stims,
                        std::vector<ScTeamTask> teams, const std::vector<TaskWin> &lane_win,
-                       const std::vector<TaskWin> &team_win, bool stage, int n_hist)
+                       const std::vector<TaskWin> &team_win, bool stage, int ratio, int n_hist)
 {
     for (size_t i = 1; i < team_t.size(); ++i)
         if (team_t[i] > team_t[i - 1]) return fail(SCAFFOLD_ERR_STATE, "team descriptors not ordered by width");
@@ -263,7 +299,7 @@ static int upload_work(std::vector<void *> &allocs, int64_t *bytes, DevWork &w, 
     w.layout = make_layout(team_t, (int)lanes.size(), (int)stims.size(), n_hist, &w.n_blocks);
     int n_staged = 0;
     std::vector<ScStage> st;
-    if (stage) st = plan_stages(w.layout, w.n_blocks, lanes, lane_win, teams, team_win, &n_staged);
+    if (stage) st = plan_stages(w.layout, w.n_blocks, lanes, lane_win, teams, team_win, ratio, &n_staged);
     w.stages = nullptr;
     w.n_staged = n_staged;
     if (n_staged > 0) {
@@ -415,10 +451,12 @@ int engine_create(Engine **out, Plan &P)
             TaskWin tw;
             tw.n = pp.n_in;
             tw.segs = T.row_seg[j];
+            tw.partial = T.idx16;
             for (int q = 0; q < pp.n_in; ++q) {
                 const ScProj &sp = P.projs[pp.in_proj[q]];
                 tw.delay[q] = sp.delay;
                 tw.pre[q] = sp.pre_pop;
+                tw.qsegs[q] = T.q_seg[(size_t)j * pp.n_in + q];
                 tw.lo[q] = T.q_lo[(size_t)j * pp.n_in + q];
                 tw.hi[q] = T.q_hi[(size_t)j * pp.n_in + q];
                 d.in[q].end = T.seg_end[(size_t)j * pp.n_in + q];
@@ -449,6 +487,7 @@ int engine_create(Engine **out, Plan &P)
                 tw.lo[l.n_sl] = H.sl_lo[gi];
                 tw.hi[l.n_sl] = H.sl_hi[gi];
                 tw.segs += H.sl_seg[gi];
+                tw.qsegs[l.n_sl] = H.sl_seg[gi];
                 tw.n = l.n_sl + 1;
                 ScSlice &s = l.sl[l.n_sl++];
                 const int64_t off = H.sell_off[gi];
@@ -478,7 +517,7 @@ int engine_create(Engine **out, Plan &P)
     int n_hist = 0;  // window-count groups of the plasticity rules
     for (const PlasticPlan &X : P.plastic) n_hist += (P.pops[P.projs[X.rule.target].pre_pop].num + SC_WAVE - 1) / SC_WAVE;
     if ((rc = upload_work(A, B, e->w, e->h_team_t, e->h_lanes, e->h_stims, e->h_teams, e->h_lane_win, e->h_team_win,
-                          P.stage != 0, n_hist)))
+                          P.stage != 0, P.stage_ratio, n_hist)))
         return bail(rc);
     // plasticity rules: per-synapse arrays; the multipliers they update are the
     // f32 arrays of the target's lane slices or merged rows
@@ -753,11 +792,25 @@ int engine_run(Engine *e, int n_steps, int use_graph, int graph_steps)
 }
 
 int engine_run_output(Engine *e, int n_steps, int use_graph, int graph_steps, int64_t first_out_step,
-                      const uint8_t **train)
+                      const uint8_t **train, Recorder *rec, int rec_T, int64_t step0)
 {
-    int rc = run_steps(e->stream, &e->gexec, &e->gexec_steps, &e->hi, e->di, 1, e->w, n_steps, use_graph,
-                       chunk_for(graph_steps, e->hi.R), &e->kt, &e->gexec_k, &e->gexec_k_steps);
-    if (rc) return rc;
+    const int chunk = chunk_for(graph_steps, e->hi.R);
+    int rc = SCAFFOLD_OK;
+    if (!rec) {
+        rc = run_steps(e->stream, &e->gexec, &e->gexec_steps, &e->hi, e->di, 1, e->w, n_steps, use_graph, chunk,
+                       &e->kt, &e->gexec_k, &e->gexec_k_steps);
+        if (rc) return rc;
+    }
+    for (int done = 0; rec && done < n_steps;) {
+        const int64_t s = step0 + done;
+        const int K = (int)std::min<int64_t>(n_steps - done, (s / rec_T + 1) * rec_T - s);
+        if ((rc = recorder_before(rec, e->stream, s + K))) return rc;
+        rc = run_steps(e->stream, &e->gexec, &e->gexec_steps, &e->hi, e->di, 1, e->w, K, use_graph, chunk, &e->kt,
+                       &e->gexec_k, &e->gexec_k_steps);
+        if (rc) return rc;
+        if ((s + K) % rec_T == 0 && (rc = recorder_window(rec, e->stream, e->hi.ring, s + K))) return rc;
+        done += K;
+    }
     if (train) {
         const int R = e->hi.R;
         const int row0 = (int)(((first_out_step % R) + R) % R);
@@ -768,6 +821,11 @@ int engine_run_output(Engine *e, int n_steps, int use_graph, int graph_steps, in
         *train = e->out_h;
     }
     return engine_finish(e);
+}
+
+int engine_record_sync(Engine *e, Recorder *rec)
+{
+    return rec ? recorder_wait_all(rec, e->stream) : SCAFFOLD_OK;
 }
 
 int engine_timer(Engine *e, int on, double *ms, int64_t *launches)
@@ -789,7 +847,8 @@ int engine_time(Engine *e, int n_steps, int use_graph, int graph_steps, double *
     const int chunk = chunk_for(graph_steps, e->hi.R);
     if (!ms_kernel_mean) {
         HIP_TRY(hipEventRecord(a, e->stream));
-        int rc = run_steps(e->stream, &e->gexec, &e->gexec_steps, &e->hi, e->di, 1, e->w, n_steps, use_graph, chunk);
+        int rc = run_steps(e->stream, &e->gexec, &e->gexec_steps, &e->hi, e->di, 1, e->w, n_steps, use_graph, chunk,
+                           &e->kt, &e->gexec_k, &e->gexec_k_steps);
         if (rc) return rc;
         HIP_TRY(hipEventRecord(b, e->stream));
     } else {
@@ -830,6 +889,8 @@ int engine_time(Engine *e, int n_steps, int use_graph, int graph_steps, double *
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, a, b));
     if (ms_total) *ms_total = ms;
+    int rc = engine_finish(e);  // the step-kernel timer's events of this run
+    if (rc) return rc;
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     return SCAFFOLD_OK;
@@ -1028,7 +1089,8 @@ int batch_build(Batch **out, std::vector<Engine *> &engines, const std::vector<c
     }
     int rc;
     if ((rc = dupload(b->allocs, nullptr, &b->di, insts))) return bail(rc);
-    if ((rc = upload_work(b->allocs, nullptr, b->w, tw, lanes, stims, tt, lane_win, team_win, stage != 0, 0)))
+    if ((rc = upload_work(b->allocs, nullptr, b->w, tw, lanes, stims, tt, lane_win, team_win, stage != 0,
+                          plans[0]->stage_ratio, 0)))
         return bail(rc);
     for (const ScInstance &x : insts) b->w.max_groups = std::max(b->w.max_groups, x.n_groups);
     b->w.nontemporal = nt_all || algo > ((int64_t)64 << 20);

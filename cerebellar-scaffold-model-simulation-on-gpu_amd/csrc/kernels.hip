@@ -136,6 +136,14 @@ __device__ __forceinline__ void lds_load16(const u32x4 *src, u32x4 *dst)
 // wavefront index), so descriptor fields are fetched with scalar loads and
 // branches on them are scalar branches.
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+template <typename T>
+__device__ __forceinline__ T *uniform_ptr(T *p)
+{
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    return reinterpret_cast<T *>(((uint64_t)hi << 32) | lo);
+}
 
 // Multipliers are loaded as raw bits, 4 or 8 entries per load, and decoded only
 // after a batch of loads has arrived.
@@ -253,6 +261,13 @@ __device__ __forceinline__ float wval8(const Raw8<WP> &r, int i)
     } else {
         return 1.f;
     }
+}
+
+// Ring row of launch offset k of a run (the run's first row ctl->row < R, k < R).
+__device__ __forceinline__ int run_row(const ScInstance &I, int k)
+{
+    const int r = I.ctl->row + k;
+    return r >= I.R ? r - I.R : r;
 }
 
 // Ring row of the step `back` steps before the step whose row is `row` (back < R).
@@ -439,7 +454,9 @@ __device__ __forceinline__ void lane_group(const ScLaneTask &t, const ScInstance
     for (int s = 0; s < SC_MAX_SLICES; ++s) {
         const bool on = s < n_sl;
         const int width = on ? t.sl[s].width : 0;
-        row[s] = gbl(reinterpret_cast<const uint32_t *>(I.ring + (int64_t)(on ? row_back(row_n, t.sl[s].delay, I.R) : 0) * I.Wr));
+        // wave-uniform (one instance per task): held in scalar registers
+        const int rb = uniform(on ? row_back(row_n, t.sl[s].delay, I.R) : 0);
+        row[s] = gbl(reinterpret_cast<const uint32_t *>(I.ring + (int64_t)rb * I.Wr));
         const void *val = t.sl[s].val;
         raw[s].w[0] = 0u;
         if (WP != SC_W_UNIT && val && width > 0) raw[s] = wraw4<WP, NT>(val, lane);
@@ -537,7 +554,8 @@ __device__ __forceinline__ TeamHead team_head(const ScTeamTask *T)
 
 template <int WP, int U, bool IDX16, bool STAGED, bool NT>
 __device__ __forceinline__ void team_gather(const TeamHead &T, gptr<uint32_t> ring32, const uint32_t *win,
-                                            const uint32_t *rowoff, uint32_t zb, int tid, int TT, double *acc)
+                                            const uint32_t *rowoff, uint32_t gmask, uint32_t zb, int tid, int TT,
+                                            double *acc)
 {
     const int len = T.len;
     const void *val = T.val;
@@ -550,16 +568,23 @@ __device__ __forceinline__ void team_gather(const TeamHead &T, gptr<uint32_t> ri
             const u32x4 x = ldg<NT>(col + b);
             const uint32_t h = ldg<NT>(hdr + b);
             const Raw8<WP> raw = wraw8<WP, NT>(val, b);
-            const uint32_t base = h & SC_BIT_MASK, ro = rowoff[h >> SC_Q_SHIFT];
+            const uint32_t q = h >> SC_Q_SHIFT, base = h & SC_BIT_MASK, ro = rowoff[q];
             uint32_t bit[8], s[8];
+            bool pad[8];
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
                 const uint32_t o = (x[i >> 1] >> (16 * (i & 1))) & 0xFFFFu;
-                const bool pad = o == SC_PAD16;
-                bit[i] = pad ? zb : base + o;  // padding reads the zero word
-                const uint32_t wi = ro + (bit[i] >> 5);
-                if constexpr (STAGED) s[i] = win[pad ? 0u : wi];
-                else s[i] = ring32[wi];
+                pad[i] = o == SC_PAD16;
+                bit[i] = pad[i] ? zb : base + o;  // padding reads the zero word
+            }
+            // a staged block reads its chunk's words from the LDS window, unless the
+            // chunk's projection was left in memory (gmask; one projection per chunk)
+            if (!STAGED || (gmask != 0u && ((gmask >> q) & 1u))) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) s[i] = ring32[ro + (bit[i] >> 5)];
+            } else {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) s[i] = win[pad[i] ? 0u : ro + (bit[i] >> 5)];
             }
 #pragma unroll
             for (int i = 0; i < 8; ++i)
@@ -717,21 +742,22 @@ __device__ __forceinline__ const ScInstance &inst_of(const ScInstance &I0, const
 // Gather of one team: storage form and multiplier type of its row.
 template <bool IDX16, bool STAGED, bool NT>
 __device__ __forceinline__ void team_dispatch(const TeamHead &T, gptr<uint32_t> ring32, const uint32_t *win,
-                                              const uint32_t *rowoff, uint32_t zb, int tid, int TT, double *acc)
+                                              const uint32_t *rowoff, uint32_t gm, uint32_t zb, int tid, int TT,
+                                              double *acc)
 {
     if (IDX16 && T.hdr) {
         switch (T.wprec) {
-        case SC_W_F32: team_gather<SC_W_F32, 8, true, STAGED, NT>(T, ring32, win, rowoff, zb, tid, TT, acc); break;
-        case SC_W_BF16: team_gather<SC_W_BF16, 8, true, STAGED, NT>(T, ring32, win, rowoff, zb, tid, TT, acc); break;
-        case SC_W_FP8: team_gather<SC_W_FP8, 8, true, STAGED, NT>(T, ring32, win, rowoff, zb, tid, TT, acc); break;
-        default: team_gather<SC_W_UNIT, 8, true, STAGED, NT>(T, ring32, win, rowoff, zb, tid, TT, acc); break;
+        case SC_W_F32: team_gather<SC_W_F32, 8, true, STAGED, NT>(T, ring32, win, rowoff, gm, zb, tid, TT, acc); break;
+        case SC_W_BF16: team_gather<SC_W_BF16, 8, true, STAGED, NT>(T, ring32, win, rowoff, gm, zb, tid, TT, acc); break;
+        case SC_W_FP8: team_gather<SC_W_FP8, 8, true, STAGED, NT>(T, ring32, win, rowoff, gm, zb, tid, TT, acc); break;
+        default: team_gather<SC_W_UNIT, 8, true, STAGED, NT>(T, ring32, win, rowoff, gm, zb, tid, TT, acc); break;
         }
     } else {
         switch (T.wprec) {
-        case SC_W_F32: team_gather<SC_W_F32, 8, false, STAGED, NT>(T, ring32, win, rowoff, zb, tid, TT, acc); break;
-        case SC_W_BF16: team_gather<SC_W_BF16, 8, false, STAGED, NT>(T, ring32, win, rowoff, zb, tid, TT, acc); break;
-        case SC_W_FP8: team_gather<SC_W_FP8, 8, false, STAGED, NT>(T, ring32, win, rowoff, zb, tid, TT, acc); break;
-        default: team_gather<SC_W_UNIT, 8, false, STAGED, NT>(T, ring32, win, rowoff, zb, tid, TT, acc); break;
+        case SC_W_F32: team_gather<SC_W_F32, 8, false, STAGED, NT>(T, ring32, win, rowoff, gm, zb, tid, TT, acc); break;
+        case SC_W_BF16: team_gather<SC_W_BF16, 8, false, STAGED, NT>(T, ring32, win, rowoff, gm, zb, tid, TT, acc); break;
+        case SC_W_FP8: team_gather<SC_W_FP8, 8, false, STAGED, NT>(T, ring32, win, rowoff, gm, zb, tid, TT, acc); break;
+        default: team_gather<SC_W_UNIT, 8, false, STAGED, NT>(T, ring32, win, rowoff, gm, zb, tid, TT, acc); break;
         }
     }
 }
@@ -764,7 +790,7 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
     const bool gather_updates = count > 1;
     const ScInstance &I = inst_of<SINGLE>(I0, insts, h.inst);
     int row_n = row_single;
-    if (!SINGLE && active) row_n = (I.ctl->row + k) % I.R;
+    if (!SINGLE && active) row_n = uniform(run_row(I, k));
     float u = 0.f;
     float2 gg = {0.f, 0.f};
     int32_t rf = 0;
@@ -784,8 +810,9 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
             // word offset of each projection's ring row: in global memory, or in a
             // staged block the LDS word that stands for word 0 of the row
             rowoff[team][lane] = lane >= h.n_in ? 0u
-                                 : win ? (uint32_t)pin.lds
-                                       : (uint32_t)row_back(row_n, pin.delay, I.R) * (uint32_t)(2 * I.Wr);
+                                 : (win && pin.lds != SC_LDS_GLOBAL)
+                                     ? (uint32_t)pin.lds
+                                     : (uint32_t)row_back(row_n, pin.delay, I.R) * (uint32_t)(2 * I.Wr);
             if (lane < h.n_in) {  // the factors that turn row sums into dg
                 fac[team][0][lane] = pin.scale;
                 fac[team][1][lane] = pin.W;
@@ -794,13 +821,16 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
         }
         if (lane < SC_POP_WORDS) par[team][lane] = pv;
     }
+    // projections of a staged block left in memory (their windows did not fit)
+    const uint32_t gmask = win ? (uint32_t)__ballot(lane < h.n_in && pin.lds == SC_LDS_GLOBAL) : 0u;
     __syncthreads();  // staged block: the windows are complete past this barrier
     if (active) {
-        gptr<uint32_t> ring32 = gbl(reinterpret_cast<const uint32_t *>(I.ring));
+        // the team's network is the same for the whole wavefront
+        gptr<uint32_t> ring32 = gbl(reinterpret_cast<const uint32_t *>(uniform_ptr(I.ring)));
         const int tid = tw * SC_WAVE + lane, TT = t * SC_WAVE;
-        const uint32_t zb = (uint32_t)(I.Wr - 1) * SC_WAVE;
-        if (win) team_dispatch<IDX16, true, NT>(h, ring32, win, rowoff[team], zb, tid, TT, acc[team]);
-        else team_dispatch<IDX16, false, NT>(h, ring32, win, rowoff[team], zb, tid, TT, acc[team]);
+        const uint32_t zb = (uint32_t)uniform((I.Wr - 1) * SC_WAVE);
+        if (win) team_dispatch<IDX16, true, NT>(h, ring32, win, rowoff[team], gmask, zb, tid, TT, acc[team]);
+        else team_dispatch<IDX16, false, NT>(h, ring32, win, rowoff[team], 0u, zb, tid, TT, acc[team]);
     }
     __syncthreads();
     float dge = 0.f, dgi = 0.f;
@@ -928,7 +958,7 @@ __device__ __forceinline__ const uint32_t *stage_windows(uint32_t sd, uint32_t *
         const int n4 = __builtin_amdgcn_readlane((int)sd, base + F_NW) >> 2;
         const int o4 = __builtin_amdgcn_readlane((int)sd, base + F_OFF) >> 2;
         const ScInstance &I = inst_of<SINGLE>(I0, insts, inst);
-        const int rn = SINGLE ? row_single : (I.ctl->row + k) % I.R;
+        const int rn = SINGLE ? row_single : run_row(I, k);
         const u32x4 *src = reinterpret_cast<const u32x4 *>(
             reinterpret_cast<const uint32_t *>(I.ring + (int64_t)row_back(rn, delay, I.R) * I.Wr) + w0);
         for (int cb = wave * SC_WAVE; cb < n4; cb += SC_BLOCK)
@@ -1000,14 +1030,14 @@ __device__ __forceinline__ void step_body(const ScInstance &I0, const ScInstance
     if (c == SC_CLASS_LANE) {
         const ScLaneTask &t = lanes[idx];
         const ScInstance &I = inst_of<SINGLE>(I0, insts, t.inst);
-        const int row_n = SINGLE ? row_single : (I.ctl->row + k) % I.R;
+        const int row_n = SINGLE ? row_single : uniform(run_row(I, k));
         const ScPop &P = I.pops[t.pop];
         if (staged) lane_dispatch<IDX16, true, NT>(t, I, P, row_n, stage, lane);
         else lane_dispatch<IDX16, false, NT>(t, I, P, row_n, stage, lane);
     } else {  // stimulus blocks (never staged)
         const ScStimTask t = stt[idx];
         const ScInstance &I = inst_of<SINGLE>(I0, insts, t.inst);
-        const int row_n = SINGLE ? row_single : (I.ctl->row + k) % I.R;
+        const int row_n = SINGLE ? row_single : run_row(I, k);
         const int d = t.group * SC_WAVE + lane;
         int spk = 0;
         if (lane < t.nvalid) {
@@ -1059,7 +1089,7 @@ __global__ void __launch_bounds__(256) k_stats_zero(ScInstance I0, const ScInsta
 {
     const int inst = blockIdx.x / K, kk = blockIdx.x - inst * K;
     const ScInstance &I = inst_of<SINGLE>(I0, insts, inst);
-    const int row = (I.ctl->row + kk) % I.R;
+    const int row = run_row(I, kk);
     if ((int)threadIdx.x < I.n_pops) I.counts[(int64_t)row * I.n_pops + threadIdx.x] = 0;
     if (threadIdx.x == 0) I.events[row] = 0ull;
 }
@@ -1076,7 +1106,7 @@ __global__ void __launch_bounds__(256) k_stats(ScInstance I0, const ScInstance *
     const int g1 = min(I.n_groups, g0 + SC_STATS_CHUNK);
     if (threadIdx.x <= SC_MAX_POPS) acc[threadIdx.x] = 0ull;
     __syncthreads();
-    const int row = (I.ctl->row + kk) % I.R;
+    const int row = run_row(I, kk);
     const uint64_t *__restrict__ w = I.ring + (int64_t)row * I.Wr;
     unsigned long long ev = 0ull;
     // popcount into the population's counter, then the out-degrees of the set

@@ -67,11 +67,12 @@ struct Options {
     int weight_precision = SCAFFOLD_WEIGHTS_F32;
     int index_bits = 32;
     int lane_max_fanin = 48;
-    int lane_synapses = 16;   // team mode: target synapses per lane of a neuron's team
+    int lane_synapses = 0;    // team mode: target synapses per lane of a neuron's team (0: by network size)
     int graph = 1;
     int graph_steps = 25;
     int reset_g = 0;
     int stage_spikes = 1;
+    int stage_ratio = 0;      // stage a block when window words <= stage_ratio x its gathers' 64-byte segments (0: by size)
     int nontemporal = -1;     // synapse streams as nontemporal loads: -1 when they exceed the L2, 0 never, 1 always
 };
 
@@ -123,6 +124,7 @@ struct TeamPop {
     // (lo > hi: none); per neuron: estimated 64-byte segments its gathers touch
     std::vector<uint32_t> q_lo, q_hi;
     std::vector<int64_t> row_seg;
+    std::vector<int64_t> q_seg;     // num x n_in: the same per incoming projection
 };
 
 // A plasticity rule lowered to the device layout: per synapse of the target
@@ -160,6 +162,8 @@ struct Plan {
     int64_t nnz = 0;
     int64_t algo_bytes = 0;
     int stage = 1;  // stage spike windows in LDS where the plan finds it pays
+    int stage_ratio = 8;
+    int lane_synapses = 8;  // team width target in effect
     bool nontemporal = false;  // load the synapse streams nontemporal
     int64_t syn16 = 0;  // synapses stored as 16-bit column offsets
     int max_delay = 0;
@@ -171,6 +175,7 @@ struct Plan {
 };
 
 struct Engine;  // device side (engine.hip)
+struct Recorder;  // spike files (recorder.cpp)
 
 // Engine entry points used by the C ABI layer.
 int engine_create(Engine **e, Plan &plan);
@@ -183,8 +188,13 @@ int engine_run(Engine *e, int n_steps, int use_graph, int graph_steps);
 // engine_run, then (train != nullptr) the output spike train of the T_out steps
 // from first_out_step on, gathered on the device and copied with one D2H copy into
 // the engine's pinned buffer (*train), valid until the next call.
+// With a recorder, the steps run in pieces that end at multiples of the
+// recorder's window (step numbers counted from step0 = the first step of the
+// call), each completed window handed to the recorder.
 int engine_run_output(Engine *e, int n_steps, int use_graph, int graph_steps, int64_t first_out_step,
-                      const uint8_t **train);
+                      const uint8_t **train, Recorder *rec = nullptr, int rec_T = 0, int64_t step0 = 0);
+// Orders the engine's stream after every copy handed to the recorder.
+int engine_record_sync(Engine *e, Recorder *rec);
 // Step-kernel timer: on 1 starts (clears) it, 0 stops it, -1 leaves it; returns
 // the measured milliseconds and the step launches they cover.
 int engine_timer(Engine *e, int on, double *ms, int64_t *launches);
@@ -234,6 +244,7 @@ struct scaffold_env {
     sc::Engine *eng = nullptr;
     std::vector<uint64_t> tmp_words;
     int in_batches = 0;           // batches referencing this environment's device state
+    sc::Recorder *rec = nullptr;  // spike files written as the run goes (scaffold_record_spikes)
 };
 
 namespace sc {

@@ -330,6 +330,15 @@ int plan_build(scaffold_env *env, Plan &P)
         weight_bytes += nnz * wbytes_of(sp.wprec);
         for (int64_t e = 0; e < nnz; ++e) P.outdeg[pre.dev_base + src.col[e]]++;
     }
+    // Layout defaults by size.  A step whose streams fit the L2 twice over (at
+    // most 64 MB) is bound by the latency of its slowest block: wide teams (8
+    // synapses per lane) shorten the long rows' gathers.  Above that the blocks
+    // queue for the CUs and the step is bound by their throughput: narrow teams
+    // (32 per lane) carry more synapses per block's fixed costs, and blocks stage
+    // windows up to 32 words per 64-byte segment their gathers would touch.
+    const bool large = P.nnz * (want16 ? 2 : 4) + weight_bytes + (int64_t)ref * 32 > ((int64_t)64 << 20);
+    P.lane_synapses = opt.lane_synapses > 0 ? opt.lane_synapses : large ? 32 : 8;
+    P.stage_ratio = opt.stage_ratio > 0 ? opt.stage_ratio : large ? 32 : 8;
     // write one encoded multiplier
     auto put_w = [&](std::vector<uint8_t> &out, int64_t pos, const ScProj &sp, float x) {
         if (sp.wprec == SC_W_F32) memcpy(&out[pos * 4], &x, 4);
@@ -513,6 +522,7 @@ int plan_build(scaffold_env *env, Plan &P)
         T.q_lo.assign((size_t)d.num * d.n_in, UINT32_MAX);
         T.q_hi.assign((size_t)d.num * d.n_in, 0);
         T.row_seg.assign(d.num, 0);
+        T.q_seg.assign((size_t)d.num * d.n_in, 0);
         for (int j = 0; j < d.num; ++j)
             for (int q = 0; q < d.n_in; ++q) {
                 const int64_t s0 = T.row_ptr[j] + (q ? T.seg_end[(size_t)j * d.n_in + q - 1] : 0);
@@ -526,7 +536,9 @@ int plan_build(scaffold_env *env, Plan &P)
                     }
                     qlo = std::min(qlo, lo);
                     qhi = std::max(qhi, hi);
-                    T.row_seg[j] += std::min<int64_t>(SC_WAVE, (hi - lo) / 512 + 1);
+                    const int64_t sg = std::min<int64_t>(SC_WAVE, (hi - lo) / 512 + 1);
+                    T.row_seg[j] += sg;
+                    T.q_seg[(size_t)j * d.n_in + q] += sg;
                 }
             }
         if (want16) {
@@ -579,7 +591,7 @@ int plan_build(scaffold_env *env, Plan &P)
             std::vector<uint32_t>().swap(T.col);
         }
         // team width: about lane_synapses synapses per lane for the mean row
-        const double per_lane = (double)tot / std::max(1, d.num) / SC_WAVE / std::max(1, opt.lane_synapses);
+        const double per_lane = (double)tot / std::max(1, d.num) / SC_WAVE / std::max(1, P.lane_synapses);
         int t = 1;
         while (t < SC_MAX_TEAM && t < per_lane) t <<= 1;
         T.team_waves = t;

@@ -218,7 +218,10 @@ int scaffold_set_input_stim_ids(scaffold_env *env, int neuron_index,
  *   "lane_max_fanin"    populations whose mean total fan-in is <= this are
  *                       gathered one lane per post neuron (sliced ELL); others by a
  *                       team of wavefronts per post neuron (merged CSR row)
- *   "lane_synapses"     team mode: target synapses per lane (sets the team width)
+ *   "lane_synapses"     team mode: target synapses per lane (sets the team width);
+ *                       0 (default): 8 when one step's synapse and state streams
+ *                       fit in 64 MB (the step is latency-bound: wide teams),
+ *                       else 32 (throughput-bound: more synapses per block)
  *   "graph"             1: replay captured hipGraphs of the step kernels
  *   "graph_steps"       exchange steps per captured graph
  *   "reset_g"           1: also zero conductances in scaffold_reset_network_env
@@ -226,6 +229,10 @@ int scaffold_set_input_stim_ids(scaffold_env *env, int neuron_index,
  *   "stage_spikes"      1 (default): blocks whose gathers scatter over a bounded
  *                       range of presynaptic neurons first copy those spike words
  *                       into LDS; 0: every gather reads the spike ring in memory
+ *   "stage_ratio"       a block is staged when its windows fit in LDS (64 KiB) and
+ *                       hold at most this many 32-bit words per 64-byte segment
+ *                       its gathers would touch; 0 (default): 8 up to 64 MB of
+ *                       streams per step, 32 above
  *   "nontemporal"       -1 (default): the synapse streams (columns, multipliers)
  *                       are loaded nontemporal when one step's exceed 64 MB, so
  *                       they do not evict descriptors and spike rows from L2;
@@ -330,6 +337,8 @@ typedef struct {
     int64_t  synapses_16bit;       /* synapses stored as 16-bit column offsets */
     int32_t  n_blocks;             /* blocks of the step kernel                */
     int32_t  n_staged_blocks;      /* of which stage spike windows in LDS      */
+    int32_t  lane_synapses;        /* team width target in effect (option or by size) */
+    int32_t  stage_ratio;          /* staging threshold in effect (option or by size) */
 } scaffold_info;
 int scaffold_get_info(scaffold_env *env, scaffold_info *out);
 
@@ -421,6 +430,26 @@ int scaffold_write_spike_text(scaffold_env *env, int step0, int n, const char *f
  * The reference keeps them in a file of their own (spike_host*.dat) and appends
  * it to the device file at the end of a trial (GPUScaffoldSimulation.cu:826). */
 int scaffold_write_output_spike_text(scaffold_env *env, int step0, int n, const char *filename);
+/* The reference's PRINT mode (printer threads print_spike_train,
+ * GPUScaffoldSimulation.cu:56-89, started at :404-432; host thread
+ * SimulationOnCPU :7-53; merge in ResetNetworkEnv :809-846): from this call on,
+ * scaffold_loop_n_steps writes the spike files into directory `dir` (the
+ * reference's "spike_output") while the simulation runs.  After each step that
+ * completes a window of T_cpu_sim steps the window's rows are copied to pinned
+ * host memory on a stream of their own and a writer thread formats them, so the
+ * steps go on meanwhile.  Window h (steps [h*T_cpu_sim, (h+1)*T_cpu_sim) of the
+ * trial) belongs to thread k = (h / (T_print/T_cpu_sim)) % 2:
+ *   spike_dev0_<k>.dat  lines of scaffold_write_spike_text, written per whole
+ *                       T_print window once it completes;
+ *   spike_host<k>.dat   lines of scaffold_write_output_spike_text, per window.
+ * scaffold_reset_network_env(env, trial) then writes spike_dev0.dat (the two
+ * device files) and spike_result_trial<trial>.dat (spike_dev0.dat, spike_host0.dat,
+ * spike_host1.dat), as the reference's `cat` calls do, and starts the four files
+ * empty for the next trial.  Lines of a T_print window left incomplete at the
+ * reset are dropped.  Steps advanced by scaffold_time_steps are not recorded.
+ * dir == NULL stops recording (pending windows are written first).  Rejected for
+ * an environment in a batch. */
+int scaffold_record_spikes(scaffold_env *env, const char *dir);
 
 /* Timing helper for benchmarks: advance n_steps (continuing the trial) and
  * return the device time measured with HIP events on the engine stream, and the
@@ -443,11 +472,12 @@ int scaffold_step_timer(scaffold_env *env, int on, double *ms, int64_t *launches
  * this network under candidate layouts -- lane_max_fanin (which populations use
  * one lane per neuron over sliced-ELL slices, i.e. the ELL width) in {16, 48, 128}
  * and lane_synapses (synapses per lane, i.e. wavefronts per team-mode neuron) in
- * {8, 16, 32} -- times n_steps step kernels of each after 50 warm-up steps, then
- * tries, one at a time on the fastest layout, the other column index width
- * (index_bits 32 / 16), the other stage_spikes setting and graph_steps 50, and
- * sets the winning options on env (12 trials).  The network's own state, seeds
- * and counters are untouched. */
+ * {8, 16, 32}, besides the environment's own options -- times n_steps step
+ * kernels of each after 50 warm-up steps, then tries, one at a time on the
+ * fastest layout, the other column index width (index_bits 32 / 16), the other
+ * stage_spikes setting and graph_steps 50, and sets the winning options on env
+ * (13 trials; ms_default is the time with the environment's own options).  The
+ * network's own state, seeds and counters are untouched. */
 typedef struct {
     int32_t lane_max_fanin, lane_synapses, graph_steps;
     int32_t n_tried;
@@ -456,6 +486,13 @@ typedef struct {
     int32_t index_bits, stage_spikes;
 } scaffold_tune_result;
 int scaffold_autotune(scaffold_env *env, int n_steps, scaffold_tune_result *out);
+/* One trial of the autotuner under the environment's current options (no
+ * counterpart in the reference): before scaffold_initialize_network_env, builds a
+ * private engine of the described network, initializes it, runs 50 warm-up steps
+ * and times n_steps graph-replayed steps; *ms_per_step is their device time per
+ * step, *kernel_ms (may be NULL) the mean step-kernel duration.  The environment
+ * is left as it was. */
+int scaffold_time_layout(scaffold_env *env, int n_steps, double *ms_per_step, double *kernel_ms);
 
 /* --------------------------------------------------------------- batching */
 /* Many initialized environments of uneven size advanced together: every step

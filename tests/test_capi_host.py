@@ -228,7 +228,7 @@ def test_stimulus_registration(pkg, tmp_path):
 def test_options(pkg):
     env = new_env(pkg, 0)
     defaults = {"weight_precision": 0, "index_bits": 32, "graph": 1, "graph_steps": 25, "lane_max_fanin": 48,
-                "lane_synapses": 16}
+                "lane_synapses": 0, "stage_ratio": 0}
     for k, v in defaults.items():
         assert env.get_option(k) == v, k
     env.set_option("weight_precision", pkg.WEIGHTS_BF16)
@@ -356,7 +356,7 @@ def test_autotune_argument_checks(pkg):
             env.autotune(10)
         assert e.value.code in (pkg.ERR_NO_DEVICE, pkg.ERR_HIP)
         # a failed tuning leaves the options as they were
-        assert env.get_option("lane_synapses") == 16 and env.get_option("lane_max_fanin") == 48
+        assert env.get_option("lane_synapses") == 0 and env.get_option("lane_max_fanin") == 48
 
 
 def test_plasticity_registration(pkg):

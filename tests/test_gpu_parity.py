@@ -530,12 +530,17 @@ def reference_text_lines(env, ref, steps, output):
     (float step, global id, ReverseNeuronTypeID[...]) for the OUTPUT population;
     ReverseNeuronTypeID inverts NeuronTypeID (:179-183), so the last column is the
     NeuronType enum the population was registered with (init_params.cu:377-511)."""
+    return text_lines_of_rows(env, {s: ref.spike_row(s) for s in steps}, steps, output)
+
+
+def text_lines_of_rows(env, rows, steps, output):
+    """reference_text_lines over oracle rows kept in a dict by step."""
     info = env.info()
     pops = [env.population(i) for i in range(info.n_populations)]
     reverse = {i: p.type for i, p in enumerate(pops)}     # internal index -> NeuronType
     lines = []
     for s in steps:
-        row = ref.spike_row(s)
+        row = rows[s]
         for i, p in enumerate(pops):
             if (p.device_type == 2) != output:
                 continue
@@ -603,6 +608,78 @@ def test_output_train_fired_and_text(pkg, orc, tmp_path):
     assert all(ln.split("\t")[2] == str(env.population(dcn[0]).type) for ln in got)   # dcn_cell = 6
     with pytest.raises(pkg.ScaffoldError):
         env.write_spike_text(cur - R, 2, dev_file)
+    env.finalize()
+
+
+def expected_trial_files(env, rows, T, T_print, n_steps):
+    """The reference's PRINT-mode files of one trial of n_steps steps, from the
+    oracle's rows.  Window h of T_cpu_sim steps belongs to printer/host thread
+    k = (h / (T_print / T_cpu_sim)) % 2 (GPUScaffoldSimulation.cu:768-783).  The
+    printer thread k prints its T_print window once all of it has arrived
+    (:66-86), the host thread k prints the OUTPUT lines of each window
+    (SimulationOnCPU, :7-53); ResetNetworkEnv concatenates spike_dev0_*.dat into
+    spike_dev0.dat and then spike_dev?.dat spike_host?.dat into
+    spike_result_trial<trial>.dat (:818-827)."""
+    dev, host = ["", ""], ["", ""]
+    per = T_print // T
+    for h in range(n_steps // T):
+        k = (h // per) % 2
+        lines = text_lines_of_rows(env, rows, range(h * T, (h + 1) * T), output=True)
+        host[k] += "".join(ln + "\n" for ln in lines)
+    for w in range(n_steps // T_print):
+        lines = text_lines_of_rows(env, rows, range(w * T_print, (w + 1) * T_print), output=False)
+        dev[w % 2] += "".join(ln + "\n" for ln in lines)
+    return dev, host
+
+
+@pytest.mark.parametrize("T_print,T,n_steps,calls", [
+    (50, 25, 250, None),                                   # main.cu's loop: one call per window
+    (25, 25, 200, (7, 11, 7, 30, 3, 1, 48, 25, 50, 18)),  # ring of 2*T_cpu_sim rows, calls across windows
+    (100, 25, 275, None),                                  # the last T_print window incomplete
+])
+def test_recorded_spike_files(pkg, orc, tmp_path, T_print, T, n_steps, calls):
+    """PRINT mode: scaffold_record_spikes writes the printer- and host-thread files
+    while the trial runs, ResetNetworkEnv merges them; two trials (SetInputFreq
+    before each call, as main.cu:40-55) against the files rendered from the oracle's
+    raster.  After the last reset the four files are empty; recording stopped
+    writes nothing."""
+    env = new_env(pkg, 0)
+    env.initialize(T_print, T)
+    ref = orc.OracleSim.from_env(env)
+    ref.initialize()
+    info = env.info()
+    assert info.T_print == T_print and info.T_cpu_sim == T
+    env.record_spikes(tmp_path)
+    names = ["spike_dev0_0.dat", "spike_dev0_1.dat", "spike_host0.dat", "spike_host1.dat"]
+    assert all((tmp_path / n).exists() for n in names)
+    for trial in range(2):
+        rows = {}
+        sizes = calls if calls else [T] * (n_steps // T)
+        assert sum(sizes) == n_steps
+        t = 0
+        for n in sizes:
+            f = [4.0 + trial + t / 100.0, 6.0]
+            env.set_input_freq(f)
+            ref.set_freq(f)
+            env.loop_n_steps(t, n, trial, want_output=True)
+            for s in range(t, t + n):   # one step at a time: the oracle keeps a ring of rows too
+                ref.run(s, 1)
+                rows[s] = ref.spike_row(s)
+            t += n
+        for s in range(t - 20, t):
+            assert np.array_equal(env.spike_row(s), rows[s]), f"trial {trial} step {s}"
+        env.reset(trial)
+        ref.reset()
+        dev, host = expected_trial_files(env, rows, T, T_print, n_steps)
+        assert dev[0].count("\n") > 500 and host[0] + host[1]
+        got_dev = (tmp_path / "spike_dev0.dat").read_text()
+        assert got_dev == dev[0] + dev[1], f"trial {trial}: spike_dev0.dat"
+        got = (tmp_path / f"spike_result_trial{trial}.dat").read_text()
+        assert got == dev[0] + dev[1] + host[0] + host[1], f"trial {trial}: merged file"
+        assert all((tmp_path / n).read_text() == "" for n in names), "files start empty after the reset"
+    env.record_spikes(None)
+    env.loop_n_steps(0, T, 2, want_output=False)
+    assert all((tmp_path / n).read_text() == "" for n in names)
     env.finalize()
 
 
@@ -762,12 +839,12 @@ def test_autotune_then_bit_exact(pkg, orc):
     are among the candidates."""
     env = new_env(pkg, 0)
     r = env.autotune(50)
-    assert r["n_tried"] == 12 and r["ms_per_step"] > 0 and r["ms_default"] > 0
+    assert r["n_tried"] == 13 and r["ms_per_step"] > 0 and r["ms_default"] > 0
     assert env.get_option("index_bits") == r["index_bits"] in (16, 32)
     assert env.get_option("stage_spikes") == r["stage_spikes"] in (0, 1)
     assert r["ms_per_step"] <= r["ms_default"]
     assert env.get_option("lane_max_fanin") == r["lane_max_fanin"] in (16, 48, 128)
-    assert env.get_option("lane_synapses") == r["lane_synapses"] in (8, 16, 32)
+    assert env.get_option("lane_synapses") == r["lane_synapses"] in (0, 8, 16, 32)
     assert env.get_option("graph_steps") == r["graph_steps"] in (25, 50)
     env.initialize(50, 25)
     ref = orc.OracleSim.from_env(env)
