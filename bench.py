@@ -187,8 +187,8 @@ def variant(pkg, iterations, warmup, preset=1, plastic=False, connectome=None, *
     return r
 
 
-def batched(pkg, n_inst, steps, warmup, threads):
-    """Config 3: 64 independent instances of uneven size (0.25x..4x of the 96.7k
+def batched(pkg, n_inst, steps, warmup, threads, layout=None):
+    """The batched form: 64 independent instances of uneven size (0.25x..4x of the 96.7k
     scaffold), each with its own seeds and mossy rate U(1, 60) Hz, BF16
     multipliers, advanced together by one step kernel per exchange step
     (hipGraph replays), against sequential plain runs of the same instances: a
@@ -201,13 +201,17 @@ def batched(pkg, n_inst, steps, warmup, threads):
     # autotune the layout on one instance of the largest scale (4x; the 4x
     # instances hold 42 % of the batch's synapses, and at that size the step is
     # throughput-bound like the batch) and use it for all
-    probe = make_env(pkg, 1, scale=max(scales), burst_n_mf=max(1, int(round(50 * max(scales)))),
-                     mf_rate_hz=float(np.median(rates)))
-    probe.set_option("weight_precision", pkg.WEIGHTS_BF16)
-    tuned = probe.autotune(100)
-    probe.finalize()
-    log(f"batch: autotuned {tuned}")
-    layout = {k: tuned[k] for k in ("lane_max_fanin", "lane_synapses", "graph_steps", "index_bits", "stage_spikes")}
+    # (a given layout skips the tuning: profiles need the same kernels every run)
+    if layout is None:
+        probe = make_env(pkg, 1, scale=max(scales), burst_n_mf=max(1, int(round(50 * max(scales)))),
+                         mf_rate_hz=float(np.median(rates)))
+        probe.set_option("weight_precision", pkg.WEIGHTS_BF16)
+        tuned = probe.autotune(100)
+        probe.finalize()
+        log(f"batch: autotuned {tuned}")
+        layout = {k: tuned[k] for k in ("lane_max_fanin", "lane_synapses", "graph_steps", "index_bits", "stage_spikes")}
+    else:
+        tuned = {"given": dict(layout)}
     envs = [[None] * len(scales), [None] * len(scales)]
     errs = []
 
@@ -340,7 +344,11 @@ def main():
     ap.add_argument("--scaled-index-bits", type=int, default=32, choices=(32, 16),
                     help="column index width of the --only-scaled run")
     ap.add_argument("--only-batched", action="store_true",
-                    help="only run the batched leg (config 3) and print its result")
+                    help="only run the batched leg (64 instances of 0.25x-4x) and print its result")
+    ap.add_argument("--batch-layout", type=json.loads, default=None,
+                    help="layout options of the batched leg as JSON (skips the autotuner), e.g. "
+                         "'{\"lane_max_fanin\": 128, \"lane_synapses\": 32, \"graph_steps\": 50, "
+                         "\"index_bits\": 16, \"stage_spikes\": 1}'")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run the calibration copy kernel on this many MiB and exit")
     args = ap.parse_args()
@@ -355,7 +363,7 @@ def main():
         print(json.dumps(scaled(pkg, args.only_scaled, args.steps, ((prec, args.scaled_index_bits),))))
         return
     if args.only_batched:
-        print(json.dumps(batched(pkg, args.batch_instances, args.batch_steps, 50,
+        print(json.dumps(batched(pkg, args.batch_instances, args.batch_steps, 50, layout=args.batch_layout,
                                  threads=min(16, os.cpu_count() or 4))))
         return
     if args.copy_probe_mb:

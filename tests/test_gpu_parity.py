@@ -67,7 +67,7 @@ def test_smoke():
 
 
 @pytest.mark.parametrize("graph", [0, 1])
-def test_config0_bit_exact(pkg, orc, graph):
+def test_scaffold9k_bit_exact(pkg, orc, graph):
     env = new_env(pkg, 0)
     env.set_option("graph", graph)
     env.initialize(50, 25)
@@ -80,7 +80,7 @@ def test_config0_bit_exact(pkg, orc, graph):
 
 
 @pytest.mark.parametrize("stage", [1, 0])
-def test_config1_bit_exact(pkg, orc, stage):
+def test_scaffold96k_bit_exact(pkg, orc, stage):
     """The 96.7k scaffold (4.23M synapses, the reference connectome's fan-ins),
     with blocks staging their spike windows in LDS (the default; 375 steps, past
     the 100 Hz burst at 300-350 ms) and with every gather reading the ring in
@@ -95,7 +95,7 @@ def test_config1_bit_exact(pkg, orc, stage):
     ref.initialize()
     steps = 375 if stage else 150   # the staged (default) layout runs through the 300-350 ms burst
     run_and_compare(env, ref, 0, steps, check_counts=True)
-    assert_same_state(env, ref, f"config1 {steps} steps, stage_spikes {stage}")
+    assert_same_state(env, ref, f"96.7k scaffold {steps} steps, stage_spikes {stage}")
     env.finalize()
 
 
@@ -705,7 +705,7 @@ def test_short_ring_one_step_delays(pkg, orc):
     env.finalize()
 
 
-def test_config3_batch_real_scales(pkg, orc):
+def test_batch_of_real_scales(pkg, orc):
     """The batched form at the instance sizes of the 64-instance workload: one
     instance each at 0.25x, 0.5x, 1x, 2x and 4x of the 96.7k scaffold (33M
     synapses in all), each with its own seeds and mossy rate (1 to 60 Hz), BF16
@@ -758,7 +758,7 @@ def test_config3_batch_real_scales(pkg, orc):
                 for i, (e, r) in enumerate(zip(envs, refs)):
                     assert_same_state(e, r, f"instance {i} (scale {specs[i][0]}) after 75 steps")
                 refs = []
-                print("config 3: 75 steps bit-exact for every instance", flush=True)
+                print("batch of 0.25x-4x instances: 75 steps bit-exact for every instance", flush=True)
         for s_ in range(t, t + 25):
             f64.step(s_)
             row = f64.spike_row(s_)
