@@ -52,6 +52,11 @@ class StimFunc(ctypes.Structure):
                 ("sine_on", ctypes.c_double), ("sine_off", ctypes.c_double)]
 
 
+class StdpAffine(ctypes.Structure):
+    """scaffold_stdp_affine: coefficient = (float)(c + a * (b - (double)w))."""
+    _fields_ = [("c", ctypes.c_double), ("a", ctypes.c_double), ("b", ctypes.c_double)]
+
+
 class SynthConfig(ctypes.Structure):
     _fields_ = [("scale", ctypes.c_double), ("counts", ctypes.c_int32 * 8), ("seed", ctypes.c_uint64),
                 ("weight_jitter", ctypes.c_int32), ("mf_rate_hz", ctypes.c_float), ("burst_rate_hz", ctypes.c_float),
@@ -130,6 +135,8 @@ def load_library(path=None):
                                           + [f32, f32, P(i64), P(i32), P(f32), P(ctypes.c_int)]),
         "scaffold_set_plasticity": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, P(f32),
                                                    ctypes.c_uint32, ctypes.c_int, P(ctypes.c_int)]),
+        "scaffold_set_plasticity_affine": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, P(StdpAffine),
+                                                          ctypes.c_uint32, ctypes.c_int, P(ctypes.c_int)]),
         "scaffold_set_input_stim": (ctypes.c_int, [vp, ctypes.c_int, cs, P(StimFunc), P(ctypes.c_int)]),
         "scaffold_set_input_stim_ids": (ctypes.c_int, [vp, ctypes.c_int, P(ctypes.c_uint32), ctypes.c_int, P(i64),
                                                        P(StimFunc), P(ctypes.c_int)]),
@@ -288,6 +295,22 @@ class NetworkEnvironment:
         idx = ctypes.c_int()
         _check(self._lib.scaffold_set_plasticity(self._h, rule, target, teacher, _ptr(c, ctypes.c_float), present,
                                                  time_window, ctypes.byref(idx)))
+        return idx.value
+
+    def set_plasticity_affine(self, rule, target, teacher=-1, coefficients=None, time_window=50):
+        """A plasticity rule with weight-dependent coefficients: `coefficients` maps
+        slots to (c, a, b), the coefficient of a synapse being the float value of
+        c + a * (b - (double)w) for its multiplier w before the update (the
+        template's PF_PC_A1_J = 0.001*(1.0 - w) is (0, 0.001, 1.0))."""
+        coefficients = coefficients or {}
+        c = (StdpAffine * 5)()
+        present = 0
+        for slot, (c0, a, b) in coefficients.items():
+            c[slot] = StdpAffine(c0, a, b)
+            present |= 1 << slot
+        idx = ctypes.c_int()
+        _check(self._lib.scaffold_set_plasticity_affine(self._h, rule, target, teacher, c, present, time_window,
+                                                        ctypes.byref(idx)))
         return idx.value
 
     def set_input_stim(self, neuron_index, id_file, func):

@@ -302,10 +302,11 @@ int scaffold_set_connectivity_csr(scaffold_env *env, int type, int preNum, int p
     return SCAFFOLD_OK;
 }
 
-int scaffold_set_plasticity(scaffold_env *env, int rule, int target, int teacher, const float coefficients[5],
-                            uint32_t present, int time_window, int *index_out)
+// The checks and the table row shared by both registration calls; the
+// coefficients are filled in by the caller.
+static int plasticity_row(scaffold_env *env, int rule, int target, int teacher, uint32_t present, int time_window,
+                          Plasticity &r)
 {
-    clear_error();
     int rc = check_described(env);
     if (rc) return rc;
     if ((int)env->plast.size() >= env->n_plast)
@@ -320,21 +321,59 @@ int scaffold_set_plasticity(scaffold_env *env, int rule, int target, int teacher
         if (env->projs[teacher].post != env->projs[target].post)
             return fail(SCAFFOLD_ERR_INVALID_ARG, "the teacher projection must share the target's post population");
     }
-    if (present && !coefficients) return fail(SCAFFOLD_ERR_INVALID_ARG, "null coefficients");
     if (present >> 5) return fail(SCAFFOLD_ERR_INVALID_ARG, "present: bits 0-4 only");
     if (time_window < 0) return fail(SCAFFOLD_ERR_INVALID_ARG, "time_window must be >= 0");
-    // 0: the window the reference's kernels use whatever the table says,
-    // s_upper_bound = 50 (SynapticPlasticity.cu:45,86)
-    if (time_window == 0) time_window = 50;
-    Plasticity r;
+    r = Plasticity();
     r.rule = rule;
     r.target = target;
     r.teacher = rule == SCAFFOLD_STDP_TEACHER ? teacher : -1;
-    r.window = time_window;
+    // 0: the window the reference's kernels use whatever the table says,
+    // s_upper_bound = 50 (SynapticPlasticity.cu:45,86)
+    r.window = time_window == 0 ? 50 : time_window;
     r.present = present;
+    return SCAFFOLD_OK;
+}
+
+int scaffold_set_plasticity(scaffold_env *env, int rule, int target, int teacher, const float coefficients[5],
+                            uint32_t present, int time_window, int *index_out)
+{
+    clear_error();
+    Plasticity r;
+    int rc = plasticity_row(env, rule, target, teacher, present, time_window, r);
+    if (rc) return rc;
+    if (present && !coefficients) return fail(SCAFFOLD_ERR_INVALID_ARG, "null coefficients");
     for (int c = 0; c < 5; ++c) {
         r.coef[c] = (present >> c) & 1 ? coefficients[c] : 0.f;
         if (!std::isfinite(r.coef[c])) return fail(SCAFFOLD_ERR_INVALID_ARG, "coefficient %d is not finite", c);
+    }
+    env->plast.push_back(r);
+    if (index_out) *index_out = (int)env->plast.size() - 1;
+    return SCAFFOLD_OK;
+}
+
+int scaffold_set_plasticity_affine(scaffold_env *env, int rule, int target, int teacher,
+                                   const scaffold_stdp_affine coefficients[5], uint32_t present, int time_window,
+                                   int *index_out)
+{
+    clear_error();
+    Plasticity r;
+    int rc = plasticity_row(env, rule, target, teacher, present, time_window, r);
+    if (rc) return rc;
+    if (present && !coefficients) return fail(SCAFFOLD_ERR_INVALID_ARG, "null coefficients");
+    for (int c = 0; c < 5; ++c) {
+        if (!((present >> c) & 1)) continue;
+        const scaffold_stdp_affine &f = coefficients[c];
+        if (!std::isfinite(f.c) || !std::isfinite(f.a) || !std::isfinite(f.b))
+            return fail(SCAFFOLD_ERR_INVALID_ARG, "coefficient %d is not finite", c);
+        if (f.a == 0.0) {  // a constant: (float)(c + 0 * (b - w)) is (float)c
+            r.coef[c] = (float)f.c;
+            if (!std::isfinite(r.coef[c])) return fail(SCAFFOLD_ERR_INVALID_ARG, "coefficient %d overflows float", c);
+            continue;
+        }
+        r.affine |= 1u << c;
+        r.ac[c] = f.c;
+        r.aa[c] = f.a;
+        r.ab[c] = f.b;
     }
     env->plast.push_back(r);
     if (index_out) *index_out = (int)env->plast.size() - 1;

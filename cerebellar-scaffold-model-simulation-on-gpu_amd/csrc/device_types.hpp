@@ -246,6 +246,8 @@ struct ScPlastic {
     int32_t grid_exp;          // updated multipliers are rounded to multiples of 2^grid_exp
     int32_t skip_quiet;
     float coef[4];             // A0, A1_I, A1_J, A2_JI
+    int32_t affine;            // bit c: coefficient c is (float)(ac[c] + aa[c] * (ab[c] - (double)w))
+    double ac[4], aa[4], ab[4];
 };
 
 // Fired-list queries without allocation: ids at most, and the word offset of the

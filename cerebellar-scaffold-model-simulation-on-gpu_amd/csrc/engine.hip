@@ -551,7 +551,13 @@ int engine_create(Engine **out, Plan &P)
         d.window = X.rule.window;
         d.present = (int32_t)(X.rule.present & 15u);
         d.grid_exp = X.grid_exp;
-        for (int c = 0; c < 4; ++c) d.coef[c] = X.rule.coef[c];
+        for (int c = 0; c < 4; ++c) {
+            d.coef[c] = X.rule.coef[c];
+            d.ac[c] = X.rule.ac[c];
+            d.aa[c] = X.rule.aa[c];
+            d.ab[c] = X.rule.ab[c];
+        }
+        d.affine = (int32_t)(X.rule.affine & 15u);
         e->w.plastic.push_back(d);
         e->plastic_views.push_back(Engine::PlasticView{X.rule.target, d.val, X.n_val, X.vidx});
     }

@@ -1260,12 +1260,19 @@ __global__ void __launch_bounds__(256) k_plastic(ScInstance I, ScPlastic P, int 
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         if (b0 + lane + u * SC_WAVE >= b1) continue;
+        // coefficient c for this synapse: a constant, or the weight-dependent
+        // value (float)(ac + aa * (ab - (double)w)) of the multiplier before the update
+        const auto coef = [&](int c) {
+            return ((P.affine >> c) & 1) ? (float)(P.ac[c] + P.aa[c] * (P.ab[c] - (double)w[u])) : P.coef[c];
+        };
         float dw = 0.f;
-        if (P.present & 1) dw += P.coef[0];
-        if ((P.present & 4) && (h[u] >> 7)) dw += P.coef[2];
-        if ((P.present & 2) && s_post) dw += P.coef[1];
-        if ((P.present & 8) && s_post)
-            for (int q = 0; q < (int)(h[u] & 127u); ++q) dw += P.coef[3];
+        if (P.present & 1) dw += coef(0);
+        if ((P.present & 4) && (h[u] >> 7)) dw += coef(2);
+        if ((P.present & 2) && s_post) dw += coef(1);
+        if ((P.present & 8) && s_post) {
+            const float a2 = coef(3);  // the same value for every s of the window (a function of w only)
+            for (int q = 0; q < (int)(h[u] & 127u); ++q) dw += a2;
+        }
         float nw = w[u] + dw;
         if (P.rule == SC_STDP_HEBB) nw = nw < 0.f ? 0.f : nw > 1.f ? 1.f : nw;
         else nw = nw < 0.f ? 0.f : nw;

@@ -51,6 +51,9 @@ struct Plasticity {
     int rule = 0, target = -1, teacher = -1, window = 50;
     uint32_t present = 0;
     float coef[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    // bit c of `affine`: slot c is (float)(ac[c] + aa[c] * (ab[c] - (double)w)) instead of coef[c]
+    uint32_t affine = 0;
+    double ac[5] = {0, 0, 0, 0, 0}, aa[5] = {0, 0, 0, 0, 0}, ab[5] = {0, 0, 0, 0, 0};
 };
 
 struct Stimulus {

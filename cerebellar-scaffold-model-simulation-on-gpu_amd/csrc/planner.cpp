@@ -618,23 +618,47 @@ int plan_build(scaffold_env *env, Plan &P)
         }
         // Row sums of the projection stay exact in f64 while every multiplier is a
         // multiple of 2^grid_exp and every row sums to less than 2^(grid_exp + 53):
-        // grid_exp is the finest grid for the largest row and multiplier bound
-        // (Hebb clamps to [0, 1]; Teacher only depresses, so its initial maximum
-        // bounds it).  Updated multipliers are rounded to that grid.
+        // grid_exp is the finest grid for the largest row and multiplier bound.
+        // Hebb clamps to [0, 1].  Teacher clamps at 0 only: with constant terms it
+        // must not potentiate, so its initial maximum bounds it; a weight-dependent
+        // term p - a*w (p = c + a*b) with a > 0 potentiates only below p/a, and
+        // while the potentiating terms' a, times the times each applies in one
+        // step (1; the window for A2_JI), sum to A <= 1, an update from w <= B
+        // gives at most w (1 - A) + A B <= B for B = max(initial maximum, p/a).
+        // The bound is widened by 2^-16 for the float rounding of the update.
         const std::vector<float> &ev = eff[r.target];
         float wmax = 0.f;
         for (float x : ev) wmax = std::max(wmax, std::fabs(x));
+        double wbound = wmax;
         if (r.rule == SC_STDP_HEBB) {
-            wmax = std::max(wmax, 1.f);
+            wbound = std::max(wbound, 1.0);
         } else {
-            for (int c = 0; c < 4; ++c)
-                if ((r.present >> c) & 1 && r.coef[c] > 0.f)
-                    return fail(SCAFFOLD_ERR_INVALID_ARG, "teacher rule with a potentiating coefficient has no upper bound "
-                                "for its multipliers");
+            double pot_a = 0.0, fix = 0.0;
+            for (int c = 0; c < 4; ++c) {
+                if (!((r.present >> c) & 1)) continue;
+                if (!((r.affine >> c) & 1)) {
+                    if (r.coef[c] > 0.f)
+                        return fail(SCAFFOLD_ERR_INVALID_ARG, "teacher rule with a potentiating coefficient has no "
+                                    "upper bound for its multipliers");
+                    continue;
+                }
+                const double a = r.aa[c], p = r.ac[c] + r.aa[c] * r.ab[c];
+                if (a < 0.0)
+                    return fail(SCAFFOLD_ERR_INVALID_ARG, "teacher rule: coefficient %d grows with the multiplier "
+                                "(a < 0), so its multipliers have no upper bound", c);
+                if (p > 0.0) {
+                    pot_a += (c == 3 ? r.window : 1) * a;
+                    fix = std::max(fix, p / a);
+                }
+            }
+            if (pot_a > 1.0)
+                return fail(SCAFFOLD_ERR_INVALID_ARG, "teacher rule: the potentiating terms' slopes sum to %g per step "
+                            "(at most 1 keeps the multipliers bounded)", pot_a);
+            wbound = std::max(wbound, fix) * (1.0 + std::ldexp(1.0, -16));
         }
         int64_t lmax = 0;
         for (size_t j = 0; j + 1 < src.row_ptr.size(); ++j) lmax = std::max(lmax, src.row_ptr[j + 1] - src.row_ptr[j]);
-        const double bound = (double)lmax * (double)wmax;
+        const double bound = (double)lmax * wbound;
         X.grid_exp = bound > 0.0 ? std::max(-126, (int)std::ceil(std::log2(bound)) - 52) : -126;
         for (float x : ev) {
             const float y = std::ldexp(x, -X.grid_exp);
@@ -659,7 +683,7 @@ int plan_build(scaffold_env *env, Plan &P)
         // quiet synapses keep their multiplier when dw = +0 leaves w as it is:
         // no A0 term, A1_J absent or zero, every multiplier inside the clamp
         // range and none of them -0
-        bool quiet_keeps = !(r.present & 1u) && (!(r.present & 4u) || r.coef[2] == 0.f);
+        bool quiet_keeps = !(r.present & 1u) && (!(r.present & 4u) || (!(r.affine & 4u) && r.coef[2] == 0.f));
         for (float x : ev)
             quiet_keeps = quiet_keeps && !std::signbit(x) && (r.rule != SC_STDP_HEBB || x <= 1.f);
         X.skip_quiet = quiet_keeps;

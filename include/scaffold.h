@@ -183,6 +183,28 @@ int scaffold_set_plasticity(scaffold_env *env, int rule, int target, int teacher
                             const float coefficients[5], uint32_t present, int time_window,
                             int *index_out);
 
+/* Weight-dependent coefficients: the reference's coefficient functions
+ * CTYPE f(CTYPE w, CTYPE s) (stdp_Coefficient_t, struct_enum_def.h:119) of the
+ * code-generation template, e.g. PF_PC_A1_J = 0.001*(1.0 - w) and
+ * PF_PC_A2_TI = -0.01*w (convertHDF52dat/base_file/SynapticPlasticity.cu:14-19).
+ * Slot c is the float value of the double expression
+ *   c + a * (b - (double)w)
+ * evaluated for each synapse with its multiplier w before the update (the
+ * template's functions are {0, 0.001, 1.0} and {0, 0.01, 0.0}: -0.01*w and
+ * 0.01*(0.0 - w) are the same double).  a = 0 gives the constant c.  Otherwise
+ * as scaffold_set_plasticity.  A Teacher rule must stay bounded: every term with
+ * a != 0 needs a > 0 (a term that decreases as w grows), a constant term must
+ * not be positive, and the potentiating terms' a, each times the number of times
+ * it can apply in one step (1, the window for A2_JI), may sum to at most 1; the
+ * multipliers then stay below the largest of the initial ones and of the points
+ * (c + a*b)/a where the potentiating terms vanish. */
+typedef struct {
+    double c, a, b;
+} scaffold_stdp_affine;
+int scaffold_set_plasticity_affine(scaffold_env *env, int rule, int target, int teacher,
+                                   const scaffold_stdp_affine coefficients[5], uint32_t present,
+                                   int time_window, int *index_out);
+
 /* In-memory variant: CSR by post row.  row_ptr has postNum+1 entries (int64),
  * col holds population-local pre ids, val the per-synapse multipliers of
  * |initial_weight| (NULL = all 1.0, the reference loader's value). */

@@ -102,6 +102,9 @@ def lib():
         L.orc_add_plasticity.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, P(ctypes.c_float),
                                          ctypes.c_uint, ctypes.c_int, ctypes.c_int]
         L.orc_add_plasticity.restype = ctypes.c_int
+        L.orc_set_plasticity_affine.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint, P(ctypes.c_double),
+                                                P(ctypes.c_double), P(ctypes.c_double)]
+        L.orc_set_plasticity_affine.restype = ctypes.c_int
         L.orc_get_multipliers.argtypes = [ctypes.c_void_p, ctypes.c_int, P(ctypes.c_float)]
         L.orc_get_multipliers.restype = None
         _lib = L
@@ -317,6 +320,22 @@ class OracleSim:
                 c[slot] = v
                 present |= 1 << slot
         if lib().orc_add_plasticity(self._h, rule, target, teacher, c, present, window, grid_exp) != 0:
+            raise ValueError("bad plasticity rule")
+        self._n_plast = getattr(self, "_n_plast", 0) + 1
+
+    def add_plasticity_affine(self, rule, target, teacher=-1, coefficients=None, window=50, grid_exp=-126):
+        """A plasticity rule whose slot c (A0, A1_I, A1_J, A2_JI) maps to (c0, a, b):
+        the coefficient is the float value of c0 + a * (b - (double)w) for the
+        synapse's multiplier w before the update; a = 0 is the constant c0."""
+        self.add_plasticity(rule, target, teacher, {k: (v[0] if v[1] == 0.0 else 0.0)
+                                                    for k, v in (coefficients or {}).items()}, window, grid_exp)
+        ac, aa, ab = (ctypes.c_double * 4)(), (ctypes.c_double * 4)(), (ctypes.c_double * 4)()
+        affine = 0
+        for slot, (c0, a, b) in (coefficients or {}).items():
+            if slot < 4 and a != 0.0:
+                ac[slot], aa[slot], ab[slot] = c0, a, b
+                affine |= 1 << slot
+        if lib().orc_set_plasticity_affine(self._h, self._n_plast - 1, affine, ac, aa, ab) != 0:
             raise ValueError("bad plasticity rule")
 
     def multipliers(self, proj):

@@ -109,7 +109,12 @@ struct orc_sim {
     uint8_t  *active;          /* driven mode: 1 = simulated here; NULL = all */
     /* plasticity rules and the multipliers they own (per projection, NULL: not plastic) */
     int       n_plast;
-    struct orc_plast { int rule, target, teacher, window, grid_exp; unsigned present; float coef[4]; } *plast;
+    struct orc_plast {
+        int rule, target, teacher, window, grid_exp;
+        unsigned present, affine;   /* affine bit c: slot c = (float)(ac + aa * (ab - (double)w)) */
+        float coef[4];
+        double ac[4], aa[4], ab[4];
+    } *plast;
     float   **own_val;
 };
 
@@ -525,8 +530,8 @@ int orc_add_plasticity(orc_sim *s, int rule, int target, int teacher, const floa
     s->plast = p;
     struct orc_plast *r = &s->plast[s->n_plast++];
     r->rule = rule; r->target = target; r->teacher = teacher; r->window = window;
-    r->grid_exp = grid_exp; r->present = present;
-    for (int c = 0; c < 4; ++c) r->coef[c] = coef[c];
+    r->grid_exp = grid_exp; r->present = present; r->affine = 0u;
+    for (int c = 0; c < 4; ++c) { r->coef[c] = coef[c]; r->ac[c] = r->aa[c] = r->ab[c] = 0.0; }
     if (!s->own_val) s->own_val = (float **)calloc((size_t)s->n_projs, sizeof(float *));
     orc_proj *pr = &s->projs[target];
     if (!s->own_val[target]) {
@@ -537,6 +542,27 @@ int orc_add_plasticity(orc_sim *s, int rule, int target, int teacher, const floa
         pr->val = v;
     }
     return 0;
+}
+
+/* Weight-dependent coefficients of rule `index` (the template's coefficient
+ * functions, convertHDF52dat/base_file/SynapticPlasticity.cu:14-19): bit c of
+ * `affine` makes slot c the float value of the double expression
+ * ac[c] + aa[c] * (ab[c] - (double)w) for the synapse's multiplier w before the
+ * update.  Pinning: tests/test_oracle_plasticity.py (the fixed point the rule
+ * converges to, and the template's functions evaluated independently). */
+int orc_set_plasticity_affine(orc_sim *s, int index, unsigned affine, const double *ac, const double *aa,
+                              const double *ab)
+{
+    if (index < 0 || index >= s->n_plast) return 1;
+    struct orc_plast *r = &s->plast[index];
+    r->affine = affine & 15u;
+    for (int c = 0; c < 4; ++c) { r->ac[c] = ac[c]; r->aa[c] = aa[c]; r->ab[c] = ab[c]; }
+    return 0;
+}
+
+static float plast_coef(const struct orc_plast *P, int c, float w)
+{
+    return ((P->affine >> c) & 1u) ? (float)(P->ac[c] + P->aa[c] * (P->ab[c] - (double)w)) : P->coef[c];
 }
 
 void orc_get_multipliers(const orc_sim *s, int proj, float *out)
@@ -573,15 +599,16 @@ static void plasticity_step(orc_sim *s, int n)
             for (int64_t e = pr->row_ptr[j]; e < pr->row_ptr[j + 1]; ++e) {
                 const int pre = pre_base + pr->col[e];
                 const int s_pre = pre_row[pre];
+                const float w0 = val[e];
                 float dw = 0.f;
-                if (P->present & 1u) dw += P->coef[0];
-                if ((P->present & 4u) && s_pre) dw += P->coef[2];
-                if ((P->present & 2u) && s_post) dw += P->coef[1];
+                if (P->present & 1u) dw += plast_coef(P, 0, w0);
+                if ((P->present & 4u) && s_pre) dw += plast_coef(P, 2, w0);
+                if ((P->present & 2u) && s_post) dw += plast_coef(P, 1, w0);
                 if ((P->present & 8u) && s_post)
                     for (int k = 0; k < P->window; ++k)
                         if (s->ring[(size_t)orc_row(s, n - pr->delay - k) * (size_t)s->total + (size_t)pre])
-                            dw += P->coef[3];
-                float w = val[e] + dw;
+                            dw += plast_coef(P, 3, w0);
+                float w = w0 + dw;
                 if (P->rule == 0) w = w < 0.f ? 0.f : (w > 1.f ? 1.f : w);
                 else w = w < 0.f ? 0.f : w;
                 val[e] = ldexpf(rintf(ldexpf(w, -P->grid_exp)), P->grid_exp);

@@ -111,6 +111,10 @@ void orc_set_freq(orc_sim *s, const float *freq_list);
  * target's multipliers (1.0 where val is NULL). */
 int  orc_add_plasticity(orc_sim *s, int rule, int target, int teacher, const float coef[4],
                         unsigned present, int window, int grid_exp);
+/* Weight-dependent coefficients of rule `index`: bit c of `affine` makes slot c
+ * (float)(ac[c] + aa[c] * (ab[c] - (double)w)). */
+int orc_set_plasticity_affine(orc_sim *s, int index, unsigned affine, const double *ac, const double *aa,
+                              const double *ab);
 /* Current multipliers of projection `proj` (nnz floats). */
 void orc_get_multipliers(const orc_sim *s, int proj, float *out);
 

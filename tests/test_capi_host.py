@@ -411,3 +411,38 @@ def test_plasticity_plan_checks(pkg):
                 env.initialize(t_print, 10 if t_print == 20 else 25)
             assert e.value.code in (pkg.ERR_NO_DEVICE, pkg.ERR_HIP)
             env.finalize()
+
+
+def test_weight_dependent_plasticity_checks(pkg):
+    """scaffold_set_plasticity_affine: a Teacher rule stays bounded only if every
+    weight-dependent term decreases as w grows (a > 0), no constant term is
+    positive and the potentiating slopes (times their multiplicity in a step:
+    the window for A2_JI) sum to at most 1; the template's PF->PC coefficients
+    (A1_J = 0.001*(1 - w), A2_JI = -0.01*w) pass.  Non-finite values are refused at
+    registration."""
+    def env_with(coefficients, window=50):
+        env = new_env(pkg, 0, plasticities=1)
+        idx = {env.projection(i).type: i for i in range(env.info().n_projections)}
+        env.set_plasticity_affine(pkg.STDP_TEACHER, idx[0], idx[11], coefficients, window)
+        return env
+    bad = [{pkg.A1_J: (0.0, -0.001, 1.0)},                  # grows with w
+           {pkg.A1_J: (0.001, 0.0, 0.0)},                   # a positive constant
+           {pkg.A2_JI: (0.0, 0.05, 1.0)},                   # 50 x 0.05 > 1
+           {pkg.A1_J: (0.0, 0.6, 1.0), pkg.A1_I: (0.0, 0.6, 2.0)}]
+    for coefficients in bad:
+        env = env_with(coefficients)
+        with pytest.raises(pkg.ScaffoldError) as e:
+            env.initialize(50, 25)
+        assert e.value.code == pkg.ERR_INVALID_ARG, (coefficients, str(e.value))
+        env.finalize()
+    env = new_env(pkg, 0, plasticities=1)
+    with pytest.raises(pkg.ScaffoldError) as e:
+        env.set_plasticity_affine(pkg.STDP_HEBB, 0, coefficients={pkg.A0: (float("nan"), 0.0, 0.0)})
+    assert e.value.code == pkg.ERR_INVALID_ARG
+    env.finalize()
+    if not HAS_GPU_NODE:  # the template's rule passes the planner and stops at the device
+        env = env_with({pkg.A1_J: (0.0, 0.001, 1.0), pkg.A2_JI: (0.0, 0.01, 0.0)})
+        with pytest.raises(pkg.ScaffoldError) as e:
+            env.initialize(50, 25)
+        assert e.value.code in (pkg.ERR_NO_DEVICE, pkg.ERR_HIP)
+        env.finalize()

@@ -897,3 +897,43 @@ def test_plasticity_bit_exact(pkg, orc, index_bits, precision, extra):
         assert bad.size == 0, f"projection {t}: multipliers differ at {bad[:10]}: {got[bad[:5]]} vs {want[bad[:5]]}"
         assert (got != w0[t]).mean() > 0.01, f"projection {t}: too few multipliers changed"
     env.finalize()
+
+
+@pytest.mark.parametrize("index_bits", [32, 16])
+def test_weight_dependent_plasticity_bit_exact(pkg, orc, index_bits):
+    """Weight-dependent coefficients on the GPU against the oracle, 150 steps: the
+    code-generation template's parallel-fibre-to-Purkinje Teacher rule
+    (A1_J = 0.001*(1.0 - w), A2_JI = -0.01*w;
+    convertHDF52dat/base_file/SynapticPlasticity.cu:14-21) with basket-to-Purkinje
+    as the teacher (team rows; every synapse with a presynaptic spike changes, so
+    no chunk is skipped), and a Hebb rule on Golgi-to-granule (lane slices) whose
+    A1_I has a constant part.  Rasters, state and every multiplier bit for bit."""
+    env = new_env(pkg, 0, plasticities=2)
+    idx = {env.projection(i).type: i for i in range(env.info().n_projections)}
+    pf_pc, bc_pc, goc_grc = idx[0], idx[11], idx[15]
+    teacher = dict(rule=pkg.STDP_TEACHER, target=pf_pc, teacher=bc_pc,
+                   coefficients={pkg.A1_J: (0.0, 0.001, 1.0), pkg.A2_JI: (0.0, 0.01, 0.0)}, time_window=0)
+    hebb = dict(rule=pkg.STDP_HEBB, target=goc_grc, teacher=-1,
+                coefficients={pkg.A0: (-1e-4, 0.0, 0.0), pkg.A1_J: (0.0, 2e-3, 1.0),
+                              pkg.A1_I: (-0.002, 0.004, 0.5), pkg.A2_JI: (0.0, -5e-4, 0.25)}, time_window=20)
+    for r in (teacher, hebb):
+        env.set_plasticity_affine(**r)
+    env.set_option("index_bits", index_bits)
+    env.initialize(50, 25)
+    ref = orc.OracleSim.from_env(env)
+    ref.initialize()
+    w0 = {}
+    for r in (teacher, hebb):
+        p = env.projection(r["target"])
+        assert p.plastic == 1
+        ref.add_plasticity_affine(r["rule"], r["target"], r["teacher"], r["coefficients"], r["time_window"] or 50,
+                                  p.grid_exp)
+        w0[r["target"]] = env.projection_csr(r["target"])[2].copy()
+    run_and_compare(env, ref, 0, 150, check_counts=False)
+    assert_same_state(env, ref, "weight-dependent plasticity 150 steps")
+    for t in (pf_pc, goc_grc):
+        got, want = env.projection_csr(t)[2], ref.multipliers(t)
+        bad = np.nonzero(bits(got) != bits(want))[0]
+        assert bad.size == 0, f"projection {t}: multipliers differ at {bad[:10]}: {got[bad[:5]]} vs {want[bad[:5]]}"
+        assert (got != w0[t]).mean() > 0.01, f"projection {t}: too few multipliers changed"
+    env.finalize()

@@ -161,3 +161,94 @@ def test_grid_rounding(orc):
         sim.add_plasticity(0, 0, -1, {0: -2 ** -12}, window=1, grid_exp=grid)
         sim.step_driven(0, np.zeros(N_PRE + N_POST + N_T, np.uint8))
         assert sim.multipliers(0)[0] == np.float32(want), grid
+
+
+def template_a1j(w):
+    """PF_PC_A1_J of the code-generation template (convertHDF52dat/base_file/
+    SynapticPlasticity.cu:14-16): 0.001*(1.0 - w), a double expression returned as
+    CTYPE (float)."""
+    return np.float32(0.001 * (1.0 - np.float64(w)))
+
+
+def template_a2(w):
+    """PF_PC_A2_TI of the template (:17-19): -0.01*w, returned as float."""
+    return np.float32(-0.01 * np.float64(w))
+
+
+def hebb_a1i(w):
+    """A weight-dependent A1_I with a constant part: -0.002 + 0.004*(0.5 - w)."""
+    return np.float32(-0.002 + 0.004 * (0.5 - np.float64(w)))
+
+
+def expected_step_literal(w, spk, n, rule, window, delay, t_delay):
+    """One update written out as the reference's kernel does it: dw is a float
+    sum in the order A0, A1_J, A1_I, then one A2_JI per spike of the window, each
+    coefficient the function value at the synapse's multiplier before the
+    update; then the rule's clamp."""
+    out = w.copy()
+    for j in range(N_POST):
+        if rule == 0:
+            s_post = at(spk, n, N_PRE + j)
+        else:
+            t = TEACHER_OF[j]
+            s_post = 0 if t is None else at(spk, n - t_delay, N_PRE + N_POST + t)
+        for e in range(T_RP[j], T_RP[j + 1]):
+            pre = int(T_COL[e])
+            w0 = w[e]
+            dw = np.float32(0.0)
+            if at(spk, n - delay, pre):
+                dw = np.float32(dw + template_a1j(w0))
+            if rule == 0 and s_post:
+                dw = np.float32(dw + hebb_a1i(w0))
+            if s_post:
+                for s in range(window):
+                    if at(spk, n - delay - s, pre):
+                        dw = np.float32(dw + template_a2(w0))
+            x = np.float32(w0 + dw)
+            x = np.float32(min(max(x, 0.0), 1.0)) if rule == 0 else np.float32(max(x, 0.0))
+            out[e] = x
+    return out
+
+
+@pytest.mark.parametrize("rule", [0, 1])
+def test_weight_dependent_coefficients_match_template(orc, rule):
+    """Weight-dependent coefficients (c, a, b) -> (float)(c + a*(b - (double)w)):
+    the template's PF_PC_A1_J = 0.001*(1.0 - w) as (0, 0.001, 1.0) and
+    PF_PC_A2_TI = -0.01*w as (0, 0.01, 0.0), plus (Hebb) an A1_I with a constant
+    part, against the functions written out literally in float64/float32; 80
+    steps of random trains, window 10, ring of 16 rows."""
+    coef = {2: (0.0, 0.001, 1.0), 3: (0.0, 0.01, 0.0)}
+    if rule == 0:
+        coef[1] = (-0.002, 0.004, 0.5)
+    val = np.array([0.5, 0.3, 0.9, 0.7, 0.1, 0.95], np.float32)
+    sim = build(orc, 16, val)
+    sim.add_plasticity_affine(rule, 0, 1 if rule == 1 else -1, coef, window=10, grid_exp=-60)
+    spk = trains(11 + rule, 80, p=0.35)
+    w = val.copy()
+    for n in range(80):
+        sim.step_driven(n, spk[n])
+        w = expected_step_literal(w, spk, n, rule, 10, 2, 1)
+        got = sim.multipliers(0)
+        assert np.array_equal(got.view(np.uint32), w.view(np.uint32)), (n, got, w)
+    assert not np.array_equal(w, val)
+
+
+def test_weight_dependent_potentiation_converges_to_its_fixed_point(orc):
+    """Teacher rule with only A1_J = 0.25*(1 - w) and a presynaptic spike every
+    step: w_{k+1} = w_k + (float)(0.25*(1 - w_k)) rises towards 1 and never above
+    it (the bound the planner relies on); the sequence equals the recursion
+    computed here."""
+    val = np.array([0.0, 0.5, 0.75, 0.99, 0.125, 1.0], np.float32)
+    sim = build(orc, 16, val)
+    sim.add_plasticity_affine(1, 0, 1, {2: (0.0, 0.25, 1.0)}, window=10, grid_exp=-60)
+    spk = np.zeros((120, N_PRE + N_POST + N_T), np.uint8)
+    spk[:, :N_PRE] = 1
+    w = val.copy()
+    for n in range(120):
+        sim.step_driven(n, spk[n])
+        if n >= 2:   # the pre spike of step n - delay exists from step 2 on
+            w = np.array([np.float32(x + np.float32(0.25 * (1.0 - np.float64(x)))) for x in w], np.float32)
+        got = sim.multipliers(0)
+        assert np.array_equal(got.view(np.uint32), w.view(np.uint32)), n
+        assert (got <= 1.0).all()
+    assert (w > 0.999).all()
