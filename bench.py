@@ -187,7 +187,7 @@ def variant(pkg, iterations, warmup, preset=1, plastic=False, connectome=None, *
     return r
 
 
-def batched(pkg, n_inst, steps, warmup, threads, layout=None):
+def batched(pkg, n_inst, steps, warmup, threads, layout=None, profile_steps=0):
     """The batched form: 64 independent instances of uneven size (0.25x..4x of the 96.7k
     scaffold), each with its own seeds and mossy rate U(1, 60) Hz, BF16
     multipliers, advanced together by one step kernel per exchange step
@@ -229,7 +229,10 @@ def batched(pkg, n_inst, steps, warmup, threads, layout=None):
         except Exception as ex:  # surfaced below
             errs.append(ex)
 
-    order = sorted(((c, i) for c in (0, 1) for i in range(len(scales))), key=lambda x: -scales[x[1]])
+    # profile_steps: the batch alone (no sequential copies), advanced one launch
+    # per call, without graph capture -- for counter passes
+    copies = (0,) if profile_steps else (0, 1)
+    order = sorted(((c, i) for c in copies for i in range(len(scales))), key=lambda x: -scales[x[1]])
     parts = [order[k::threads] for k in range(threads)]
     t0 = time.perf_counter()
     ths = [threading.Thread(target=work, args=(p,)) for p in parts]
@@ -239,8 +242,18 @@ def batched(pkg, n_inst, steps, warmup, threads, layout=None):
         t.join()
     if errs:
         raise errs[0]
-    log(f"batch: built 2 x {len(scales)} instances in {time.perf_counter() - t0:.1f} s")
+    log(f"batch: built {len(copies)} x {len(scales)} instances in {time.perf_counter() - t0:.1f} s")
     batch_envs, seq_envs = envs
+    if profile_steps:
+        b = pkg.Batch(batch_envs)
+        w0 = time.perf_counter()
+        for _ in range(profile_steps):
+            b.run(1)
+        wall = time.perf_counter() - w0
+        b.destroy()
+        for e in batch_envs:
+            e.finalize()
+        return {"profile_steps": profile_steps, "instances": len(scales), "layout": tuned, "wall_s": wall}
     # sequential plain runs: each instance alone, one after another
     seq_wall = 0.0
     for e in seq_envs:
@@ -349,6 +362,9 @@ def main():
                     help="layout options of the batched leg as JSON (skips the autotuner), e.g. "
                          "'{\"lane_max_fanin\": 128, \"lane_synapses\": 32, \"graph_steps\": 50, "
                          "\"index_bits\": 16, \"stage_spikes\": 1}'")
+    ap.add_argument("--batch-profile-steps", type=int, default=0,
+                    help="with --only-batched: build the batch alone and advance it this many steps, one launch "
+                         "per call and no graph capture (for counter passes)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run the calibration copy kernel on this many MiB and exit")
     args = ap.parse_args()
@@ -364,6 +380,7 @@ def main():
         return
     if args.only_batched:
         print(json.dumps(batched(pkg, args.batch_instances, args.batch_steps, 50, layout=args.batch_layout,
+                                 profile_steps=args.batch_profile_steps,
                                  threads=min(16, os.cpu_count() or 4))))
         return
     if args.copy_probe_mb:

@@ -1119,7 +1119,7 @@ int scaffold_autotune(scaffold_env *env, int n_steps, scaffold_tune_result *out)
     scaffold_tune_result r{};
     r.ms_per_step = 1e300;
     r.ms_default = -1.0;
-    const int fanins[] = {16, 48, 128}, synapses[] = {8, 16, 32};
+    const int fanins[] = {16, 48, 128}, synapses[] = {8, 16, 32, 64};
     // the environment's own options are the first candidate
     r.lane_max_fanin = saved.lane_max_fanin;
     r.lane_synapses = saved.lane_synapses;

@@ -333,11 +333,13 @@ int plan_build(scaffold_env *env, Plan &P)
     // Layout defaults by size.  A step whose streams fit the L2 twice over (at
     // most 64 MB) is bound by the latency of its slowest block: wide teams (8
     // synapses per lane) shorten the long rows' gathers.  Above that the blocks
-    // queue for the CUs and the step is bound by their throughput: narrow teams
-    // (32 per lane) carry more synapses per block's fixed costs, and blocks stage
-    // windows up to 32 words per 64-byte segment their gathers would touch.
+    // queue for the CUs and the step is bound by their throughput: every block
+    // costs about the same chain of dependent round trips whatever its work, so
+    // narrow teams (64 synapses per lane: fewer, fuller blocks) carry more
+    // synapses per block, and blocks stage windows up to 32 words per 64-byte
+    // segment their gathers would touch.
     const bool large = P.nnz * (want16 ? 2 : 4) + weight_bytes + (int64_t)ref * 32 > ((int64_t)64 << 20);
-    P.lane_synapses = opt.lane_synapses > 0 ? opt.lane_synapses : large ? 32 : 8;
+    P.lane_synapses = opt.lane_synapses > 0 ? opt.lane_synapses : large ? 64 : 8;
     P.stage_ratio = opt.stage_ratio > 0 ? opt.stage_ratio : large ? 32 : 8;
     // write one encoded multiplier
     auto put_w = [&](std::vector<uint8_t> &out, int64_t pos, const ScProj &sp, float x) {

@@ -243,7 +243,7 @@ int scaffold_set_input_stim_ids(scaffold_env *env, int neuron_index,
  *   "lane_synapses"     team mode: target synapses per lane (sets the team width);
  *                       0 (default): 8 when one step's synapse and state streams
  *                       fit in 64 MB (the step is latency-bound: wide teams),
- *                       else 32 (throughput-bound: more synapses per block)
+ *                       else 64 (throughput-bound: more synapses per block)
  *   "graph"             1: replay captured hipGraphs of the step kernels
  *   "graph_steps"       exchange steps per captured graph
  *   "reset_g"           1: also zero conductances in scaffold_reset_network_env
@@ -494,11 +494,11 @@ int scaffold_step_timer(scaffold_env *env, int on, double *ms, int64_t *launches
  * this network under candidate layouts -- lane_max_fanin (which populations use
  * one lane per neuron over sliced-ELL slices, i.e. the ELL width) in {16, 48, 128}
  * and lane_synapses (synapses per lane, i.e. wavefronts per team-mode neuron) in
- * {8, 16, 32}, besides the environment's own options -- times n_steps step
+ * {8, 16, 32, 64}, besides the environment's own options -- times n_steps step
  * kernels of each after 50 warm-up steps, then tries, one at a time on the
  * fastest layout, the other column index width (index_bits 32 / 16), the other
  * stage_spikes setting and graph_steps 50, and sets the winning options on env
- * (13 trials; ms_default is the time with the environment's own options).  The
+ * (16 trials; ms_default is the time with the environment's own options).  The
  * network's own state, seeds and counters are untouched. */
 typedef struct {
     int32_t lane_max_fanin, lane_synapses, graph_steps;
