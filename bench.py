@@ -192,9 +192,10 @@ def batched(pkg, n_inst, steps, warmup, threads, layout=None, profile_steps=0):
     scaffold), each with its own seeds and mossy rate U(1, 60) Hz, BF16
     multipliers, advanced together by one step kernel per exchange step
     (hipGraph replays), against sequential plain runs of the same instances: a
-    second, identical copy of every instance advanced alone, one after another.
-    Both copies run the same steps from initialization, so their spike totals
-    must agree exactly."""
+    second, identical copy of every instance (its own default layout, which gives
+    the same results bit for bit) advanced alone, one after another.  Both copies
+    run the same steps from initialization, so their spike totals must agree
+    exactly."""
     scales = ([0.25] * 16 + [0.5] * 16 + [1.0] * 16 + [2.0] * 8 + [4.0] * 8)[:n_inst]
     rng = np.random.default_rng(1234)
     rates = rng.uniform(1.0, 60.0, len(scales))
@@ -222,8 +223,9 @@ def batched(pkg, n_inst, steps, warmup, threads, layout=None, profile_steps=0):
                              burst_n_mf=max(1, int(round(50 * scales[i]))))
                 e.set_option("weight_precision", pkg.WEIGHTS_BF16)
                 e.set_option("neuron_seed", 77 + i)
-                for k, v in layout.items():
-                    e.set_option(k, v)
+                if c == 0:  # the batch's copy takes the tuned layout; the sequential plain
+                    for k, v in layout.items():  # runs keep each instance's own defaults
+                        e.set_option(k, v)
                 e.initialize(T_PRINT, WINDOW)
                 envs[c][i] = e
         except Exception as ex:  # surfaced below

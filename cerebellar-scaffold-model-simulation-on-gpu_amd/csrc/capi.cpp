@@ -1063,12 +1063,14 @@ int scaffold_time_steps(scaffold_env *env, int n_steps, double *ms_total, double
 // One trial engine of the network under the options in env->opt: built,
 // initialized like InitializeNetworkEnv, warmed up, timed (and its step kernels
 // timed with the step-kernel timer when kernel_ms is given), destroyed.
-static int trial_time(scaffold_env *env, int T_print, int n_steps, double *ms, double *kernel_ms = nullptr)
+static int trial_time(scaffold_env *env, int T_print, int n_steps, double *ms, double *kernel_ms = nullptr,
+                      int *lane_synapses = nullptr)
 {
     Plan P;
     P.R = 2 * T_print;
     int rc = plan_build(env, P);
     if (rc) return rc;
+    if (lane_synapses) *lane_synapses = P.lane_synapses;  // the value the options resolve to
     Engine *e = nullptr;
     rc = engine_create(&e, P);
     if (rc) return rc;
@@ -1140,7 +1142,14 @@ int scaffold_autotune(scaffold_env *env, int n_steps, scaffold_tune_result *out)
         if (!trc) ++r.n_tried;
         return trc;
     };
-    if ((rc = trial([](Options &) {}, &r.ms_default))) return rc;
+    // the default trial reports the team width its options resolve to (0 picks by
+    // network size), so that the result can be applied to other networks as is
+    env->opt = saved;
+    if ((rc = trial_time(env, T_print, n_steps, &r.ms_default, nullptr, &r.lane_synapses))) {
+        env->opt = saved;
+        return rc;
+    }
+    ++r.n_tried;
     r.ms_per_step = r.ms_default;
     for (int lf : fanins)
         for (int ls : synapses) {

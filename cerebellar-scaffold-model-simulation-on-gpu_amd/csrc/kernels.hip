@@ -1095,7 +1095,7 @@ __global__ void __launch_bounds__(256) k_stats_zero(ScInstance I0, const ScInsta
 }
 
 template <bool SINGLE>
-__global__ void __launch_bounds__(256) k_stats(ScInstance I0, const ScInstance *__restrict__ insts, int K, int nchunk)
+__global__ void __launch_bounds__(1024) k_stats(ScInstance I0, const ScInstance *__restrict__ insts, int K, int nchunk)
 {
     __shared__ unsigned long long acc[SC_MAX_POPS + 1];
     const int chunk = blockIdx.x % nchunk, rest = blockIdx.x / nchunk;
@@ -1129,7 +1129,8 @@ __global__ void __launch_bounds__(256) k_stats(ScInstance I0, const ScInstance *
             for (int u = 0; u < 8; ++u) ev += v[u];
         }
     };
-    // two words per thread and pass, their population ids loaded alongside
+    // two words per thread (one pass over a chunk of 2048 words with 1024
+    // threads), their population ids loaded alongside
     for (int g = g0 + threadIdx.x; g < g1; g += 2 * blockDim.x) {
         const int gb = g + blockDim.x;
         const uint64_t wa = w[g];
@@ -1366,10 +1367,10 @@ void launch_stats(hipStream_t s, const ScInstance *single, const ScInstance *ins
     const int nchunk = std::max(1, (max_groups + SC_STATS_CHUNK - 1) / SC_STATS_CHUNK);
     if (single) {
         if (nchunk > 1) k_stats_zero<true><<<K, 256, 0, s>>>(*single, insts, K);
-        k_stats<true><<<K * nchunk, 256, 0, s>>>(*single, insts, K, nchunk);
+        k_stats<true><<<K * nchunk, SC_STATS_CHUNK / 2, 0, s>>>(*single, insts, K, nchunk);
     } else {
         if (nchunk > 1) k_stats_zero<false><<<K * n_inst, 256, 0, s>>>(ScInstance{}, insts, K);
-        k_stats<false><<<K * n_inst * nchunk, 256, 0, s>>>(ScInstance{}, insts, K, nchunk);
+        k_stats<false><<<K * n_inst * nchunk, SC_STATS_CHUNK / 2, 0, s>>>(ScInstance{}, insts, K, nchunk);
     }
 }
 

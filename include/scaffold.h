@@ -498,8 +498,10 @@ int scaffold_step_timer(scaffold_env *env, int on, double *ms, int64_t *launches
  * kernels of each after 50 warm-up steps, then tries, one at a time on the
  * fastest layout, the other column index width (index_bits 32 / 16), the other
  * stage_spikes setting and graph_steps 50, and sets the winning options on env
- * (16 trials; ms_default is the time with the environment's own options).  The
- * network's own state, seeds and counters are untouched. */
+ * (16 trials; ms_default is the time with the environment's own options, and a
+ * lane_synapses of 0 is reported as the value it resolves to for this network,
+ * so the result can be applied to other networks as is).  The network's own
+ * state, seeds and counters are untouched. */
 typedef struct {
     int32_t lane_max_fanin, lane_synapses, graph_steps;
     int32_t n_tried;
