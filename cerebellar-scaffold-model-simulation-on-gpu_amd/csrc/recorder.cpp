@@ -91,6 +91,10 @@ static void write_window(Recorder *r, int s, int64_t end)
     const uint64_t *rows = r->slot[(size_t)s];
     const int64_t h = end / r->T - 1;                 // hand-off index in the trial
     const int k = (int)((h / std::max(1, r->T_print / r->T)) % 2);
+    if (!r->host[k] || !r->dev[k]) {  // a merge that failed left the files closed
+        r->failed = 1;
+        return;
+    }
     std::string hostbuf;
     const int np = (int)r->pop_kind.size();
     for (int i = 0; i < r->T; ++i) {
@@ -130,8 +134,8 @@ static void writer_loop(Recorder *r)
         r->busy = true;
         const int64_t end = r->slot_end[(size_t)s];
         lock.unlock();
-        (void)hipEventSynchronize(r->copied[(size_t)s]);
-        write_window(r, s, end);
+        if (hipEventSynchronize(r->copied[(size_t)s]) == hipSuccess) write_window(r, s, end);
+        else r->failed = 1;  // reported by the next merge, restart or finish
         lock.lock();
         r->slot_end[(size_t)s] = -1;
         r->busy = false;
