@@ -69,6 +69,7 @@ struct Engine {
     const int32_t *out_bits = nullptr;
     uint8_t *out_d = nullptr;
     uint8_t *out_h = nullptr;
+    uint8_t *out_hd = nullptr;  // the device's address of out_h (mapped pinned memory), nullptr: copy instead
     int n_out = 0, T_out = 0;
     // plastic projections: multiplier array and per-synapse index, for read-back
     struct PlasticView {
@@ -579,9 +580,15 @@ int engine_create(Engine **out, Plan &P)
         e->out_bits = ob;
         const size_t nb = (size_t)e->T_out * (size_t)std::max(1, e->n_out);
         if ((rc = dalloc(A, B, &e->out_d, nb))) return bail(rc);
-        if (hipHostMalloc(reinterpret_cast<void **>(&e->out_h), nb, hipHostMallocDefault) != hipSuccess)
+        if (hipHostMalloc(reinterpret_cast<void **>(&e->out_h), nb, hipHostMallocMapped) != hipSuccess)
             return bail(fail(SCAFFOLD_ERR_HIP, "hipHostMalloc of the output train failed"));
         memset(e->out_h, 0, nb);
+        // the gather writes the train straight into the mapped pinned buffer (no
+        // separate device-to-host copy); where the runtime gives no device address
+        // for it, the train is gathered on the device and copied
+        void *dp = nullptr;
+        if (hipHostGetDevicePointer(&dp, e->out_h, 0) == hipSuccess && dp) e->out_hd = static_cast<uint8_t *>(dp);
+        else (void)hipGetLastError();
     }
     std::vector<ScInstance> one{I};
     if ((rc = dupload(A, B, &e->di, one))) return bail(rc);
@@ -821,10 +828,11 @@ int engine_run_output(Engine *e, int n_steps, int use_graph, int graph_steps, in
         const int R = e->hi.R;
         const int row0 = (int)(((first_out_step % R) + R) % R);
         const size_t nb = (size_t)e->T_out * (size_t)e->n_out;
-        launch_out_train(e->stream, e->hi.ring, e->hi.Wr, R, row0, e->T_out, e->out_bits, e->n_out, e->out_d);
+        launch_out_train(e->stream, e->hi.ring, e->hi.Wr, R, row0, e->T_out, e->out_bits, e->n_out,
+                         e->out_hd ? e->out_hd : e->out_d);
         HIP_TRY(hipGetLastError());
-        if (nb) HIP_TRY(hipMemcpyAsync(e->out_h, e->out_d, nb, hipMemcpyDeviceToHost, e->stream));
-        *train = e->out_h;
+        if (!e->out_hd && nb) HIP_TRY(hipMemcpyAsync(e->out_h, e->out_d, nb, hipMemcpyDeviceToHost, e->stream));
+        *train = e->out_h;  // complete once engine_finish has synchronized the stream
     }
     return engine_finish(e);
 }
