@@ -243,9 +243,15 @@ def test_options(pkg):
     assert env.get_option("nontemporal") == -1
     env.set_option("nontemporal", 1)
     assert env.get_option("nontemporal") == 1
+    env.set_option("stage_ratio", 32)
+    assert env.get_option("stage_ratio") == 32
+    env.set_option("lane_synapses", 64)
+    assert env.get_option("lane_synapses") == 64
     for k, v, code in [("no_such_option", 1, "ERR_INVALID_ARG"), ("weight_precision", 7, "ERR_INVALID_ARG"),
                        ("index_bits", 8, "ERR_INVALID_ARG"), ("graph_steps", 0, "ERR_INVALID_ARG"),
-                       ("nontemporal", 2, "ERR_INVALID_ARG")]:
+                       ("nontemporal", 2, "ERR_INVALID_ARG"), ("stage_ratio", -1, "ERR_INVALID_ARG"),
+                       ("stage_ratio", 1000001, "ERR_INVALID_ARG"), ("lane_synapses", -1, "ERR_INVALID_ARG"),
+                       ("lane_synapses", 4097, "ERR_INVALID_ARG")]:
         with pytest.raises(pkg.ScaffoldError) as e:
             env.set_option(k, v)
         assert e.value.code == getattr(pkg, code), k
