@@ -98,8 +98,13 @@ struct ScCtl {
     int64_t n;      // trial-relative step of launch offset k = 0
     int64_t m;      // stimulus draws since init (3 Philox blocks per draw)
     int32_t row;    // ring row of step n (n mod R)
-    int32_t pad_;
+    int32_t recount;  // 1: the ring changed outside the step sequence (initialization,
+                      // reset, a jump of the step count); the plasticity window counts
+                      // of the next run are recomputed from the ring, not updated
 };
+
+// 64-neuron groups per wavefront of the plasticity window-count class
+#define SC_HIST_GROUPS 8
 
 // Team-mode column words: projection index (segment of the merged row) in the top
 // bits, presynaptic device bit index below.

@@ -516,7 +516,10 @@ int engine_create(Engine **out, Plan &P)
         }
     }
     int n_hist = 0;  // window-count groups of the plasticity rules
-    for (const PlasticPlan &X : P.plastic) n_hist += (P.pops[P.projs[X.rule.target].pre_pop].num + SC_WAVE - 1) / SC_WAVE;
+    for (const PlasticPlan &X : P.plastic) {
+        const int groups = (P.pops[P.projs[X.rule.target].pre_pop].num + SC_WAVE - 1) / SC_WAVE;
+        n_hist += (groups + SC_HIST_GROUPS - 1) / SC_HIST_GROUPS;  // wavefronts of the window-count class
+    }
     if ((rc = upload_work(A, B, e->w, e->h_team_t, e->h_lanes, e->h_stims, e->h_teams, e->h_lane_win, e->h_team_win,
                           P.stage != 0, P.stage_ratio, n_hist)))
         return bail(rc);

@@ -906,38 +906,69 @@ __device__ __forceinline__ void do_team(const ScInstance &I0, const ScInstance *
     }
 }
 
-// Window counts of the plasticity rules for step n: wavefront `item` of the
-// class takes one 64-neuron group of one rule's presynaptic population, one lane
-// per neuron, and writes its byte of the rule's `hist` (spikes at n - delay - s,
-// s < window, and at n - delay).  Those rows are complete before step n starts
-// (every delay is at least one step).  The group's word of each window row is
-// one wave-uniform load; the loads go out 16 at a time.
+// Window counts of the plasticity rules for step n: per neuron of a rule's
+// presynaptic population, one byte of the rule's `hist` holds its spikes in rows
+// n - delay - s (s < window) and, in bit 7, its spike at n - delay.  Those rows are
+// complete before step n starts (every delay is at least one step).  Wavefront
+// `item` of the class takes SC_HIST_GROUPS 64-neuron groups of one rule, one lane
+// per neuron of each.  The count is updated from the previous step's,
+//   count(n) = count(n - 1) + [spike at n - delay] - [spike at n - delay - window],
+// which reads two ring words per group (the ring holds delay + window + 2 rows, so
+// the row leaving the window is still intact).  In the first step after the ring
+// changed outside the step sequence (ctl->recount, cleared by the k_plastic launch
+// that follows the step) the whole window is counted again, 16 rows in flight.
 __device__ __forceinline__ void plastic_hist(const ScInstance &I, const ScPlastic *__restrict__ rules, int item,
                                              int row_n, int lane)
 {
     const ScPlastic *P = rules;
-    while (item >= P->pre_groups) {  // the rule and group of this item
-        item -= P->pre_groups;
+    for (;;) {  // the rule of this item
+        const int nw = (P->pre_groups + SC_HIST_GROUPS - 1) / SC_HIST_GROUPS;
+        if (item < nw) break;
+        item -= nw;
         ++P;
     }
     const int R = I.R, window = P->window;
     const int pre_t = row_back(row_n, P->delay, R);
-    const uint64_t *col = I.ring + (P->pre_group0 + item);
-    int cnt = 0, s0 = 0;
-    for (int b = 0; b < window; b += 16) {
-        uint64_t w[16];
+    const int g0 = item * SC_HIST_GROUPS, g1 = min(P->pre_groups, g0 + SC_HIST_GROUPS);
+    const uint64_t *ring = I.ring + P->pre_group0;
+    uint8_t *hist = P->hist;
+    if (uniform(I.ctl->recount)) {
+        for (int g = g0; g < g1; ++g) {
+            const uint64_t *col = ring + g;
+            int cnt = 0, s0 = 0;
+            for (int b = 0; b < window; b += 16) {
+                uint64_t w[16];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) w[u] = b + u < window ? col[(int64_t)row_back(pre_t, b + u, R) * I.Wr] : 0ull;
+                for (int u = 0; u < 16; ++u)
+                    w[u] = b + u < window ? col[(int64_t)row_back(pre_t, b + u, R) * I.Wr] : 0ull;
 #pragma unroll
-        for (int u = 0; u < 16; ++u) cnt += (int)((w[u] >> lane) & 1ull);
-        if (b == 0) s0 = (int)(w[0] >> lane) & 1;
+                for (int u = 0; u < 16; ++u) cnt += (int)((w[u] >> lane) & 1ull);
+                if (b == 0) s0 = (int)(w[0] >> lane) & 1;
+            }
+            hist[(int64_t)g * SC_WAVE + lane] = (uint8_t)(cnt | (s0 << 7));
+        }
+        return;
     }
-    P->hist[(int64_t)item * SC_WAVE + lane] = (uint8_t)(cnt | (s0 << 7));
+    const int64_t r_in = (int64_t)pre_t * I.Wr, r_out = (int64_t)row_back(pre_t, window, R) * I.Wr;
+    uint64_t a[SC_HIST_GROUPS], b[SC_HIST_GROUPS];
+    uint32_t old[SC_HIST_GROUPS];
+#pragma unroll
+    for (int u = 0; u < SC_HIST_GROUPS; ++u) {
+        const int g = g0 + u;
+        const bool v = g < g1;
+        a[u] = v ? ring[r_in + g] : 0ull;
+        b[u] = v ? ring[r_out + g] : 0ull;
+        old[u] = v ? hist[(int64_t)g * SC_WAVE + lane] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < SC_HIST_GROUPS; ++u) {
+        const int g = g0 + u;
+        if (g >= g1) break;
+        const uint32_t in = (uint32_t)(a[u] >> lane) & 1u, out = (uint32_t)(b[u] >> lane) & 1u;
+        hist[(int64_t)g * SC_WAVE + lane] = (uint8_t)((((old[u] & 127u) + in - out) & 127u) | (in << 7));
+    }
 }
 
-// Issues the copies of a staged block's windows into LDS (see ScStage); sd holds
-// dword `lane` of the block's window table.  Returns the LDS windows, or nullptr
-// for a block without windows.
 template <bool SINGLE>
 __device__ __forceinline__ const uint32_t *stage_windows(uint32_t sd, uint32_t *win_lds, const ScInstance &I0,
                                                          const ScInstance *__restrict__ insts, int row_single,
@@ -1167,6 +1198,7 @@ __global__ void k_advance(const ScInstance *__restrict__ insts, int n_inst, int 
         c->n += K;
         c->m += K;
         c->row = (c->row + K) % insts[i].R;
+        c->recount = 0;
     }
 }
 
@@ -1176,6 +1208,7 @@ __global__ void k_set_ctl(ScCtl *ctl, int64_t n, int64_t m, int row)
         ctl->n = n;
         ctl->m = m;
         ctl->row = row;
+        ctl->recount = 1;
     }
 }
 
@@ -1187,6 +1220,7 @@ __global__ void k_init_params(const ScInstance *__restrict__ ip, uint32_t k0, ui
     const ScInstance &I = *ip;
     const int d = blockIdx.x * blockDim.x + threadIdx.x;
     if (d < I.Wr) I.ring[d] = 0ull;
+    if (d == 0) I.ctl->recount = 1;  // ring row 0 changed
     if (d >= I.Ndev) return;
     const ScGroup &G = I.groups[d / SC_WAVE];
     const int lane = d & (SC_WAVE - 1);
@@ -1231,6 +1265,9 @@ __global__ void __launch_bounds__(256) k_plastic(ScInstance I, ScPlastic P, int 
 {
     const int c = uniform((int)blockIdx.x * 4 + (int)(threadIdx.x / SC_WAVE));
     const int lane = threadIdx.x & (SC_WAVE - 1);
+    // the window counts of this step are complete (the step kernel ran before this
+    // launch): the next step updates them from these
+    if (c == 0 && lane == 0) I.ctl->recount = 0;
     if (c >= P.n_chunks) return;
     const int R = I.R;
     const int row = (I.ctl->row + k) % R;

@@ -937,3 +937,41 @@ def test_weight_dependent_plasticity_bit_exact(pkg, orc, index_bits):
         assert bad.size == 0, f"projection {t}: multipliers differ at {bad[:10]}: {got[bad[:5]]} vs {want[bad[:5]]}"
         assert (got != w0[t]).mean() > 0.01, f"projection {t}: too few multipliers changed"
     env.finalize()
+
+
+def test_plasticity_across_trials(pkg, orc):
+    """Plasticity window counts are updated from the previous step's and counted
+    again from the ring after initialization, a reset and a jump of the step
+    count: two trials of the reference's driver loop with ResetNetworkEnv between
+    them and a jump inside the second, the Teacher rule on parallel fibre ->
+    Purkinje and a Hebb rule on Golgi -> granule (two rules: the window-count
+    wavefronts of both), rasters, state and every multiplier against the oracle."""
+    env = new_env(pkg, 0, plasticities=2)
+    idx = {env.projection(i).type: i for i in range(env.info().n_projections)}
+    pf_pc, bc_pc, goc_grc = idx[0], idx[11], idx[15]
+    rules = [dict(rule=pkg.STDP_TEACHER, target=pf_pc, teacher=bc_pc,
+                  coefficients={pkg.A1_J: 0.0, pkg.A2_JI: -0.01}, time_window=50),
+             dict(rule=pkg.STDP_HEBB, target=goc_grc, teacher=-1,
+                  coefficients={pkg.A0: -1e-4, pkg.A1_J: 2e-3, pkg.A1_I: -1e-3, pkg.A2_JI: 5e-4}, time_window=20)]
+    for r in rules:
+        env.set_plasticity(**r)
+    env.initialize(50, 25)
+    ref = orc.OracleSim.from_env(env)
+    ref.initialize()
+    for r in rules:
+        ref.add_plasticity(r["rule"], r["target"], r["teacher"], r["coefficients"], r["time_window"],
+                           env.projection(r["target"]).grid_exp)
+    for trial in range(2):
+        starts = [0, 25, 50, 75] if trial == 0 else [0, 25, 150, 175]   # a jump from 50 to 150
+        for t in starts:
+            env.loop_n_steps(t, 25, trial, want_output=False)
+            for s in range(t, t + 25):
+                ref.run(s, 1)
+                assert np.array_equal(env.spike_row(s), ref.spike_row(s)), f"trial {trial} step {s}"
+        assert_same_state(env, ref, f"end of trial {trial}")
+        for r in rules:
+            got, want = env.projection_csr(r["target"])[2], ref.multipliers(r["target"])
+            assert np.array_equal(bits(got), bits(want)), f"trial {trial}: multipliers of {r['target']}"
+        env.reset(trial)
+        ref.reset()
+    env.finalize()
