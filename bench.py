@@ -15,8 +15,8 @@ over at its end -- the drop-in path a caller of the reference runs.
 
 Prints one JSON line: synaptic events/s (value) and real-time factor of the timed
 iterations, the roofline figure of the step kernel (its duration measured with
-HIP events that the captured graphs record around the step kernels during the
-timed region), the CPU baseline (the single-threaded C oracle on a bounded sample
+HIP events recorded on the kernels' stream around each call's step kernels during
+the timed region), the CPU baseline (the single-threaded C oracle on a bounded sample
 of the same network), and further legs: the device loop alone, the rule-text
 density, 16-bit columns, the reference's Purkinje plasticity rule, 64 batched
 uneven instances against sequential runs of the same instances, and the 16x /
@@ -80,7 +80,7 @@ def drop_in_loop(env, t0, iterations, freq):
 def timed_drop_in(env, t, iterations, freq):
     """Runs `iterations` driver-loop iterations and measures them: wall time (the
     calls return when their steps are done), synaptic events, per-population
-    spikes, and the step-kernel time from the HIP events the graphs record."""
+    spikes, and the step-kernel time from the HIP events around the step kernels."""
     info = env.info()
     env.step_timer(1)
     sp0, ev0 = env.totals()
@@ -392,8 +392,7 @@ def main():
 
     t0 = time.perf_counter()
     env = make_env(pkg, 1)
-    env.set_option("graph", 1)
-    env.set_option("graph_steps", WINDOW)
+    env.set_option("graph_steps", WINDOW)  # one run (and one run-end launch) per call
     env.initialize(T_PRINT, WINDOW)
     info = env.info()
     log(f"built scaffold: {info.total_neurons} cells, {info.total_synapses} synapses, "
@@ -427,7 +426,9 @@ def main():
                    "first_timed_exchange_step": args.warmup * WINDOW,
                    "column_index_bits": 32, "neurons": info.total_neurons, "synapses": info.total_synapses,
                    "synapses_per_projection": proj_nnz(env),
-                   "input": "4 Hz mossy Poisson + 100 Hz burst to 50 fibres, 300-350 ms"},
+                   "input": "4 Hz mossy Poisson + 100 Hz burst to 50 fibres, 300-350 ms",
+                   "launches": "per call: 25 direct step-kernel launches, then one run-end launch (statistics, "
+                               "output train, step counters)"},
         "real_time_factor": r["real_time_factor"],
         "synapses_scanned_per_s": r["synapses_scanned_per_s"],
         "events_in_timed_region": r["events"],
@@ -436,9 +437,9 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
                      "kernel": "k_step", "kernel_mean_ms": kmean, "launches_timed": r["kernel_launches_timed"],
                      "algorithmic_bytes_per_launch": r["algorithmic_bytes_per_step"],
-                     "timing": "HIP events recorded by the captured graphs around the step kernels, timed region"},
+                     "timing": "HIP events on the step kernels' stream around each call's 25 step kernels, timed region"},
     }
-    # the device loop alone (graph replays of 25 steps, no output train, no host calls between)
+    # the device loop alone (runs of 25 steps, no output train, no host calls between)
     env.step_timer(1)
     w0 = time.perf_counter()
     env.time_steps(args.device_steps)

@@ -39,6 +39,26 @@ struct KTimer {
     int64_t launches = 0;
 };
 
+// The captured graphs of an engine or a batch, one per use: a whole run (step
+// kernels, then the run's end), a whole run whose end also gathers the output
+// train, and the step kernels alone (timed runs).  Each is captured on first use
+// and again when the run length or the output train changes.
+struct RunGraphs {
+    enum { RUN, RUN_OUT, STEPS, N };
+    struct Slot {
+        hipGraphExec_t exec = nullptr;
+        int chunk = 0;
+        ScOut out;
+    } slot[N];
+    void destroy()
+    {
+        for (Slot &x : slot) {
+            if (x.exec) (void)hipGraphExecDestroy(x.exec);
+            x.exec = nullptr;
+        }
+    }
+};
+
 struct Engine {
     hipStream_t stream = nullptr;
     std::vector<void *> allocs;
@@ -59,10 +79,7 @@ struct Engine {
     int32_t *scratch = nullptr;  // the static scratch on this engine's device (no-allocation queries)
     std::vector<int32_t> dev_to_ref;
     int Ndev = 0, total_ref = 0;
-    hipGraphExec_t gexec = nullptr;
-    int gexec_steps = 0;
-    hipGraphExec_t gexec_k = nullptr;  // timed runs: the step kernels of a run only
-    int gexec_k_steps = 0;
+    RunGraphs graphs;
     KTimer kt;
     // output spike train: device bits of the OUTPUT neurons, the T_out x n_out
     // byte buffer on the device and its pinned host copy
@@ -314,6 +331,7 @@ static int upload_work(std::vector<void *> &allocs, int64_t *bytes, DevWork &w, 
     w.lanes = l;
     w.stims = s;
     w.teams = t;
+    if ((rc = dalloc(allocs, bytes, &w.end_done, 1))) return rc;  // zeroed
     w.idx16 = false;
     for (const ScLaneTask &x : lanes) w.idx16 = w.idx16 || x.idx16;
     for (const ScTeamTask &x : teams) w.idx16 = w.idx16 || x.hdr;
@@ -603,8 +621,7 @@ void engine_destroy(Engine *e)
 {
     if (!e) return;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
-    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
-    if (e->gexec_k) (void)hipGraphExecDestroy(e->gexec_k);
+    e->graphs.destroy();
     for (void *p : e->allocs) (void)hipFree(p);
     if (e->out_h) (void)hipHostFree(e->out_h);
     for (hipEvent_t x : e->kt.pool) (void)hipEventDestroy(x);
@@ -673,10 +690,10 @@ int engine_upload_stims(Engine *e, const std::vector<ScStim> &stims)
     return SCAFFOLD_OK;
 }
 
-// Steps in runs of at most `chunk` (< ring rows, so the statistics of a run still
-// find every row of the run): the step kernels, then one k_stats over the run,
-// then the step counter advance.  Full runs replay one captured hipGraph.
-// The step kernels of a run of K steps (and the plasticity kernels after each).
+// Steps in runs of at most `chunk` (< ring rows, so the end of a run still finds
+// every row of the run): the step kernels (and the plasticity kernels after
+// each), then the run's end (statistics, the output train when the run is the
+// last of a hand-off, the counter advance).  Full runs replay a captured hipGraph.
 static void launch_step_kernels(hipStream_t s, const ScInstance *single, const ScInstance *insts, const DevWork &w,
                                 int K)
 {
@@ -686,29 +703,50 @@ static void launch_step_kernels(hipStream_t s, const ScInstance *single, const S
     }
 }
 
-// The statistics of a run and the advance of the step counters.
-static void launch_run_tail(hipStream_t s, const ScInstance *single, const ScInstance *insts, int n_inst,
-                            const DevWork &w, int K)
+static void launch_end(hipStream_t s, const ScInstance *single, const ScInstance *insts, int n_inst, const DevWork &w,
+                       int K, const ScOut *out)
 {
-    launch_stats(s, single, insts, n_inst, K, w.max_groups);
-    launch_advance(s, insts, n_inst, K);
+    launch_run_end(s, single, insts, n_inst, K, w.max_groups, out, w.end_done);
 }
 
-// Captures `chunk` steps as a graph: the step kernels, and with `tail` the
-// statistics and the counter advance as well.
+static bool same_out(const ScOut &a, const ScOut &b)
+{
+    return a.bits == b.bits && a.out == b.out && a.T == b.T && a.n_out == b.n_out;
+}
+
+// Captures `chunk` steps as a graph: the step kernels, and with `end` the run's
+// end as well (with the output train when out is set).
 static hipError_t capture_run(hipStream_t s, hipGraphExec_t *gexec, const ScInstance *single, const ScInstance *insts,
-                              int n_inst, const DevWork &w, int chunk, bool tail)
+                              int n_inst, const DevWork &w, int chunk, bool end, const ScOut *out)
 {
     hipGraph_t g = nullptr;
     hipError_t err = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
     if (err != hipSuccess) return err;
     launch_step_kernels(s, single, insts, w, chunk);
-    if (tail) launch_run_tail(s, single, insts, n_inst, w, chunk);
+    if (end) launch_end(s, single, insts, n_inst, w, chunk, out);
     err = hipStreamEndCapture(s, &g);
     if (err == hipSuccess) err = hipGraphInstantiate(gexec, g, nullptr, nullptr, 0);
     if (g) (void)hipGraphDestroy(g);
     if (err != hipSuccess) *gexec = nullptr;
     return err;
+}
+
+// The graph `which` of G for runs of `chunk` steps, captured on first use.
+static int graph_for(hipStream_t s, RunGraphs &G, int which, const ScInstance *single, const ScInstance *insts,
+                     int n_inst, const DevWork &w, int chunk, const ScOut *out, hipGraphExec_t *ge)
+{
+    RunGraphs::Slot &x = G.slot[which];
+    const ScOut o = out ? *out : ScOut{};
+    if (!x.exec || x.chunk != chunk || !same_out(x.out, o)) {
+        if (x.exec) (void)hipGraphExecDestroy(x.exec);
+        x.exec = nullptr;
+        hipError_t err = capture_run(s, &x.exec, single, insts, n_inst, w, chunk, which != RunGraphs::STEPS, out);
+        if (err != hipSuccess) return fail(SCAFFOLD_ERR_HIP, "graph capture: %s", hipGetErrorString(err));
+        x.chunk = chunk;
+        x.out = o;
+    }
+    *ge = x.exec;
+    return SCAFFOLD_OK;
 }
 
 // The next pair of timer events (created on first use).
@@ -726,58 +764,48 @@ static int timer_pair(KTimer *kt, int K, hipEvent_t *a, hipEvent_t *b)
     return SCAFFOLD_OK;
 }
 
-// Graph replays of `chunk` steps, then direct launches for the rest.  With the
-// timer on, each replay is of the graph of the step kernels alone, between two
-// events, its statistics and counter advance launched after it; otherwise of the
-// graph of the whole run.
-static int run_steps(hipStream_t s, hipGraphExec_t *gexec, int *gexec_steps, const ScInstance *single,
-                     const ScInstance *insts, int n_inst, const DevWork &w, int n_steps, int use_graph, int chunk,
-                     KTimer *kt = nullptr, hipGraphExec_t *gexec_k = nullptr, int *gexec_k_steps = nullptr)
+// n_steps steps in runs: graph replays of `chunk` steps, then direct launches for
+// the rest (every run direct without graphs).  With the timer on, each run's step
+// kernels lie between two events and its end is launched after them.  `out`: the
+// output train of the last out->T steps, gathered by the last run's end when that
+// run holds them all (*out_taken then set; otherwise the caller gathers it).
+static int run_steps(hipStream_t s, RunGraphs &G, const ScInstance *single, const ScInstance *insts, int n_inst,
+                     const DevWork &w, int n_steps, int use_graph, int chunk, KTimer *kt = nullptr,
+                     const ScOut *out = nullptr, bool *out_taken = nullptr)
 {
+    if (out_taken) *out_taken = false;
+    if (n_steps <= 0) return SCAFFOLD_OK;
+    const bool timed = kt && kt->on;
+    const int last_k = n_steps % chunk ? n_steps % chunk : chunk;  // steps of the last run
+    const ScOut *last_out = out && out->out && out->T > 0 && out->T <= last_k ? out : nullptr;
     int done = 0;
-    const bool timed = kt && kt->on && gexec_k;
-    hipGraphExec_t *ge = timed ? gexec_k : gexec;
-    int *ge_steps = timed ? gexec_k_steps : gexec_steps;
-    if (use_graph && n_steps >= chunk) {
-        if (!*ge || *ge_steps != chunk) {
-            if (*ge) {
-                (void)hipGraphExecDestroy(*ge);
-                *ge = nullptr;
-            }
-            hipError_t err = capture_run(s, ge, single, insts, n_inst, w, chunk, !timed);
-            if (err != hipSuccess) return fail(SCAFFOLD_ERR_HIP, "graph capture: %s", hipGetErrorString(err));
-            *ge_steps = chunk;
-        }
-        while (n_steps - done >= chunk) {
-            hipEvent_t a = nullptr, b = nullptr;
-            if (timed) {
-                int rc = timer_pair(kt, chunk, &a, &b);
-                if (rc) return rc;
-                HIP_TRY(hipEventRecord(a, s));
-            }
-            HIP_TRY(hipGraphLaunch(*ge, s));
-            if (timed) {
-                HIP_TRY(hipEventRecord(b, s));
-                launch_run_tail(s, single, insts, n_inst, w, chunk);
-                HIP_TRY(hipGetLastError());
-            }
-            done += chunk;
-        }
-    }
     while (done < n_steps) {
         const int K = std::min(chunk, n_steps - done);
+        const bool last = done + K == n_steps;
+        const ScOut *o = last ? last_out : nullptr;
+        const bool graph = use_graph && K == chunk;
         hipEvent_t a = nullptr, b = nullptr;
         if (timed) {
             int rc = timer_pair(kt, K, &a, &b);
             if (rc) return rc;
             HIP_TRY(hipEventRecord(a, s));
         }
-        launch_step_kernels(s, single, insts, w, K);
+        if (graph) {
+            hipGraphExec_t ge = nullptr;
+            const int which = timed ? RunGraphs::STEPS : o ? RunGraphs::RUN_OUT : RunGraphs::RUN;
+            int rc = graph_for(s, G, which, single, insts, n_inst, w, chunk, which == RunGraphs::RUN_OUT ? o : nullptr,
+                               &ge);
+            if (rc) return rc;
+            HIP_TRY(hipGraphLaunch(ge, s));
+        } else {
+            launch_step_kernels(s, single, insts, w, K);
+        }
         if (timed) HIP_TRY(hipEventRecord(b, s));
-        launch_run_tail(s, single, insts, n_inst, w, K);
+        if (timed || !graph) launch_end(s, single, insts, n_inst, w, K, o);
         HIP_TRY(hipGetLastError());
         done += K;
     }
+    if (out_taken) *out_taken = last_out != nullptr;
     return SCAFFOLD_OK;
 }
 
@@ -801,8 +829,8 @@ static int engine_finish(Engine *e)
 
 int engine_run(Engine *e, int n_steps, int use_graph, int graph_steps)
 {
-    int rc = run_steps(e->stream, &e->gexec, &e->gexec_steps, &e->hi, e->di, 1, e->w, n_steps, use_graph,
-                       chunk_for(graph_steps, e->hi.R), &e->kt, &e->gexec_k, &e->gexec_k_steps);
+    int rc = run_steps(e->stream, e->graphs, &e->hi, e->di, 1, e->w, n_steps, use_graph,
+                       chunk_for(graph_steps, e->hi.R), &e->kt);
     if (rc) return rc;
     return engine_finish(e);
 }
@@ -812,17 +840,27 @@ int engine_run_output(Engine *e, int n_steps, int use_graph, int graph_steps, in
 {
     const int chunk = chunk_for(graph_steps, e->hi.R);
     int rc = SCAFFOLD_OK;
+    // the train of steps [first_out_step, first_out_step + T_out) is gathered by
+    // the end of the call's last run when the window closes the call and that run
+    // holds all of it; otherwise by a launch of its own after the runs
+    ScOut out;
+    out.bits = e->out_bits;
+    out.out = e->out_hd ? e->out_hd : e->out_d;
+    out.T = e->T_out;
+    out.n_out = e->n_out;
+    const bool closes = train && out.T * out.n_out > 0 && first_out_step + out.T == step0 + n_steps;
+    bool taken = false;
     if (!rec) {
-        rc = run_steps(e->stream, &e->gexec, &e->gexec_steps, &e->hi, e->di, 1, e->w, n_steps, use_graph, chunk,
-                       &e->kt, &e->gexec_k, &e->gexec_k_steps);
+        rc = run_steps(e->stream, e->graphs, &e->hi, e->di, 1, e->w, n_steps, use_graph, chunk, &e->kt,
+                       closes ? &out : nullptr, &taken);
         if (rc) return rc;
     }
     for (int done = 0; rec && done < n_steps;) {
         const int64_t s = step0 + done;
         const int K = (int)std::min<int64_t>(n_steps - done, (s / rec_T + 1) * rec_T - s);
         if ((rc = recorder_before(rec, e->stream, s + K))) return rc;
-        rc = run_steps(e->stream, &e->gexec, &e->gexec_steps, &e->hi, e->di, 1, e->w, K, use_graph, chunk, &e->kt,
-                       &e->gexec_k, &e->gexec_k_steps);
+        rc = run_steps(e->stream, e->graphs, &e->hi, e->di, 1, e->w, K, use_graph, chunk, &e->kt,
+                       closes && done + K == n_steps ? &out : nullptr, &taken);
         if (rc) return rc;
         if ((s + K) % rec_T == 0 && (rc = recorder_window(rec, e->stream, e->hi.ring, s + K))) return rc;
         done += K;
@@ -831,9 +869,10 @@ int engine_run_output(Engine *e, int n_steps, int use_graph, int graph_steps, in
         const int R = e->hi.R;
         const int row0 = (int)(((first_out_step % R) + R) % R);
         const size_t nb = (size_t)e->T_out * (size_t)e->n_out;
-        launch_out_train(e->stream, e->hi.ring, e->hi.Wr, R, row0, e->T_out, e->out_bits, e->n_out,
-                         e->out_hd ? e->out_hd : e->out_d);
-        HIP_TRY(hipGetLastError());
+        if (!taken) {
+            launch_out_train(e->stream, e->hi.ring, e->hi.Wr, R, row0, e->T_out, e->out_bits, e->n_out, out.out);
+            HIP_TRY(hipGetLastError());
+        }
         if (!e->out_hd && nb) HIP_TRY(hipMemcpyAsync(e->out_h, e->out_d, nb, hipMemcpyDeviceToHost, e->stream));
         *train = e->out_h;  // complete once engine_finish has synchronized the stream
     }
@@ -864,8 +903,7 @@ int engine_time(Engine *e, int n_steps, int use_graph, int graph_steps, double *
     const int chunk = chunk_for(graph_steps, e->hi.R);
     if (!ms_kernel_mean) {
         HIP_TRY(hipEventRecord(a, e->stream));
-        int rc = run_steps(e->stream, &e->gexec, &e->gexec_steps, &e->hi, e->di, 1, e->w, n_steps, use_graph, chunk,
-                           &e->kt, &e->gexec_k, &e->gexec_k_steps);
+        int rc = run_steps(e->stream, e->graphs, &e->hi, e->di, 1, e->w, n_steps, use_graph, chunk, &e->kt);
         if (rc) return rc;
         HIP_TRY(hipEventRecord(b, e->stream));
     } else {
@@ -887,8 +925,7 @@ int engine_time(Engine *e, int n_steps, int use_graph, int graph_steps, double *
                 launch_plastic(e->stream, e->hi, e->w, k);
             }
             HIP_TRY(hipEventRecord(s1, e->stream));
-            launch_stats(e->stream, &e->hi, e->di, 1, K, e->w.max_groups);
-            launch_advance(e->stream, e->di, 1, K);
+            launch_end(e->stream, &e->hi, e->di, 1, e->w, K, nullptr);
             done += K;
         }
         HIP_TRY(hipEventRecord(b, e->stream));
@@ -1042,15 +1079,14 @@ struct Batch {
     int n_inst = 0;
     int min_R = 0;
     DevWork w;
-    hipGraphExec_t gexec = nullptr;
-    int gexec_steps = 0;
+    RunGraphs graphs;
 };
 
 void batch_destroy(Batch *b)
 {
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
-    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    b->graphs.destroy();
     for (void *p : b->allocs) (void)hipFree(p);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -1122,7 +1158,7 @@ int batch_run(Batch *b, int n_steps, int graph_steps, double *ms)
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     HIP_TRY(hipEventRecord(e0, b->stream));
-    int rc = run_steps(b->stream, &b->gexec, &b->gexec_steps, nullptr, b->di, b->n_inst, b->w, n_steps, 1,
+    int rc = run_steps(b->stream, b->graphs, nullptr, b->di, b->n_inst, b->w, n_steps, 1,
                        chunk_for(graph_steps, b->min_R));
     if (rc) return rc;
     HIP_TRY(hipEventRecord(e1, b->stream));

@@ -18,10 +18,11 @@
 //         after the step and updates the rule's multipliers.
 // A step reads ring rows of earlier steps only (every delay >= 1) and writes the
 // row of the current step, so blocks never wait on each other; the kernel
-// boundary orders consecutive steps.  k_stats popcounts the spike words of a run
-// of steps into per-population counts and sums the out-degrees of the spiking
-// neurons (synaptic events); k_advance moves the device step counter.  Runs of
-// steps are captured once as a hipGraph and replayed.
+// boundary orders consecutive steps.  k_run_end closes a run of steps: it
+// popcounts the run's spike words into per-population counts, sums the
+// out-degrees of the spiking neurons (synaptic events), gathers the output spike
+// train and moves the device step counters.  Runs of steps are captured once as a
+// hipGraph and replayed.
 //
 // Blocks whose gathers read a bounded range of presynaptic neurons first copy
 // those spike words ("windows", planned per block by the engine) into LDS with
@@ -1125,11 +1126,13 @@ __global__ void __launch_bounds__(256) k_stats_zero(ScInstance I0, const ScInsta
     if (threadIdx.x == 0) I.events[row] = 0ull;
 }
 
+// One statistics block (block `b` of K x instances x nchunk; blockDim = SC_STATS_CHUNK / 2).
 template <bool SINGLE>
-__global__ void __launch_bounds__(1024) k_stats(ScInstance I0, const ScInstance *__restrict__ insts, int K, int nchunk)
+__device__ __forceinline__ void stats_block(const ScInstance &I0, const ScInstance *__restrict__ insts, int K,
+                                            int nchunk, int b)
 {
     __shared__ unsigned long long acc[SC_MAX_POPS + 1];
-    const int chunk = blockIdx.x % nchunk, rest = blockIdx.x / nchunk;
+    const int chunk = b % nchunk, rest = b / nchunk;
     const int inst = rest / K, kk = rest - inst * K;
     const ScInstance &I = inst_of<SINGLE>(I0, insts, inst);
     const int g0 = chunk * SC_STATS_CHUNK;
@@ -1190,16 +1193,49 @@ __global__ void __launch_bounds__(1024) k_stats(ScInstance I0, const ScInstance 
     }
 }
 
-__global__ void k_advance(const ScInstance *__restrict__ insts, int n_inst, int K)
+// The end of a run of K steps in one launch: blocks [0, n_stats) are the
+// statistics blocks above; the blocks after them (one instance, O.out set) write
+// the output spike train of the run's last O.T steps: byte (s, c) is the spike of
+// output neuron c (device bit O.bits[c]) at step s, row-major by step, as
+// loop_n_steps hands it over (GPUScaffoldSimulation.cu:777).  Every block reads the
+// step counters (the run's ring rows); once it no longer does, it counts itself in
+// *done, and the last block to arrive advances the counters of every instance by K
+// (n, m, ring row; the window-count recount flag cleared) and resets *done for the
+// next launch.
+template <bool SINGLE>
+__global__ void __launch_bounds__(SC_STATS_CHUNK / 2) k_run_end(ScInstance I0, const ScInstance *__restrict__ insts,
+                                                                 int n_inst, int K, int nchunk, int n_stats, ScOut O,
+                                                                 unsigned int *__restrict__ done)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n_inst) {
-        ScCtl *c = insts[i].ctl;
+    __shared__ int last;
+    const int b = blockIdx.x;
+    if (b < n_stats) {
+        stats_block<SINGLE>(I0, insts, K, nchunk, b);
+    } else {
+        const int i = (b - n_stats) * (int)blockDim.x + (int)threadIdx.x;
+        if (i < O.T * O.n_out) {
+            const int st = i / O.n_out, c = i - st * O.n_out;
+            const int row = run_row(I0, K - O.T + st);
+            const int bit = O.bits[c];
+            O.out[i] = (uint8_t)((I0.ring[(int64_t)row * I0.Wr + (bit >> 6)] >> (bit & 63)) & 1ull);
+        }
+    }
+    __syncthreads();  // every read of the counters in this block is done
+    if (threadIdx.x == 0) {
+        __threadfence();
+        last = atomicAdd(done, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    for (int i = threadIdx.x; i < n_inst; i += blockDim.x) {
+        const ScInstance &I = inst_of<SINGLE>(I0, insts, i);
+        ScCtl *c = I.ctl;
         c->n += K;
         c->m += K;
-        c->row = (c->row + K) % insts[i].R;
+        c->row = (c->row + K) % I.R;
         c->recount = 0;
     }
+    if (threadIdx.x == 0) *done = 0u;
 }
 
 __global__ void k_set_ctl(ScCtl *ctl, int64_t n, int64_t m, int row)
@@ -1397,23 +1433,26 @@ void launch_step(hipStream_t s, const ScInstance *single, const ScInstance *inst
     else launch_step_nt<false>(s, single, insts, w, k);
 }
 
-void launch_stats(hipStream_t s, const ScInstance *single, const ScInstance *insts, int n_inst, int K,
-                  int max_groups)
+void launch_run_end(hipStream_t s, const ScInstance *single, const ScInstance *insts, int n_inst, int K,
+                    int max_groups, const ScOut *out, unsigned int *done)
 {
     if (K <= 0 || n_inst <= 0) return;
     const int nchunk = std::max(1, (max_groups + SC_STATS_CHUNK - 1) / SC_STATS_CHUNK);
+    const int n_stats = K * n_inst * nchunk;
+    constexpr int bs = SC_STATS_CHUNK / 2;
+    ScOut o{};
+    int n_out_blocks = 0;
+    if (single && out && out->out && out->T * out->n_out > 0) {
+        o = *out;
+        n_out_blocks = (o.T * o.n_out + bs - 1) / bs;
+    }
     if (single) {
         if (nchunk > 1) k_stats_zero<true><<<K, 256, 0, s>>>(*single, insts, K);
-        k_stats<true><<<K * nchunk, SC_STATS_CHUNK / 2, 0, s>>>(*single, insts, K, nchunk);
+        k_run_end<true><<<n_stats + n_out_blocks, bs, 0, s>>>(*single, insts, 1, K, nchunk, n_stats, o, done);
     } else {
         if (nchunk > 1) k_stats_zero<false><<<K * n_inst, 256, 0, s>>>(ScInstance{}, insts, K);
-        k_stats<false><<<K * n_inst * nchunk, SC_STATS_CHUNK / 2, 0, s>>>(ScInstance{}, insts, K, nchunk);
+        k_run_end<false><<<n_stats, bs, 0, s>>>(ScInstance{}, insts, n_inst, K, nchunk, n_stats, o, done);
     }
-}
-
-void launch_advance(hipStream_t s, const ScInstance *insts, int n_inst, int K)
-{
-    k_advance<<<(n_inst + 63) / 64, 64, 0, s>>>(insts, n_inst, K);
 }
 
 void launch_set_ctl(hipStream_t s, ScCtl *ctl, int64_t n, int64_t m, int row)

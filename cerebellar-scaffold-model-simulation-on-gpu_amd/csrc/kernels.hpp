@@ -27,14 +27,28 @@ struct DevWork {
     int max_groups = 0;               // most 64-neuron groups of one instance (statistics grid)
     std::vector<ScPlastic> plastic;   // plasticity rules, applied in order after each step
     const ScPlastic *d_plastic = nullptr;  // the rules on the device (window-count blocks of the step kernel)
+    unsigned int *end_done = nullptr;      // arrival counter of the run-end kernel (zero between launches)
+};
+
+// Output spike train of a run's last T steps: n_out output neurons at device bits
+// bits[0..n_out), written T x n_out bytes, row-major by step, to out.
+struct ScOut {
+    const int32_t *bits = nullptr;
+    uint8_t *out = nullptr;
+    int T = 0, n_out = 0;
 };
 
 // All launchers are asynchronous on stream s.  `single` != nullptr selects the
 // kernels that take one instance by value; otherwise instances come from `insts`.
 void launch_step(hipStream_t s, const ScInstance *single, const ScInstance *insts, const DevWork &w, int k);
-void launch_stats(hipStream_t s, const ScInstance *single, const ScInstance *insts, int n_inst, int K,
-                  int max_groups);
-void launch_advance(hipStream_t s, const ScInstance *insts, int n_inst, int K);
+// The end of a run of K steps, one launch (plus a zeroing launch when a ring row
+// spans several statistics chunks): per-population counts and synaptic events of
+// the run's rows, the output spike train of its last out->T steps when out is set
+// (one instance only), then the advance of every instance's step counters by K.
+// `done` is a zeroed device counter kept for this launch sequence (the kernel
+// leaves it zero again).
+void launch_run_end(hipStream_t s, const ScInstance *single, const ScInstance *insts, int n_inst, int K,
+                    int max_groups, const ScOut *out, unsigned int *done);
 void launch_set_ctl(hipStream_t s, ScCtl *ctl, int64_t n, int64_t m, int row);
 void launch_init_params(hipStream_t s, const ScInstance *di, int n_threads, uint64_t seed, uint64_t off);
 void launch_compact(hipStream_t s, const uint64_t *row, int n_groups, const ScGroup *groups, const ScPop *pops,

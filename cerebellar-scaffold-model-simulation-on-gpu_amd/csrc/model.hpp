@@ -71,7 +71,7 @@ struct Options {
     int index_bits = 32;
     int lane_max_fanin = 48;
     int lane_synapses = 0;    // team mode: target synapses per lane of a neuron's team (0: by network size)
-    int graph = 1;
+    int graph = 0;            // 1: replay captured hipGraphs of whole runs (0: direct launches)
     int graph_steps = 25;
     int reset_g = 0;
     int stage_spikes = 1;

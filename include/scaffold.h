@@ -244,8 +244,14 @@ int scaffold_set_input_stim_ids(scaffold_env *env, int neuron_index,
  *                       0 (default): 8 when one step's synapse and state streams
  *                       fit in 64 MB (the step is latency-bound: wide teams),
  *                       else 64 (throughput-bound: more synapses per block)
- *   "graph"             1: replay captured hipGraphs of the step kernels
- *   "graph_steps"       exchange steps per captured graph
+ *   "graph"             0 (default): launch every step kernel directly; 1: replay
+ *                       captured hipGraphs of whole runs (on ROCm 7 the direct
+ *                       launches measured as fast or faster at every size: the
+ *                       host keeps ahead of 10 us step kernels either way, and
+ *                       graph-launched step kernels ran ~2 % longer)
+ *   "graph_steps"       exchange steps per run: a run's statistics, output train
+ *                       and counter advance are one launch at its end (the length
+ *                       of a captured graph with "graph" 1)
  *   "reset_g"           1: also zero conductances in scaffold_reset_network_env
  *                       (the reference keeps them, GPUScaffoldSimulation.cu:802-854)
  *   "stage_spikes"      1 (default): blocks whose gathers scatter over a bounded
@@ -482,10 +488,9 @@ int scaffold_time_steps(scaffold_env *env, int n_steps, double *ms_total,
 /* Step-kernel timer (no counterpart in the reference): on = 1 clears and starts
  * it, 0 stops it, -1 only reads it.  While it runs, every run of steps
  * (scaffold_loop_n_steps, scaffold_time_steps) records HIP events on the
- * engine's stream around its step kernels: around each replay of a graph that
- * holds the run's step kernels alone (the run's statistics and counter advance
- * follow it as two direct launches), or around the step kernels of a run
- * launched directly.  *ms receives the measured time, *launches the step
+ * engine's stream around its step kernels: around the step kernels of each run
+ * (launched directly, or with "graph" 1 as a replay of a graph that holds the
+ * run's step kernels alone; the run's end follows as a direct launch).  *ms receives the measured time, *launches the step
  * kernels it covers (either may be NULL). */
 int scaffold_step_timer(scaffold_env *env, int on, double *ms, int64_t *launches);
 
@@ -505,7 +510,7 @@ int scaffold_step_timer(scaffold_env *env, int on, double *ms, int64_t *launches
 typedef struct {
     int32_t lane_max_fanin, lane_synapses, graph_steps;
     int32_t n_tried;
-    double ms_per_step;     /* of the winner, graph replays included */
+    double ms_per_step;     /* of the winner, run ends included */
     double ms_default;      /* of the default options, for comparison */
     int32_t index_bits, stage_spikes;
 } scaffold_tune_result;
@@ -513,8 +518,9 @@ int scaffold_autotune(scaffold_env *env, int n_steps, scaffold_tune_result *out)
 /* One trial of the autotuner under the environment's current options (no
  * counterpart in the reference): before scaffold_initialize_network_env, builds a
  * private engine of the described network, initializes it, runs 50 warm-up steps
- * and times n_steps graph-replayed steps; *ms_per_step is their device time per
- * step, *kernel_ms (may be NULL) the mean step-kernel duration.  The environment
+ * and times n_steps steps (launched as the "graph" option says); *ms_per_step is
+ * their device time per step, *kernel_ms (may be NULL) the mean step-kernel
+ * duration.  The environment
  * is left as it was. */
 int scaffold_time_layout(scaffold_env *env, int n_steps, double *ms_per_step, double *kernel_ms);
 

@@ -227,7 +227,7 @@ def test_stimulus_registration(pkg, tmp_path):
 
 def test_options(pkg):
     env = new_env(pkg, 0)
-    defaults = {"weight_precision": 0, "index_bits": 32, "graph": 1, "graph_steps": 25, "lane_max_fanin": 48,
+    defaults = {"weight_precision": 0, "index_bits": 32, "graph": 0, "graph_steps": 25, "lane_max_fanin": 48,
                 "lane_synapses": 0, "stage_ratio": 0}
     for k, v in defaults.items():
         assert env.get_option(k) == v, k
