@@ -326,7 +326,7 @@ def scaled(pkg, preset, steps, precisions):
                      "synaptic_events_per_s": (ev1 - ev0) / wall, "kernel_mean_ms": kmean,
                      "algorithmic_bytes_per_step": algo, "achieved_GBs": algo / (kmean * 1e-3) / 1e9,
                      "frac": algo / (kmean * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "synapses_16bit": info.synapses_16bit}
+                     "synapses_16bit": info.synapses_16bit, "word_group_synapses": info.word_group_synapses}
         out["neurons"], out["synapses"] = info.total_neurons, info.total_synapses
         env.finalize()
     out["steps"] = steps
@@ -425,6 +425,7 @@ def main():
                    "exchange_steps_timed": r["exchange_steps"], "simulated_ms": r["exchange_steps"],
                    "first_timed_exchange_step": args.warmup * WINDOW,
                    "column_index_bits": 32, "neurons": info.total_neurons, "synapses": info.total_synapses,
+                   "word_groups": info.word_groups, "word_group_synapses": info.word_group_synapses,
                    "synapses_per_projection": proj_nnz(env),
                    "input": "4 Hz mossy Poisson + 100 Hz burst to 50 fibres, 300-350 ms",
                    "launches": "per call: 25 direct step-kernel launches, then one run-end launch (statistics, "

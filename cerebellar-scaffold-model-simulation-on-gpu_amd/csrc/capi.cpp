@@ -490,7 +490,11 @@ int scaffold_set_option(scaffold_env *env, const char *key, int64_t value)
         if (value < 1 || value > 100000) return fail(SCAFFOLD_ERR_INVALID_ARG, "graph_steps in [1,100000]");
         o.graph_steps = (int)value;
     } else if (!strcmp(key, "reset_g")) o.reset_g = value ? 1 : 0;
-    else if (!strcmp(key, "stage_spikes")) {
+    else if (!strcmp(key, "word_groups")) {
+        if (frozen) return frozen_err();
+        if (value < -1 || value > 64) return fail(SCAFFOLD_ERR_INVALID_ARG, "word_groups in [-1, 64]");
+        o.word_groups = (int)value;
+    } else if (!strcmp(key, "stage_spikes")) {
         if (frozen) return frozen_err();
         o.stage_spikes = value ? 1 : 0;
     } else if (!strcmp(key, "stage_ratio")) {
@@ -521,6 +525,7 @@ int scaffold_get_option(scaffold_env *env, const char *key, int64_t *value)
     else if (!strcmp(key, "graph_steps")) *value = o.graph_steps;
     else if (!strcmp(key, "reset_g")) *value = o.reset_g;
     else if (!strcmp(key, "stage_spikes")) *value = o.stage_spikes;
+    else if (!strcmp(key, "word_groups")) *value = o.word_groups;
     else if (!strcmp(key, "stage_ratio")) *value = o.stage_ratio;
     else if (!strcmp(key, "nontemporal")) *value = o.nontemporal;
     else return fail(SCAFFOLD_ERR_INVALID_ARG, "unknown option %s", key);
@@ -738,6 +743,8 @@ int scaffold_get_info(scaffold_env *env, scaffold_info *out)
         engine_blocks(env->eng, &i.n_blocks, &i.n_staged_blocks);
         i.lane_synapses = env->plan.lane_synapses;
         i.stage_ratio = env->plan.stage_ratio;
+        i.word_groups = env->plan.n_grp;
+        i.word_group_synapses = env->plan.grp_syn;
     }
     *out = i;
     return SCAFFOLD_OK;

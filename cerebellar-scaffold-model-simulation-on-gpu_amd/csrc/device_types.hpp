@@ -111,6 +111,16 @@ struct ScCtl {
 #define SC_Q_SHIFT  29
 #define SC_BIT_MASK 0x1FFFFFFFu
 
+// Word groups: a team-mode projection with enough synapses per 64-bit spike word
+// of its presynaptic neurons (over all its rows; 8 by default) is held as one
+// 16-byte record per (post neuron, presynaptic word): x = slot << SC_Q_SHIFT |
+// word index, y = index of the first multiplier, z/w = mask of the connected
+// neurons of the word (bits 0-31 / 32-63), their multipliers in ascending bit
+// order.  A gather reads the record and the spike word, and multipliers only
+// for the set bits of (spikes & mask).  For the team width a group counts as
+// SC_GROUP_WORK merged-row entries.
+#define SC_GROUP_WORK    8
+
 // Planner form of one unit of work.
 struct ScTask {
     int32_t kind;
@@ -189,9 +199,10 @@ static_assert(sizeof(ScStage) / 4 <= 64, "a block's window table is loaded with 
 struct ScTeamTask {        // one neuron of a team-mode population
     const void *col;       // merged row columns (32-bit words, or 16-bit offsets)
     const uint32_t *hdr;   // 16-bit form: group headers of the row (nullptr: 32-bit)
-    const void *val;       // merged row multipliers (nullptr: unit)
+    const void *val;       // merged row multipliers (nullptr: unit); the word groups' follow
+    const uint32_t *grp;   // the neuron's word groups (4 words each; nullptr: none)
     int32_t inst, group, bit, pop;
-    int32_t n_in, wprec, len, pad_;
+    int32_t n_in, wprec, len, n_grp;
     ScTeamIn in[SC_MAX_IN];
 };
 

@@ -420,8 +420,14 @@ int engine_create(Engine **out, Plan &P)
     std::vector<const uint8_t *> tcol(P.teams.size(), nullptr);
     std::vector<const uint32_t *> thdr(P.teams.size(), nullptr);
     std::vector<const uint8_t *> tval(P.teams.size(), nullptr);
+    std::vector<const uint32_t *> tgrp(P.teams.size(), nullptr);
     for (size_t ti = 0; ti < P.teams.size(); ++ti) {
         TeamPop &T = P.teams[ti];
+        if (!T.grp.empty()) {
+            uint32_t *g;
+            if ((rc = dupload(A, B, &g, T.grp))) return bail(rc);
+            tgrp[ti] = g;
+        }
         if (T.idx16) {
             uint16_t *c16;
             uint32_t *h;
@@ -443,6 +449,7 @@ int engine_create(Engine **out, Plan &P)
         std::vector<uint16_t>().swap(T.col16);
         std::vector<uint32_t>().swap(T.hdr);
         std::vector<uint8_t>().swap(T.val);
+        std::vector<uint32_t>().swap(T.grp);
     }
 
     // lower the planner's work list to descriptors with absolute pointers
@@ -467,6 +474,8 @@ int engine_create(Engine **out, Plan &P)
             // 16-bit form: the padded group length; 32-bit: the synapse count
             d.len = T.idx16 ? (int)(T.row_ptr[j + 1] - T.row_ptr[j])  // chunks
                             : (pp.n_in ? T.seg_end[(size_t)j * pp.n_in + pp.n_in - 1] : 0);
+            d.n_grp = (int)(T.grp_ptr[j + 1] - T.grp_ptr[j]);
+            d.grp = d.n_grp ? tgrp[ti] + 4 * T.grp_ptr[j] : nullptr;
             TaskWin tw;
             tw.n = pp.n_in;
             tw.segs = T.row_seg[j];
@@ -1112,7 +1121,12 @@ int batch_build(Batch **out, std::vector<Engine *> &engines, const std::vector<c
     std::vector<ScInstance> insts;
     std::vector<ScLaneTask> lanes;
     std::vector<ScStimTask> stims;
-    std::vector<std::pair<int, ScTeamTask>> teams;
+    struct TeamEntry {
+        int width;  // wavefronts
+        size_t win; // index of its ranges in team_win_u
+        ScTeamTask t;
+    };
+    std::vector<TeamEntry> teams;
     std::vector<TaskWin> lane_win, team_win_u;
     b->min_R = 1 << 30;
     for (size_t i = 0; i < engines.size(); ++i) {
@@ -1125,20 +1139,18 @@ int batch_build(Batch **out, std::vector<Engine *> &engines, const std::vector<c
         for (size_t j = 0; j < e->h_teams.size(); ++j) {
             ScTeamTask t = e->h_teams[j];
             t.inst = (int)i;
-            t.pad_ = (int32_t)team_win_u.size();  // index of its ranges, restored below
-            teams.push_back({e->h_team_t[j], t});
+            teams.push_back({e->h_team_t[j], team_win_u.size(), t});
             team_win_u.push_back(e->h_team_win[j]);
         }
     }
-    std::stable_sort(teams.begin(), teams.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
+    std::stable_sort(teams.begin(), teams.end(), [](const TeamEntry &x, const TeamEntry &y) { return x.width > y.width; });
     std::vector<ScTeamTask> tt;
     std::vector<int> tw;
     std::vector<TaskWin> team_win;
-    for (auto &p : teams) {
-        tw.push_back(p.first);
-        team_win.push_back(team_win_u[(size_t)p.second.pad_]);
-        p.second.pad_ = 0;
-        tt.push_back(p.second);
+    for (const TeamEntry &p : teams) {
+        tw.push_back(p.width);
+        team_win.push_back(team_win_u[p.win]);
+        tt.push_back(p.t);
     }
     int rc;
     if ((rc = dupload(b->allocs, nullptr, &b->di, insts))) return bail(rc);

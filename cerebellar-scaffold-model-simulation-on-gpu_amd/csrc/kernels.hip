@@ -546,11 +546,57 @@ struct TeamHead {
     const void *col;
     const uint32_t *hdr;
     const void *val;
-    int32_t inst, group, bit, pop, n_in, wprec, len;
+    const uint32_t *grp;
+    int32_t inst, group, bit, pop, n_in, wprec, len, n_grp;
 };
 __device__ __forceinline__ TeamHead team_head(const ScTeamTask *T)
 {
-    return TeamHead{T->col, T->hdr, T->val, T->inst, T->group, T->bit, T->pop, T->n_in, T->wprec, T->len};
+    return TeamHead{T->col, T->hdr, T->val, T->grp, T->inst, T->group, T->bit, T->pop, T->n_in, T->wprec, T->len,
+                    T->n_grp};
+}
+
+// One multiplier (index i of a row's array) in storage WP.
+template <int WP>
+__device__ __forceinline__ float wone(const void *val, uint32_t i)
+{
+    if constexpr (WP == SC_W_F32) return wdec<WP>(gbl(static_cast<const uint32_t *>(val))[i]);
+    else if constexpr (WP == SC_W_BF16) return wdec<WP>(gbl(static_cast<const uint16_t *>(val))[i]);
+    else if constexpr (WP == SC_W_FP8) return wdec<WP>(gbl(static_cast<const uint8_t *>(val))[i]);
+    else return 1.f;
+}
+
+// Word groups of a team (thread tid of TT): one 16-byte record per group, then
+// the group's spike word -- both halves from LDS in a staged block unless its
+// projection was left in memory (gmask), each half read only where the mask has
+// bits -- then one multiplier per set bit of (spikes & mask), at the group's
+// first index plus the number of mask bits below it.  Spiking entries add to the
+// team's f64 LDS accumulator of the projection, as in the merged rows.
+template <int WP, bool STAGED, bool NT>
+__device__ __forceinline__ void team_groups(const TeamHead &T, gptr<uint32_t> ring32, const uint32_t *win,
+                                            const uint32_t *rowoff, uint32_t gmask, int tid, int TT, double *acc)
+{
+    gptr<u32x4> grp = gbl(reinterpret_cast<const u32x4 *>(T.grp));
+    const void *val = T.val;
+    for (int b = tid; b < T.n_grp; b += TT) {
+        const u32x4 g = ldg<NT>(grp + b);
+        const uint32_t q = g.x >> SC_Q_SHIFT, wi = rowoff[q] + 2u * (g.x & SC_BIT_MASK);
+        uint32_t lo = 0u, hi = 0u;
+        if (!STAGED || (gmask != 0u && ((gmask >> q) & 1u))) {
+            if (g.z) lo = ring32[wi];
+            if (g.w) hi = ring32[wi + 1];
+        } else {
+            if (g.z) lo = win[wi];
+            if (g.w) hi = win[wi + 1];
+        }
+        const uint64_t mask = ((uint64_t)g.w << 32) | g.z;
+        uint64_t act = (((uint64_t)hi << 32) | lo) & mask;
+        while (act) {
+            const int bit = __ffsll((long long)act) - 1;
+            act &= act - 1;
+            const uint32_t i = g.y + (uint32_t)__popcll(mask & ((1ull << bit) - 1ull));
+            atomicAdd(&acc[q], (double)wone<WP>(val, i));
+        }
+    }
 }
 
 template <int WP, int U, bool IDX16, bool STAGED, bool NT>
@@ -746,6 +792,14 @@ __device__ __forceinline__ void team_dispatch(const TeamHead &T, gptr<uint32_t> 
                                               const uint32_t *rowoff, uint32_t gm, uint32_t zb, int tid, int TT,
                                               double *acc)
 {
+    if (T.n_grp > 0) {
+        switch (T.wprec) {
+        case SC_W_F32: team_groups<SC_W_F32, STAGED, NT>(T, ring32, win, rowoff, gm, tid, TT, acc); break;
+        case SC_W_BF16: team_groups<SC_W_BF16, STAGED, NT>(T, ring32, win, rowoff, gm, tid, TT, acc); break;
+        case SC_W_FP8: team_groups<SC_W_FP8, STAGED, NT>(T, ring32, win, rowoff, gm, tid, TT, acc); break;
+        default: team_groups<SC_W_UNIT, STAGED, NT>(T, ring32, win, rowoff, gm, tid, TT, acc); break;
+        }
+    }
     if (IDX16 && T.hdr) {
         switch (T.wprec) {
         case SC_W_F32: team_gather<SC_W_F32, 8, true, STAGED, NT>(T, ring32, win, rowoff, gm, zb, tid, TT, acc); break;

@@ -72,6 +72,8 @@ struct Options {
     int lane_max_fanin = 48;
     int lane_synapses = 0;    // team mode: target synapses per lane of a neuron's team (0: by network size)
     int graph = 0;            // 1: replay captured hipGraphs of whole runs (0: direct launches)
+    int word_groups = -1;     // team rows hold projections with at least this many synapses per spike word
+                              // as word groups (0: none; -1: by network size)
     int graph_steps = 25;
     int reset_g = 0;
     int stage_spikes = 1;
@@ -128,6 +130,12 @@ struct TeamPop {
     std::vector<uint32_t> q_lo, q_hi;
     std::vector<int64_t> row_seg;
     std::vector<int64_t> q_seg;     // num x n_in: the same per incoming projection
+    // word groups of the dense incoming projections: 4 words per group (projection
+    // slot << 29 | presynaptic 64-bit spike word, index of its first multiplier in
+    // val, mask bits 0-31, mask bits 32-63); grp_ptr: num + 1 group offsets
+    std::vector<uint32_t> grp;
+    std::vector<int64_t> grp_ptr;
+    int64_t n_grp = 0;
 };
 
 // A plasticity rule lowered to the device layout: per synapse of the target
@@ -169,6 +177,7 @@ struct Plan {
     int lane_synapses = 8;  // team width target in effect
     bool nontemporal = false;  // load the synapse streams nontemporal
     int64_t syn16 = 0;  // synapses stored as 16-bit column offsets
+    int64_t n_grp = 0, grp_syn = 0;  // word groups of the team rows and the synapses they hold
     int max_delay = 0;
     std::vector<int> out_pops;            // OUTPUT populations
     int n_output = 0;

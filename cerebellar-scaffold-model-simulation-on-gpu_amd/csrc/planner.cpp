@@ -341,6 +341,13 @@ int plan_build(scaffold_env *env, Plan &P)
     const bool large = P.nnz * (want16 ? 2 : 4) + weight_bytes + (int64_t)ref * 32 > ((int64_t)64 << 20);
     P.lane_synapses = opt.lane_synapses > 0 ? opt.lane_synapses : large ? 64 : 8;
     P.stage_ratio = opt.stage_ratio > 0 ? opt.stage_ratio : large ? 32 : 8;
+    // word groups for team-mode projections with at least group_min synapses per
+    // presynaptic spike word: 4 within 64 MB (at 96.7k cells this takes in the
+    // parallel fibres onto Golgi cells and the stellate/basket inputs of Purkinje
+    // cells besides the parallel fibres onto Purkinje cells), 8 above (measured
+    // best at 16x; at 100x the parallel fibres hold 5 per word and are slower as
+    // groups)
+    const int group_min = opt.word_groups >= 0 ? opt.word_groups : large ? 8 : 4;
     // write one encoded multiplier
     auto put_w = [&](std::vector<uint8_t> &out, int64_t pos, const ScProj &sp, float x) {
         if (sp.wprec == SC_W_F32) memcpy(&out[pos * 4], &x, 4);
@@ -480,7 +487,9 @@ int plan_build(scaffold_env *env, Plan &P)
 
     // team-mode merged rows
     P.teams.clear();
+    P.n_grp = P.grp_syn = 0;
     P.team_of.assign(npop, -1);
+    std::vector<std::pair<uint32_t, int64_t>> srow;  // one row of a dense projection: (column, entry), sorted
     for (int p = 0; p < npop; ++p) {
         if (!team_pop[p]) continue;
         const ScPop &d = P.pops[p];
@@ -488,13 +497,41 @@ int plan_build(scaffold_env *env, Plan &P)
         T.pop = p;
         T.wprec = storage_for(weighted[p], plastic_post[p]);
         const int wb = wbytes_of(T.wprec);
+        // Dense incoming projections -- at least group_min synapses per
+        // 64-bit spike word of their presynaptic neurons, over all rows -- are held
+        // as word groups: per post neuron and presynaptic spike word one 16-byte
+        // record (projection slot << 29 | word index, index of its first multiplier,
+        // 64-bit mask of the connected neurons of the word) and the multipliers of
+        // its set bits, ascending.  A column repeated within a row stays in the
+        // merged row (a mask holds each presynaptic neuron once).
+        bool dense[SC_MAX_IN] = {};
+        auto sort_row = [&](const Projection &src, int j) {
+            srow.clear();
+            for (int64_t e = src.row_ptr[j]; e < src.row_ptr[j + 1]; ++e) srow.push_back({src.col[e], e});
+            std::sort(srow.begin(), srow.end());
+        };
+        for (int q = 0; q < d.n_in && group_min > 0; ++q) {
+            const Projection &src = env->projs[d.in_proj[q]];
+            int64_t words = 0;
+            for (int j = 0; j < d.num; ++j) {
+                sort_row(src, j);
+                for (size_t i = 0; i < srow.size(); ++i)
+                    words += i == 0 || (srow[i].first >> 6) != (srow[i - 1].first >> 6);
+            }
+            dense[q] = !src.col.empty() && (int64_t)src.col.size() >= (int64_t)group_min * words;
+        }
         T.row_ptr.assign(d.num + 1, 0);
         T.seg_end.assign((size_t)d.num * d.n_in, 0);
         for (int j = 0; j < d.num; ++j) {
             int64_t len = 0;
             for (int q = 0; q < d.n_in; ++q) {
                 const Projection &src = env->projs[d.in_proj[q]];
-                len += src.row_ptr[j + 1] - src.row_ptr[j];
+                if (dense[q]) {  // repeated columns only
+                    sort_row(src, j);
+                    for (size_t i = 1; i < srow.size(); ++i) len += srow[i].first == srow[i - 1].first;
+                } else {
+                    len += src.row_ptr[j + 1] - src.row_ptr[j];
+                }
                 T.seg_end[(size_t)j * d.n_in + q] = (int32_t)len;
             }
             // 32-bit form: rows padded to whole 4-entry vectors (16-byte loads)
@@ -503,21 +540,57 @@ int plan_build(scaffold_env *env, Plan &P)
         const int64_t tot = T.row_ptr[d.num];
         T.col.resize(tot);
         if (wb) T.val.assign(tot * wb, 0);
+        T.grp_ptr.assign(d.num + 1, 0);
+        std::vector<uint8_t> gval;  // multipliers of the word groups, group after group
+        int64_t n_gval = 0;
         for (int j = 0; j < d.num; ++j) {
             int64_t pos = T.row_ptr[j];
             for (int q = 0; q < d.n_in; ++q) {
                 const int pi = d.in_proj[q];
                 const Projection &src = env->projs[pi];
                 const ScPop &pre = P.pops[src.pre];
-                for (int64_t e = src.row_ptr[j]; e < src.row_ptr[j + 1]; ++e, ++pos) {
-                    T.col[pos] = ((uint32_t)q << SC_Q_SHIFT) | (uint32_t)(pre.dev_base + src.col[e]);
-                    if (wb) put_w(T.val, pos, P.projs[pi], eff[pi][e]);
-                    if (plastic_proj[pi]) vpos[pi][e] = pos;
+                if (dense[q]) {
+                    sort_row(src, j);
+                    for (size_t i = 0; i < srow.size(); ++i) {
+                        const uint32_t bit = (uint32_t)(pre.dev_base + srow[i].first);
+                        const int64_t e = srow[i].second;
+                        if (i > 0 && srow[i].first == srow[i - 1].first) {  // repeated: merged row
+                            T.col[pos] = ((uint32_t)q << SC_Q_SHIFT) | bit;
+                            if (wb) put_w(T.val, pos, P.projs[pi], eff[pi][e]);
+                            if (plastic_proj[pi]) vpos[pi][e] = pos;
+                            ++pos;
+                            continue;
+                        }
+                        const uint32_t word = bit >> 6;
+                        const size_t g = T.grp.size() / 4;
+                        if (g == 0 || (int64_t)g == T.grp_ptr[j] ||
+                            T.grp[4 * (g - 1)] != (((uint32_t)q << SC_Q_SHIFT) | word)) {
+                            T.grp.push_back(((uint32_t)q << SC_Q_SHIFT) | word);
+                            T.grp.push_back((uint32_t)n_gval);
+                            T.grp.push_back(0u);
+                            T.grp.push_back(0u);
+                        }
+                        const size_t gl = T.grp.size() - 4;
+                        T.grp[gl + 2 + ((bit >> 5) & 1)] |= 1u << (bit & 31);
+                        if (wb) {
+                            gval.resize((size_t)(n_gval + 1) * wb);
+                            put_w(gval, n_gval, P.projs[pi], eff[pi][e]);
+                        }
+                        if (plastic_proj[pi]) vpos[pi][e] = -1 - n_gval;  // placed after the merged rows below
+                        ++n_gval;
+                    }
+                } else {
+                    for (int64_t e = src.row_ptr[j]; e < src.row_ptr[j + 1]; ++e, ++pos) {
+                        T.col[pos] = ((uint32_t)q << SC_Q_SHIFT) | (uint32_t)(pre.dev_base + src.col[e]);
+                        if (wb) put_w(T.val, pos, P.projs[pi], eff[pi][e]);
+                        if (plastic_proj[pi]) vpos[pi][e] = pos;
+                    }
                 }
             }
             // padding repeats the last entry (a bit the row reads anyway) with a
             // zero multiplier; the kernel masks it by the row length as well
             for (; pos < T.row_ptr[j + 1]; ++pos) T.col[pos] = T.col[pos - 1];
+            T.grp_ptr[j + 1] = (int64_t)T.grp.size() / 4;
         }
         // bit ranges per neuron and projection, segments per neuron (per 64 entries
         // of a projection segment spanning b bits: min(64, b / 512 + 1))
@@ -541,6 +614,22 @@ int plan_build(scaffold_env *env, Plan &P)
                     const int64_t sg = std::min<int64_t>(SC_WAVE, (hi - lo) / 512 + 1);
                     T.row_seg[j] += sg;
                     T.q_seg[(size_t)j * d.n_in + q] += sg;
+                }
+            }
+        // word groups: the bits they read and the 64-byte segments of their words
+        for (int j = 0; j < d.num; ++j)
+            for (int64_t g = T.grp_ptr[j]; g < T.grp_ptr[j + 1]; ++g) {
+                const uint32_t *x = &T.grp[4 * g];
+                const int q = (int)(x[0] >> SC_Q_SHIFT);
+                const uint32_t word = x[0] & SC_BIT_MASK;
+                const uint64_t m = (uint64_t)x[2] | ((uint64_t)x[3] << 32);
+                const size_t k = (size_t)j * d.n_in + q;
+                T.q_lo[k] = std::min(T.q_lo[k], 64 * word + (uint32_t)__builtin_ctzll(m));
+                T.q_hi[k] = std::max(T.q_hi[k], 64 * word + 63 - (uint32_t)__builtin_clzll(m));
+                if (g == T.grp_ptr[j] || (T.grp[4 * (g - 1)] & SC_BIT_MASK) >> 3 != word >> 3 ||
+                    T.grp[4 * (g - 1)] >> SC_Q_SHIFT != (uint32_t)q) {
+                    T.row_seg[j] += 1;
+                    T.q_seg[k] += 1;
                 }
             }
         if (want16) {
@@ -584,7 +673,8 @@ int plan_build(scaffold_env *env, Plan &P)
                 rp[j + 1] = (int64_t)h.size();
             }
             for (int q = 0; q < d.n_in && !newpos.empty(); ++q)
-                for (int64_t &x : vpos[d.in_proj[q]]) x = newpos[x];
+                for (int64_t &x : vpos[d.in_proj[q]])
+                    if (x >= 0) x = newpos[x];  // (word-group entries are placed below)
             T.idx16 = true;
             T.row_ptr = std::move(rp);
             T.col16 = std::move(c16);
@@ -592,8 +682,30 @@ int plan_build(scaffold_env *env, Plan &P)
             T.val = std::move(v16);
             std::vector<uint32_t>().swap(T.col);
         }
-        // team width: about lane_synapses synapses per lane for the mean row
-        const double per_lane = (double)tot / std::max(1, d.num) / SC_WAVE / std::max(1, P.lane_synapses);
+        // the word groups' multipliers follow the merged rows' in the same array; a
+        // group's first-multiplier index is taken relative to its neuron's row
+        // start (entry row_ptr[j], or 8 x row_ptr[j] in the 16-bit form)
+        if (n_gval > 0) {
+            const int64_t base = wb ? (int64_t)T.val.size() / wb : 0;
+            if (base + n_gval >= (int64_t)UINT32_MAX)
+                return fail(SCAFFOLD_ERR_CAPACITY, "population %d: multipliers beyond 2^32 entries", p);
+            for (int j = 0; j < d.num; ++j) {
+                const int64_t e0 = T.idx16 ? T.row_ptr[j] * 8 : T.row_ptr[j];
+                for (int64_t g = T.grp_ptr[j]; g < T.grp_ptr[j + 1]; ++g)
+                    T.grp[4 * g + 1] = (uint32_t)(T.grp[4 * g + 1] + base - e0);
+            }
+            T.val.insert(T.val.end(), gval.begin(), gval.end());
+            for (int q = 0; q < d.n_in; ++q)
+                for (int64_t &x : vpos[d.in_proj[q]])
+                    if (x < 0) x = base + (-1 - x);
+        }
+        T.n_grp = (int64_t)T.grp.size() / 4;
+        P.n_grp += T.n_grp;
+        P.grp_syn += n_gval;
+        // team width: about lane_synapses synapses per lane for the mean row, a
+        // word group counting as SC_GROUP_WORK entries
+        const double work = (double)tot + (double)SC_GROUP_WORK * (double)T.n_grp;
+        const double per_lane = work / std::max(1, d.num) / SC_WAVE / std::max(1, P.lane_synapses);
         int t = 1;
         while (t < SC_MAX_TEAM && t < per_lane) t <<= 1;
         T.team_waves = t;

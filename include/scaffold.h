@@ -264,7 +264,17 @@ int scaffold_set_input_stim_ids(scaffold_env *env, int neuron_index,
  *   "nontemporal"       -1 (default): the synapse streams (columns, multipliers)
  *                       are loaded nontemporal when one step's exceed 64 MB, so
  *                       they do not evict descriptors and spike rows from L2;
- *                       0: never; 1: always.  Results are identical either way */
+ *                       0: never; 1: always.  Results are identical either way
+ *   "word_groups"       n >= 1: a team-mode projection with at least n synapses
+ *                       per 64-bit presynaptic spike word, over all its rows
+ *                       (the parallel fibres onto Purkinje cells of the
+ *                       reference-size scaffold hold 51), is stored as one
+ *                       16-byte record per (post neuron, presynaptic word) --
+ *                       word index, connection mask, first multiplier -- and its
+ *                       multipliers are read only for connected neurons that
+ *                       spiked; 0: every synapse in the merged rows; -1
+ *                       (default): 4 when one step's streams fit in 64 MB, else
+ *                       8 (at most 64).  Results are identical */
 int scaffold_set_option(scaffold_env *env, const char *key, int64_t value);
 int scaffold_get_option(scaffold_env *env, const char *key, int64_t *value);
 
@@ -367,6 +377,8 @@ typedef struct {
     int32_t  n_staged_blocks;      /* of which stage spike windows in LDS      */
     int32_t  lane_synapses;        /* team width target in effect (option or by size) */
     int32_t  stage_ratio;          /* staging threshold in effect (option or by size) */
+    int64_t  word_groups;          /* word-group records of the team rows      */
+    int64_t  word_group_synapses;  /* synapses they hold                        */
 } scaffold_info;
 int scaffold_get_info(scaffold_env *env, scaffold_info *out);
 

@@ -227,8 +227,8 @@ def test_stimulus_registration(pkg, tmp_path):
 
 def test_options(pkg):
     env = new_env(pkg, 0)
-    defaults = {"weight_precision": 0, "index_bits": 32, "graph": 0, "graph_steps": 25, "lane_max_fanin": 48,
-                "lane_synapses": 0, "stage_ratio": 0}
+    defaults = {"weight_precision": 0, "index_bits": 32, "graph": 0, "graph_steps": 25, "word_groups": -1,
+                "lane_max_fanin": 48, "lane_synapses": 0, "stage_ratio": 0}
     for k, v in defaults.items():
         assert env.get_option(k) == v, k
     env.set_option("weight_precision", pkg.WEIGHTS_BF16)
@@ -247,11 +247,16 @@ def test_options(pkg):
     assert env.get_option("stage_ratio") == 32
     env.set_option("lane_synapses", 64)
     assert env.get_option("lane_synapses") == 64
+    env.set_option("word_groups", 0)
+    assert env.get_option("word_groups") == 0
+    env.set_option("word_groups", 64)
+    assert env.get_option("word_groups") == 64
     for k, v, code in [("no_such_option", 1, "ERR_INVALID_ARG"), ("weight_precision", 7, "ERR_INVALID_ARG"),
                        ("index_bits", 8, "ERR_INVALID_ARG"), ("graph_steps", 0, "ERR_INVALID_ARG"),
                        ("nontemporal", 2, "ERR_INVALID_ARG"), ("stage_ratio", -1, "ERR_INVALID_ARG"),
                        ("stage_ratio", 1000001, "ERR_INVALID_ARG"), ("lane_synapses", -1, "ERR_INVALID_ARG"),
-                       ("lane_synapses", 4097, "ERR_INVALID_ARG")]:
+                       ("lane_synapses", 4097, "ERR_INVALID_ARG"), ("word_groups", -2, "ERR_INVALID_ARG"),
+                       ("word_groups", 65, "ERR_INVALID_ARG")]:
         with pytest.raises(pkg.ScaffoldError) as e:
             env.set_option(k, v)
         assert e.value.code == getattr(pkg, code), k

@@ -99,6 +99,30 @@ def test_scaffold96k_bit_exact(pkg, orc, stage):
     env.finalize()
 
 
+@pytest.mark.parametrize("word_groups,index_bits,stage", [(0, 32, 1), (2, 32, 1), (2, 16, 1), (1, 16, 0), (64, 32, 1)])
+def test_word_groups_bit_exact(pkg, orc, word_groups, index_bits, stage):
+    """Team rows whose dense projections are held as word groups (one record per
+    presynaptic spike word: word index, connection mask, first multiplier) at
+    several density thresholds -- none (0), every team projection (1, 2), only the
+    densest (64: none of this network) -- in both column forms, staged and reading
+    the ring in memory: the same raster and state as the oracle."""
+    env = new_env(pkg, 0)
+    env.set_option("word_groups", word_groups)
+    env.set_option("index_bits", index_bits)
+    env.set_option("stage_spikes", stage)
+    env.initialize(50, 25)
+    info = env.info()
+    if word_groups in (1, 2):
+        assert info.word_groups > 0 and info.word_group_synapses > info.word_groups
+    elif word_groups in (0, 64):
+        assert info.word_groups == 0
+    ref = orc.OracleSim.from_env(env)
+    ref.initialize()
+    run_and_compare(env, ref, 0, 200)
+    assert_same_state(env, ref, f"word_groups {word_groups}, {index_bits}-bit columns, stage {stage}")
+    env.finalize()
+
+
 @pytest.mark.parametrize("index_bits", [32, 16])
 def test_nontemporal_streams_bit_exact(pkg, orc, index_bits):
     """The step kernels that load the synapse streams nontemporal (chosen on its
