@@ -326,6 +326,8 @@ def scaled(pkg, preset, steps, precisions):
                      "synaptic_events_per_s": (ev1 - ev0) / wall, "kernel_mean_ms": kmean,
                      "algorithmic_bytes_per_step": algo, "achieved_GBs": algo / (kmean * 1e-3) / 1e9,
                      "frac": algo / (kmean * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "layout_bytes_per_step": info.stream_bytes_per_step,
+                     "layout_frac": info.stream_bytes_per_step / (kmean * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "synapses_16bit": info.synapses_16bit, "word_group_synapses": info.word_group_synapses}
         out["neurons"], out["synapses"] = info.total_neurons, info.total_synapses
         env.finalize()
@@ -438,6 +440,11 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
                      "kernel": "k_step", "kernel_mean_ms": kmean, "launches_timed": r["kernel_launches_timed"],
                      "algorithmic_bytes_per_launch": r["algorithmic_bytes_per_step"],
+                     # the bytes the stored layout streams per launch (word groups read one
+                     # 16-byte record per presynaptic spike word instead of an index and a
+                     # multiplier per synapse), and the fraction of the peak they reach
+                     "layout_bytes_per_launch": info.stream_bytes_per_step,
+                     "layout_frac": info.stream_bytes_per_step / (kmean * 1e-3) / 1e9 / HBM_PEAK_GBS if kmean else None,
                      "timing": "HIP events on the step kernels' stream around each call's 25 step kernels, timed region"},
     }
     # the device loop alone (runs of 25 steps, no output train, no host calls between)

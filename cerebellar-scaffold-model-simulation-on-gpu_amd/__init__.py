@@ -75,7 +75,8 @@ class Info(ctypes.Structure):
                 ("n_tasks", ctypes.c_int32), ("n_groups", ctypes.c_int32), ("synapses_16bit", ctypes.c_int64),
                 ("n_blocks", ctypes.c_int32), ("n_staged_blocks", ctypes.c_int32),
                 ("lane_synapses", ctypes.c_int32), ("stage_ratio", ctypes.c_int32),
-                ("word_groups", ctypes.c_int64), ("word_group_synapses", ctypes.c_int64)]
+                ("word_groups", ctypes.c_int64), ("word_group_synapses", ctypes.c_int64),
+                ("stream_bytes_per_step", ctypes.c_int64)]
 
 
 class PopulationInfo(ctypes.Structure):

@@ -745,6 +745,7 @@ int scaffold_get_info(scaffold_env *env, scaffold_info *out)
         i.stage_ratio = env->plan.stage_ratio;
         i.word_groups = env->plan.n_grp;
         i.word_group_synapses = env->plan.grp_syn;
+        i.stream_bytes_per_step = env->plan.stream_bytes;
     }
     *out = i;
     return SCAFFOLD_OK;

@@ -136,6 +136,7 @@ struct TeamPop {
     std::vector<uint32_t> grp;
     std::vector<int64_t> grp_ptr;
     int64_t n_grp = 0;
+    int64_t n_gval = 0;  // synapses in word groups (their multipliers end val)
 };
 
 // A plasticity rule lowered to the device layout: per synapse of the target
@@ -178,6 +179,11 @@ struct Plan {
     bool nontemporal = false;  // load the synapse streams nontemporal
     int64_t syn16 = 0;  // synapses stored as 16-bit column offsets
     int64_t n_grp = 0, grp_syn = 0;  // word groups of the team rows and the synapses they hold
+    // bytes the stored layout streams per step: every column / header / multiplier
+    // of the lane slices and merged rows (padding included), 16 per word-group
+    // record (not the multipliers of spiking connected neurons, which depend on
+    // activity), and the neuron state and spike bits as in algo_bytes
+    int64_t stream_bytes = 0;
     int max_delay = 0;
     std::vector<int> out_pops;            // OUTPUT populations
     int n_output = 0;

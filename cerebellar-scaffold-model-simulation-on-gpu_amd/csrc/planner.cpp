@@ -700,6 +700,7 @@ int plan_build(scaffold_env *env, Plan &P)
                     if (x < 0) x = base + (-1 - x);
         }
         T.n_grp = (int64_t)T.grp.size() / 4;
+        T.n_gval = n_gval;
         P.n_grp += T.n_grp;
         P.grp_syn += n_gval;
         // team width: about lane_synapses synapses per lane for the mean row, a
@@ -840,6 +841,15 @@ int plan_build(scaffold_env *env, Plan &P)
         if (is16) P.syn16 += (int64_t)src.col.size();
     }
     P.algo_bytes = index_bytes + weight_bytes + (int64_t)ref * 32 + ref / 8;
+    P.stream_bytes = (int64_t)ref * 32 + ref / 8;
+    for (int pi = 0; pi < nproj; ++pi) {
+        const HostProjArrays &A = P.parr[pi];
+        if (!env->projs[pi].set || P.projs[pi].mode != SC_MODE_LANE) continue;
+        P.stream_bytes += (int64_t)(A.col.size() * 4 + A.col16.size() * 2 + A.hdr.size() * 4 + A.val.size());
+    }
+    for (const TeamPop &T : P.teams)
+        P.stream_bytes += (int64_t)(T.col.size() * 4 + T.col16.size() * 2 + T.hdr.size() * 4 + T.grp.size() * 4 +
+                                    T.val.size()) - T.n_gval * wbytes_of(T.wprec);
     P.stage = opt.stage_spikes;
     // the synapse streams of one step beyond 64 MB (twice the aggregate L2) are
     // loaded nontemporal, so they do not evict descriptors and spike rows

@@ -379,6 +379,11 @@ typedef struct {
     int32_t  stage_ratio;          /* staging threshold in effect (option or by size) */
     int64_t  word_groups;          /* word-group records of the team rows      */
     int64_t  word_group_synapses;  /* synapses they hold                        */
+    int64_t  stream_bytes_per_step; /* bytes the stored layout streams per step:
+                                      every column, header and multiplier of the
+                                      gather layouts (padding included), 16 per
+                                      word group (not the multipliers read for
+                                      spiking connected neurons), N*32 + N/8 */
 } scaffold_info;
 int scaffold_get_info(scaffold_env *env, scaffold_info *out);
 
