@@ -112,14 +112,12 @@ struct ScCtl {
 #define SC_BIT_MASK 0x1FFFFFFFu
 
 // Word groups: a team-mode projection with enough synapses per 64-bit spike word
-// of its presynaptic neurons (over all its rows; 8 by default) is held as one
+// of its presynaptic neurons (over all its rows) is held as one
 // 16-byte record per (post neuron, presynaptic word): x = slot << SC_Q_SHIFT |
 // word index, y = index of the first multiplier, z/w = mask of the connected
 // neurons of the word (bits 0-31 / 32-63), their multipliers in ascending bit
 // order.  A gather reads the record and the spike word, and multipliers only
-// for the set bits of (spikes & mask).  For the team width a group counts as
-// SC_GROUP_WORK merged-row entries.
-#define SC_GROUP_WORK    8
+// for the set bits of (spikes & mask).
 
 // Planner form of one unit of work.
 struct ScTask {

@@ -565,36 +565,51 @@ __device__ __forceinline__ float wone(const void *val, uint32_t i)
     else return 1.f;
 }
 
-// Word groups of a team (thread tid of TT): one 16-byte record per group, then
-// the group's spike word -- both halves from LDS in a staged block unless its
+// Word groups of a team (thread tid of TT), SC_GROUP_BATCH groups per lane a
+// batch: their 16-byte records (indices past the end clamped to the last group,
+// so every load is unconditional; the extra copies are skipped below), then each
+// group's spike word -- both halves from LDS in a staged block unless its
 // projection was left in memory (gmask), each half read only where the mask has
 // bits -- then one multiplier per set bit of (spikes & mask), at the group's
 // first index plus the number of mask bits below it.  Spiking entries add to the
 // team's f64 LDS accumulator of the projection, as in the merged rows.
+#define SC_GROUP_BATCH 2
 template <int WP, bool STAGED, bool NT>
 __device__ __forceinline__ void team_groups(const TeamHead &T, gptr<uint32_t> ring32, const uint32_t *win,
                                             const uint32_t *rowoff, uint32_t gmask, int tid, int TT, double *acc)
 {
+    constexpr int GB = SC_GROUP_BATCH;
     gptr<u32x4> grp = gbl(reinterpret_cast<const u32x4 *>(T.grp));
     const void *val = T.val;
-    for (int b = tid; b < T.n_grp; b += TT) {
-        const u32x4 g = ldg<NT>(grp + b);
-        const uint32_t q = g.x >> SC_Q_SHIFT, wi = rowoff[q] + 2u * (g.x & SC_BIT_MASK);
-        uint32_t lo = 0u, hi = 0u;
-        if (!STAGED || (gmask != 0u && ((gmask >> q) & 1u))) {
-            if (g.z) lo = ring32[wi];
-            if (g.w) hi = ring32[wi + 1];
-        } else {
-            if (g.z) lo = win[wi];
-            if (g.w) hi = win[wi + 1];
+    const int n = T.n_grp;
+    for (int b = tid; b < n; b += TT * GB) {
+        u32x4 g[GB];
+#pragma unroll
+        for (int u = 0; u < GB; ++u) g[u] = ldg<NT>(grp + min(b + u * TT, n - 1));
+        uint32_t lo[GB], hi[GB];
+#pragma unroll
+        for (int u = 0; u < GB; ++u) {
+            const uint32_t q = g[u].x >> SC_Q_SHIFT, wi = rowoff[q] + 2u * (g[u].x & SC_BIT_MASK);
+            lo[u] = hi[u] = 0u;
+            if (!STAGED || (gmask != 0u && ((gmask >> q) & 1u))) {
+                if (g[u].z) lo[u] = ring32[wi];
+                if (g[u].w) hi[u] = ring32[wi + 1];
+            } else {
+                if (g[u].z) lo[u] = win[wi];
+                if (g[u].w) hi[u] = win[wi + 1];
+            }
         }
-        const uint64_t mask = ((uint64_t)g.w << 32) | g.z;
-        uint64_t act = (((uint64_t)hi << 32) | lo) & mask;
-        while (act) {
-            const int bit = __ffsll((long long)act) - 1;
-            act &= act - 1;
-            const uint32_t i = g.y + (uint32_t)__popcll(mask & ((1ull << bit) - 1ull));
-            atomicAdd(&acc[q], (double)wone<WP>(val, i));
+#pragma unroll
+        for (int u = 0; u < GB; ++u) {
+            if (b + u * TT >= n) break;
+            const uint64_t mask = ((uint64_t)g[u].w << 32) | g[u].z;
+            uint64_t act = (((uint64_t)hi[u] << 32) | lo[u]) & mask;
+            while (act) {
+                const int bit = __ffsll((long long)act) - 1;
+                act &= act - 1;
+                const uint32_t i = g[u].y + (uint32_t)__popcll(mask & ((1ull << bit) - 1ull));
+                atomicAdd(&acc[g[u].x >> SC_Q_SHIFT], (double)wone<WP>(val, i));
+            }
         }
     }
 }

@@ -273,7 +273,7 @@ int scaffold_set_input_stim_ids(scaffold_env *env, int neuron_index,
  *                       word index, connection mask, first multiplier -- and its
  *                       multipliers are read only for connected neurons that
  *                       spiked; 0: every synapse in the merged rows; -1
- *                       (default): 4 when one step's streams fit in 64 MB, else
+ *                       (default): 2 when one step's streams fit in 64 MB, else
  *                       8 (at most 64).  Results are identical */
 int scaffold_set_option(scaffold_env *env, const char *key, int64_t value);
 int scaffold_get_option(scaffold_env *env, const char *key, int64_t *value);
