@@ -210,7 +210,8 @@ def batched(pkg, n_inst, steps, warmup, threads, layout=None, profile_steps=0):
         tuned = probe.autotune(100)
         probe.finalize()
         log(f"batch: autotuned {tuned}")
-        layout = {k: tuned[k] for k in ("lane_max_fanin", "lane_synapses", "graph_steps", "index_bits", "stage_spikes")}
+        layout = {k: tuned[k] for k in ("lane_max_fanin", "lane_synapses", "graph_steps", "index_bits", "stage_spikes",
+                                        "word_groups")}
     else:
         tuned = {"given": dict(layout)}
     envs = [[None] * len(scales), [None] * len(scales)]

@@ -99,7 +99,8 @@ class ProjectionInfo(ctypes.Structure):
 class TuneResult(ctypes.Structure):
     _fields_ = [("lane_max_fanin", ctypes.c_int32), ("lane_synapses", ctypes.c_int32),
                 ("graph_steps", ctypes.c_int32), ("n_tried", ctypes.c_int32), ("ms_per_step", ctypes.c_double),
-                ("ms_default", ctypes.c_double), ("index_bits", ctypes.c_int32), ("stage_spikes", ctypes.c_int32)]
+                ("ms_default", ctypes.c_double), ("index_bits", ctypes.c_int32), ("stage_spikes", ctypes.c_int32),
+                ("word_groups", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class StimInfo(ctypes.Structure):

@@ -1072,13 +1072,14 @@ int scaffold_time_steps(scaffold_env *env, int n_steps, double *ms_total, double
 // initialized like InitializeNetworkEnv, warmed up, timed (and its step kernels
 // timed with the step-kernel timer when kernel_ms is given), destroyed.
 static int trial_time(scaffold_env *env, int T_print, int n_steps, double *ms, double *kernel_ms = nullptr,
-                      int *lane_synapses = nullptr)
+                      int *lane_synapses = nullptr, int *word_groups = nullptr)
 {
     Plan P;
     P.R = 2 * T_print;
     int rc = plan_build(env, P);
     if (rc) return rc;
-    if (lane_synapses) *lane_synapses = P.lane_synapses;  // the value the options resolve to
+    if (lane_synapses) *lane_synapses = P.lane_synapses;  // the values the options resolve to
+    if (word_groups) *word_groups = P.group_min;
     Engine *e = nullptr;
     rc = engine_create(&e, P);
     if (rc) return rc;
@@ -1136,6 +1137,7 @@ int scaffold_autotune(scaffold_env *env, int n_steps, scaffold_tune_result *out)
     r.graph_steps = saved.graph_steps;
     r.index_bits = saved.index_bits;
     r.stage_spikes = saved.stage_spikes;
+    r.word_groups = saved.word_groups;
     // the winner so far as options, with one field changed
     auto trial = [&](auto &&change, double *ms) {
         env->opt = saved;
@@ -1144,6 +1146,7 @@ int scaffold_autotune(scaffold_env *env, int n_steps, scaffold_tune_result *out)
         env->opt.graph_steps = r.graph_steps;
         env->opt.index_bits = r.index_bits;
         env->opt.stage_spikes = r.stage_spikes;
+        env->opt.word_groups = r.word_groups;
         change(env->opt);
         const int trc = trial_time(env, T_print, n_steps, ms);
         env->opt = saved;
@@ -1153,7 +1156,7 @@ int scaffold_autotune(scaffold_env *env, int n_steps, scaffold_tune_result *out)
     // the default trial reports the team width its options resolve to (0 picks by
     // network size), so that the result can be applied to other networks as is
     env->opt = saved;
-    if ((rc = trial_time(env, T_print, n_steps, &r.ms_default, nullptr, &r.lane_synapses))) {
+    if ((rc = trial_time(env, T_print, n_steps, &r.ms_default, nullptr, &r.lane_synapses, &r.word_groups))) {
         env->opt = saved;
         return rc;
     }
@@ -1180,12 +1183,31 @@ int scaffold_autotune(scaffold_env *env, int n_steps, scaffold_tune_result *out)
     if (ms < r.ms_per_step) { r.ms_per_step = ms; r.stage_spikes = other_stage; }
     if ((rc = trial([&](Options &o) { o.graph_steps = 50; }, &ms))) return rc;
     if (ms < r.ms_per_step) { r.ms_per_step = ms; r.graph_steps = 50; }
+    // word groups pay or cost with the firing rates; off, or on at the threshold
+    // the network's size picks when the environment had them off
+    int other_groups = 0;
+    if (r.word_groups == 0) {
+        env->opt = saved;
+        env->opt.word_groups = -1;
+        Plan Q;
+        Q.R = 2 * T_print;
+        if ((rc = plan_build(env, Q))) { env->opt = saved; return rc; }
+        other_groups = Q.group_min;
+        env->opt = saved;
+    }
+    if (other_groups != r.word_groups) {
+        if ((rc = trial([&](Options &o) { o.word_groups = other_groups; }, &ms))) return rc;
+        if (ms < r.ms_per_step) { r.ms_per_step = ms; r.word_groups = other_groups; }
+    } else {
+        ++r.n_tried;  // the same layout: nothing to time
+    }
     env->opt = saved;
     env->opt.lane_max_fanin = r.lane_max_fanin;
     env->opt.lane_synapses = r.lane_synapses;
     env->opt.graph_steps = r.graph_steps;
     env->opt.index_bits = r.index_bits;
     env->opt.stage_spikes = r.stage_spikes;
+    env->opt.word_groups = r.word_groups;
     *out = r;
     return SCAFFOLD_OK;
 }

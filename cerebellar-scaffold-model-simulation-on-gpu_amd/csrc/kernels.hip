@@ -573,7 +573,8 @@ __device__ __forceinline__ float wone(const void *val, uint32_t i)
 // bits -- then one multiplier per set bit of (spikes & mask), at the group's
 // first index plus the number of mask bits below it.  Spiking entries add to the
 // team's f64 LDS accumulator of the projection, as in the merged rows.
-#define SC_GROUP_BATCH 2
+#define SC_GROUP_BATCH 1
+#define SC_GROUP_LOADS 4
 template <int WP, bool STAGED, bool NT>
 __device__ __forceinline__ void team_groups(const TeamHead &T, gptr<uint32_t> ring32, const uint32_t *win,
                                             const uint32_t *rowoff, uint32_t gmask, int tid, int TT, double *acc)
@@ -604,11 +605,26 @@ __device__ __forceinline__ void team_groups(const TeamHead &T, gptr<uint32_t> ri
             if (b + u * TT >= n) break;
             const uint64_t mask = ((uint64_t)g[u].w << 32) | g[u].z;
             uint64_t act = (((uint64_t)hi[u] << 32) | lo[u]) & mask;
+            double *a = &acc[g[u].x >> SC_Q_SHIFT];
+            // the multipliers of up to SC_GROUP_LOADS spiking neurons are loaded
+            // together (a slot without one reloads the group's first multiplier,
+            // which exists), then added
             while (act) {
-                const int bit = __ffsll((long long)act) - 1;
-                act &= act - 1;
-                const uint32_t i = g[u].y + (uint32_t)__popcll(mask & ((1ull << bit) - 1ull));
-                atomicAdd(&acc[g[u].x >> SC_Q_SHIFT], (double)wone<WP>(val, i));
+                uint32_t idx[SC_GROUP_LOADS];
+                bool on[SC_GROUP_LOADS];
+#pragma unroll
+                for (int k = 0; k < SC_GROUP_LOADS; ++k) {
+                    on[k] = act != 0ull;
+                    const int bit = on[k] ? __ffsll((long long)act) - 1 : 0;
+                    idx[k] = g[u].y + (on[k] ? (uint32_t)__popcll(mask & ((1ull << bit) - 1ull)) : 0u);
+                    act &= act - 1;
+                }
+                float wv[SC_GROUP_LOADS];
+#pragma unroll
+                for (int k = 0; k < SC_GROUP_LOADS; ++k) wv[k] = wone<WP>(val, idx[k]);
+#pragma unroll
+                for (int k = 0; k < SC_GROUP_LOADS; ++k)
+                    if (on[k]) atomicAdd(a, (double)wv[k]);
             }
         }
     }

@@ -176,6 +176,7 @@ struct Plan {
     int stage = 1;  // stage spike windows in LDS where the plan finds it pays
     int stage_ratio = 8;
     int lane_synapses = 8;  // team width target in effect
+    int group_min = 0;      // word-group threshold in effect (0: none)
     bool nontemporal = false;  // load the synapse streams nontemporal
     int64_t syn16 = 0;  // synapses stored as 16-bit column offsets
     int64_t n_grp = 0, grp_syn = 0;  // word groups of the team rows and the synapses they hold

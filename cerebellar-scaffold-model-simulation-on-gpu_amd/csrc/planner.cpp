@@ -342,14 +342,13 @@ int plan_build(scaffold_env *env, Plan &P)
     P.lane_synapses = opt.lane_synapses > 0 ? opt.lane_synapses : large ? 64 : 8;
     P.stage_ratio = opt.stage_ratio > 0 ? opt.stage_ratio : large ? 32 : 8;
     // word groups for team-mode projections with at least group_min synapses per
-    // presynaptic spike word, and the merged-row entries a group counts as in
-    // the team width: within 64 MB (latency-bound; at 96.7k cells this takes in
-    // every parallel-fibre projection besides the one onto Purkinje cells) 2 and
-    // 4; above (throughput-bound: fuller blocks) 8 and 2 -- the pairs measured
-    // best at 96.7k and 16x; at 100x the parallel fibres hold 5 per word and are
-    // slower as groups
+    // presynaptic spike word: 2 within 64 MB (at 96.7k cells every parallel-fibre
+    // projection and the stellate and basket cells' own recurrent inputs
+    // qualify, so their rows hold groups only), 8 above (the parallel fibres onto
+    // Purkinje cells at 4x and 16x; at 100x they hold 5 per word and are slower as
+    // groups)
     const int group_min = opt.word_groups >= 0 ? opt.word_groups : large ? 8 : 2;
-    const int group_work = large ? 2 : 4;
+    P.group_min = group_min;
     // write one encoded multiplier
     auto put_w = [&](std::vector<uint8_t> &out, int64_t pos, const ScProj &sp, float x) {
         if (sp.wprec == SC_W_F32) memcpy(&out[pos * 4], &x, 4);
@@ -705,9 +704,11 @@ int plan_build(scaffold_env *env, Plan &P)
         T.n_gval = n_gval;
         P.n_grp += T.n_grp;
         P.grp_syn += n_gval;
-        // team width: about lane_synapses synapses per lane for the mean row, a
-        // word group counting as group_work entries
-        const double work = (double)tot + (double)group_work * (double)T.n_grp;
+        // team width: about lane_synapses synapses per lane for the mean row,
+        // counting the synapses of the word groups (narrower teams for rows of
+        // groups ran faster at low firing rates but up to 2x slower at high ones:
+        // a lane's spiking bits are added one after another)
+        const double work = (double)tot + (double)n_gval;
         const double per_lane = work / std::max(1, d.num) / SC_WAVE / std::max(1, P.lane_synapses);
         int t = 1;
         while (t < SC_MAX_TEAM && t < per_lane) t <<= 1;

@@ -519,17 +519,20 @@ int scaffold_step_timer(scaffold_env *env, int on, double *ms, int64_t *launches
  * {8, 16, 32, 64}, besides the environment's own options -- times n_steps step
  * kernels of each after 50 warm-up steps, then tries, one at a time on the
  * fastest layout, the other column index width (index_bits 32 / 16), the other
- * stage_spikes setting and graph_steps 50, and sets the winning options on env
- * (16 trials; ms_default is the time with the environment's own options, and a
- * lane_synapses of 0 is reported as the value it resolves to for this network,
- * so the result can be applied to other networks as is).  The network's own
- * state, seeds and counters are untouched. */
+ * stage_spikes setting, graph_steps 50 and word groups off (or on at their
+ * default threshold when the environment turned them off), and sets the winning
+ * options on env (17 trials; ms_default is the time with the environment's own
+ * options, and a lane_synapses or word_groups of -1 / 0 picking by size is
+ * reported as the value it resolves to for this network, so the result can be
+ * applied to other networks as is).  The network's own state, seeds and
+ * counters are untouched. */
 typedef struct {
     int32_t lane_max_fanin, lane_synapses, graph_steps;
     int32_t n_tried;
     double ms_per_step;     /* of the winner, run ends included */
     double ms_default;      /* of the default options, for comparison */
     int32_t index_bits, stage_spikes;
+    int32_t word_groups, reserved;
 } scaffold_tune_result;
 int scaffold_autotune(scaffold_env *env, int n_steps, scaffold_tune_result *out);
 /* One trial of the autotuner under the environment's current options (no

@@ -863,7 +863,8 @@ def test_autotune_then_bit_exact(pkg, orc):
     are among the candidates."""
     env = new_env(pkg, 0)
     r = env.autotune(50)
-    assert r["n_tried"] == 16 and r["ms_per_step"] > 0 and r["ms_default"] > 0
+    assert r["n_tried"] == 17 and r["ms_per_step"] > 0 and r["ms_default"] > 0
+    assert env.get_option("word_groups") == r["word_groups"] and r["word_groups"] >= 0
     assert env.get_option("index_bits") == r["index_bits"] in (16, 32)
     assert env.get_option("stage_spikes") == r["stage_spikes"] in (0, 1)
     assert r["ms_per_step"] <= r["ms_default"]
