@@ -18,9 +18,10 @@
 //                  incoming projections (projection after projection, in index
 //                  order); a team of 1..16 wavefronts strides over the row.
 //     Column entries are device bit indices of the presynaptic neuron.
-//   * Row sums are accumulated in f64.  With multipliers that pass the exactness
-//     check every sum is exact, so the order in which lanes and wavefronts combine
-//     them cannot change the result; no floating-point atomics are used anywhere.
+//   * Row sums are accumulated in f64 (team rows: LDS atomics on f64 per
+//     projection).  With multipliers that pass the exactness check every sum is
+//     exact, so the order in which lanes and wavefronts combine them cannot change
+//     the result; no floating-point atomics touch global memory.
 #pragma once
 #include <stdint.h>
 
