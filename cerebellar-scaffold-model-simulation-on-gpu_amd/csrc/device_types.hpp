@@ -16,7 +16,8 @@
 //                  256-byte run; one lane per post neuron;
 //       team mode: one merged CSR row per post neuron with the synapses of all its
 //                  incoming projections (projection after projection, in index
-//                  order); a team of 1..16 wavefronts strides over the row.
+//                  order), and the word groups of its dense projections (below);
+//                  a team of 1..16 wavefronts strides over both.
 //     Column entries are device bit indices of the presynaptic neuron.
 //   * Row sums are accumulated in f64 (team rows: LDS atomics on f64 per
 //     projection).  With multipliers that pass the exactness check every sum is
