@@ -141,6 +141,29 @@ struct Cells {
     std::vector<double> x, z;   // positions (um), sorted by z (ids are z ranks)
 };
 
+// Granule cells are numbered strip by strip: strips kStrip um thin in z, one
+// after another along z, each ordered by x (the parallel-fibre axis).  A
+// parallel-fibre band (|dz| < h, |dx| < 200 um) then covers one run of
+// consecutive granule ids in each strip it crosses, so a Purkinje cell's fibres
+// fill its presynaptic spike words densely at every sheet size (in plain z order
+// the band's cells are interleaved with the rest of the slab: 0.8 x 400 / side
+// of a word at most), while the id range of a band stays the z-slab's, as in z
+// order.  The cells and their connections are drawn in z order; the result maps a
+// z rank to the granule id.
+constexpr double kStrip = 4.0;
+std::vector<int32_t> strip_order(const Cells &c)
+{
+    const int n = (int)c.x.size();
+    std::vector<int32_t> byz(n), newid(n);
+    for (int i = 0; i < n; ++i) byz[i] = i;
+    std::sort(byz.begin(), byz.end(), [&](int32_t a, int32_t b) {
+        const int sa = (int)(c.z[a] / kStrip), sb = (int)(c.z[b] / kStrip);
+        return sa != sb ? sa < sb : c.x[a] != c.x[b] ? c.x[a] < c.x[b] : a < b;
+    });
+    for (int k = 0; k < n; ++k) newid[byz[k]] = k;
+    return newid;
+}
+
 Cells place(int n, double X, double Z, uint64_t seed)
 {
     Rng r(seed);
@@ -338,6 +361,15 @@ int synth_describe(scaffold_env *env, const scaffold_synth_config *cfg)
         P.pos_x.assign(cell[i].x.begin(), cell[i].x.end());
         P.pos_z.assign(cell[i].z.begin(), cell[i].z.end());
     }
+    // granule ids by strips (positions stored under the new ids)
+    const std::vector<int32_t> grc_id = strip_order(cell[GRC]);
+    {
+        Population &P = env->pops[idx[GRC]];
+        for (int i = 0; i < counts[GRC]; ++i) {
+            P.pos_x[grc_id[i]] = cell[GRC].x[i];
+            P.pos_z[grc_id[i]] = cell[GRC].z[i];
+        }
+    }
     const double dens_grc = counts[GRC] / (X * Z);
     Grid g_grc(cell[GRC], 50.0, X, Z);
     auto grid_for = [&](int p, double per_cell) {
@@ -457,6 +489,35 @@ int synth_describe(scaffold_env *env, const scaffold_synth_config *cfg)
                      const int k = r.uni() < 0.3 ? 3 : 2;
                      nearest_pick(cell[GOC], g_goc, cell[post].x[j], cell[post].z[j], k, 8, r, o); });
             break;
+        }
+        if (pre == GRC) {  // granule columns to granule ids, each row ascending again
+            std::vector<std::pair<int32_t, float>> row;
+            for (size_t j = 0; j + 1 < E.row_ptr.size(); ++j) {
+                row.clear();
+                for (int64_t e = E.row_ptr[j]; e < E.row_ptr[j + 1]; ++e)
+                    row.push_back({grc_id[E.col[e]], J ? E.val[e] : 0.f});
+                std::sort(row.begin(), row.end());
+                for (size_t t = 0; t < row.size(); ++t) {
+                    E.col[E.row_ptr[j] + (int64_t)t] = row[t].first;
+                    if (J) E.val[E.row_ptr[j] + (int64_t)t] = row[t].second;
+                }
+            }
+        }
+        if (post == GRC) {  // rows in granule-id order
+            Edges F;
+            F.row_ptr.assign(E.row_ptr.size(), 0);
+            std::vector<int32_t> old_of(counts[GRC]);
+            for (int i = 0; i < counts[GRC]; ++i) old_of[grc_id[i]] = i;
+            for (int k = 0; k < counts[GRC]; ++k)
+                F.row_ptr[k + 1] = F.row_ptr[k] + (E.row_ptr[old_of[k] + 1] - E.row_ptr[old_of[k]]);
+            F.col.reserve(E.col.size());
+            if (J) F.val.reserve(E.val.size());
+            for (int k = 0; k < counts[GRC]; ++k)
+                for (int64_t e = E.row_ptr[old_of[k]]; e < E.row_ptr[old_of[k] + 1]; ++e) {
+                    F.col.push_back(E.col[e]);
+                    if (J) F.val.push_back(E.val[e]);
+                }
+            E = std::move(F);
         }
         float delay = rp.delay;
         if (!cfg->reference_delays) delay = (float)std::min(maxd, std::max(1, (int)rp.delay));

@@ -328,7 +328,10 @@ int scaffold_describe_reference(scaffold_env *env, const char *data_dir,
 
 /* Seeded synthetic scaffold (the network the benchmark configurations name):
  * populations placed on an x-z sheet of side 400*sqrt(scale) um, rule-based
- * connectivity, mossy fibres each feeding a set of glomeruli. */
+ * connectivity, mossy fibres each feeding a set of glomeruli.  Cells are
+ * numbered by z; granule cells by 4 um strips of z, each ordered by x (the
+ * parallel-fibre axis), so a parallel-fibre band holds runs of consecutive
+ * granule ids. */
 typedef struct {
     double   scale;              /* sheet area / (400 um x 400 um)                       */
     int32_t  counts[8];          /* 0 = derive from scale.  Order: granule, glomerulus,
