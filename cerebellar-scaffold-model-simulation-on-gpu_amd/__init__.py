@@ -147,7 +147,7 @@ def load_library(path=None):
         "scaffold_get_option": (ctypes.c_int, [vp, cs, P(i64)]),
         "scaffold_initialize_network_env": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
         "scaffold_loop_n_steps": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, P(ctypes.c_void_p)]),
-        "scaffold_set_input_freq": (ctypes.c_int, [vp, P(f32)]),
+        "scaffold_set_input_freq": (ctypes.c_int, [vp, ctypes.c_void_p]),  # float array (address)
         "scaffold_reset_network_env": (ctypes.c_int, [vp, ctypes.c_int]),
         "scaffold_finalize_network_env": (ctypes.c_int, [vp]),
         "scaffold_describe_reference": (ctypes.c_int, [vp, cs, u64]),
@@ -365,6 +365,8 @@ class NetworkEnvironment:
     # ------------------------------------------------------------ lifecycle
     def initialize(self, T_print=50, T_cpu_sim=25):
         _check(self._lib.scaffold_initialize_network_env(self._h, T_print, T_cpu_sim))
+        info = self.info()
+        self._train_shape = (info.T_cpu_sim, info.n_output_neurons)  # fixed until finalize
 
     def loop_n_steps(self, start_step, n_steps, trial=0, want_output=True):
         """Advance n_steps from start_step.  With want_output, returns the output
@@ -375,16 +377,15 @@ class NetworkEnvironment:
                                                ctypes.byref(out) if want_output else None))
         if not want_output or not out.value:
             return None
-        info = self.info()
-        n = info.T_cpu_sim * info.n_output_neurons
-        if n == 0:
-            return np.zeros((info.T_cpu_sim, 0), dtype=np.uint8)
-        buf = (ctypes.c_uint8 * n).from_address(out.value)
-        return np.frombuffer(buf, dtype=np.uint8).copy().reshape(info.T_cpu_sim, info.n_output_neurons)
+        T, n_out = self._train_shape
+        if T * n_out == 0:
+            return np.zeros((T, 0), dtype=np.uint8)
+        buf = (ctypes.c_uint8 * (T * n_out)).from_address(out.value)
+        return np.frombuffer(buf, dtype=np.uint8).reshape(T, n_out).copy()
 
     def set_input_freq(self, freq_list):
         f = np.ascontiguousarray(freq_list, dtype=np.float32)
-        _check(self._lib.scaffold_set_input_freq(self._h, _ptr(f, ctypes.c_float)))
+        _check(self._lib.scaffold_set_input_freq(self._h, f.ctypes.data))
 
     def reset(self, trial=0):
         _check(self._lib.scaffold_reset_network_env(self._h, trial))

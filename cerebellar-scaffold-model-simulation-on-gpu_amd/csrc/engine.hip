@@ -87,6 +87,7 @@ struct Engine {
     uint8_t *out_d = nullptr;
     uint8_t *out_h = nullptr;
     uint8_t *out_hd = nullptr;  // the device's address of out_h (mapped pinned memory), nullptr: copy instead
+    hipEvent_t train_ev = nullptr;  // recorded once a call's output train is in out_h (with out_hd)
     int n_out = 0, T_out = 0;
     // plastic projections: multiplier array and per-synapse index, for read-back
     struct PlasticView {
@@ -619,6 +620,8 @@ int engine_create(Engine **out, Plan &P)
         void *dp = nullptr;
         if (hipHostGetDevicePointer(&dp, e->out_h, 0) == hipSuccess && dp) e->out_hd = static_cast<uint8_t *>(dp);
         else (void)hipGetLastError();
+        if (e->out_hd && hipEventCreateWithFlags(&e->train_ev, hipEventDisableTiming) != hipSuccess)
+            return bail(fail(SCAFFOLD_ERR_HIP, "hipEventCreate failed"));
     }
     std::vector<ScInstance> one{I};
     if ((rc = dupload(A, B, &e->di, one))) return bail(rc);
@@ -633,6 +636,7 @@ void engine_destroy(Engine *e)
     e->graphs.destroy();
     for (void *p : e->allocs) (void)hipFree(p);
     if (e->out_h) (void)hipHostFree(e->out_h);
+    if (e->train_ev) (void)hipEventDestroy(e->train_ev);
     for (hipEvent_t x : e->kt.pool) (void)hipEventDestroy(x);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
@@ -778,11 +782,17 @@ static int timer_pair(KTimer *kt, int K, hipEvent_t *a, hipEvent_t *b)
 // kernels lie between two events and its end is launched after them.  `out`: the
 // output train of the last out->T steps, gathered by the last run's end when that
 // run holds them all (*out_taken then set; otherwise the caller gathers it).
+// With `train_ev` (one instance), a last run launched directly gathers the train
+// by a launch of its own (its first ring row out_row0), records train_ev after it
+// and only then launches its end, so a caller that waits for the train alone
+// does not wait for the run's statistics (*train_ev_set then set).
 static int run_steps(hipStream_t s, RunGraphs &G, const ScInstance *single, const ScInstance *insts, int n_inst,
                      const DevWork &w, int n_steps, int use_graph, int chunk, KTimer *kt = nullptr,
-                     const ScOut *out = nullptr, bool *out_taken = nullptr)
+                     const ScOut *out = nullptr, bool *out_taken = nullptr, hipEvent_t train_ev = nullptr,
+                     int out_row0 = 0, bool *train_ev_set = nullptr)
 {
     if (out_taken) *out_taken = false;
+    if (train_ev_set) *train_ev_set = false;
     if (n_steps <= 0) return SCAFFOLD_OK;
     const bool timed = kt && kt->on;
     const int last_k = n_steps % chunk ? n_steps % chunk : chunk;  // steps of the last run
@@ -810,7 +820,14 @@ static int run_steps(hipStream_t s, RunGraphs &G, const ScInstance *single, cons
             launch_step_kernels(s, single, insts, w, K);
         }
         if (timed) HIP_TRY(hipEventRecord(b, s));
-        if (timed || !graph) launch_end(s, single, insts, n_inst, w, K, o);
+        if (!graph && o && train_ev && single) {
+            launch_out_train(s, single->ring, single->Wr, single->R, out_row0, o->T, o->bits, o->n_out, o->out);
+            HIP_TRY(hipEventRecord(train_ev, s));
+            if (train_ev_set) *train_ev_set = true;
+            launch_end(s, single, insts, n_inst, w, K, nullptr);
+        } else if (timed || !graph) {
+            launch_end(s, single, insts, n_inst, w, K, o);
+        }
         HIP_TRY(hipGetLastError());
         done += K;
     }
@@ -820,10 +837,12 @@ static int run_steps(hipStream_t s, RunGraphs &G, const ScInstance *single, cons
 
 static int chunk_for(int graph_steps, int R) { return std::max(1, std::min(graph_steps, R - 1)); }
 
-// Waits for the engine's stream, then adds what the step-kernel timer measured.
-static int engine_finish(Engine *e)
+// Waits for the engine's stream (or only for `upto`, an event recorded on it after
+// every timer event), then adds what the step-kernel timer measured.
+static int engine_finish(Engine *e, hipEvent_t upto = nullptr)
 {
-    HIP_TRY(hipStreamSynchronize(e->stream));
+    if (upto) HIP_TRY(hipEventSynchronize(upto));
+    else HIP_TRY(hipStreamSynchronize(e->stream));
     KTimer &kt = e->kt;
     float x = 0.f;
     for (size_t i = 0; i < kt.piece_k.size(); ++i) {
@@ -858,10 +877,15 @@ int engine_run_output(Engine *e, int n_steps, int use_graph, int graph_steps, in
     out.T = e->T_out;
     out.n_out = e->n_out;
     const bool closes = train && out.T * out.n_out > 0 && first_out_step + out.T == step0 + n_steps;
-    bool taken = false;
+    const int R = e->hi.R;
+    const int row0 = (int)(((first_out_step % R) + R) % R);
+    // the call returns once the train is in host memory (mapped pinned memory:
+    // written by the gather itself), not after the last run's statistics
+    hipEvent_t early = e->out_hd ? e->train_ev : nullptr;
+    bool taken = false, early_set = false;
     if (!rec) {
         rc = run_steps(e->stream, e->graphs, &e->hi, e->di, 1, e->w, n_steps, use_graph, chunk, &e->kt,
-                       closes ? &out : nullptr, &taken);
+                       closes ? &out : nullptr, &taken, early, row0, &early_set);
         if (rc) return rc;
     }
     for (int done = 0; rec && done < n_steps;) {
@@ -869,14 +893,12 @@ int engine_run_output(Engine *e, int n_steps, int use_graph, int graph_steps, in
         const int K = (int)std::min<int64_t>(n_steps - done, (s / rec_T + 1) * rec_T - s);
         if ((rc = recorder_before(rec, e->stream, s + K))) return rc;
         rc = run_steps(e->stream, e->graphs, &e->hi, e->di, 1, e->w, K, use_graph, chunk, &e->kt,
-                       closes && done + K == n_steps ? &out : nullptr, &taken);
+                       closes && done + K == n_steps ? &out : nullptr, &taken, early, row0, &early_set);
         if (rc) return rc;
         if ((s + K) % rec_T == 0 && (rc = recorder_window(rec, e->stream, e->hi.ring, s + K))) return rc;
         done += K;
     }
     if (train) {
-        const int R = e->hi.R;
-        const int row0 = (int)(((first_out_step % R) + R) % R);
         const size_t nb = (size_t)e->T_out * (size_t)e->n_out;
         if (!taken) {
             launch_out_train(e->stream, e->hi.ring, e->hi.Wr, R, row0, e->T_out, e->out_bits, e->n_out, out.out);
@@ -885,7 +907,7 @@ int engine_run_output(Engine *e, int n_steps, int use_graph, int graph_steps, in
         if (!e->out_hd && nb) HIP_TRY(hipMemcpyAsync(e->out_h, e->out_d, nb, hipMemcpyDeviceToHost, e->stream));
         *train = e->out_h;  // complete once engine_finish has synchronized the stream
     }
-    return engine_finish(e);
+    return engine_finish(e, early_set ? e->train_ev : nullptr);
 }
 
 int engine_record_sync(Engine *e, Recorder *rec)
