@@ -99,14 +99,18 @@ def test_scaffold96k_bit_exact(pkg, orc, stage):
     env.finalize()
 
 
-@pytest.mark.parametrize("word_groups,index_bits,stage", [(0, 32, 1), (2, 32, 1), (2, 16, 1), (1, 16, 0), (64, 32, 1)])
-def test_word_groups_bit_exact(pkg, orc, word_groups, index_bits, stage):
+@pytest.mark.parametrize("word_groups,index_bits,stage,rate", [(0, 32, 1, 4.0), (2, 32, 1, 4.0), (2, 16, 1, 4.0),
+                                                            (1, 16, 0, 4.0), (64, 32, 1, 4.0), (2, 16, 1, 60.0),
+                                                            (2, 32, 0, 60.0)])
+def test_word_groups_bit_exact(pkg, orc, word_groups, index_bits, stage, rate):
     """Team rows whose dense projections are held as word groups (one record per
     presynaptic spike word: word index, connection mask, first multiplier) at
     several density thresholds -- none (0), every team projection (1, 2), only the
     densest (64: none of this network) -- in both column forms, staged and reading
-    the ring in memory: the same raster and state as the oracle."""
-    env = new_env(pkg, 0)
+    the ring in memory, and at a 60 Hz mossy input (many spiking bits per group,
+    their multipliers loaded four at a time): the same raster and state as the
+    oracle."""
+    env = new_env(pkg, 0, mf_rate_hz=rate)
     env.set_option("word_groups", word_groups)
     env.set_option("index_bits", index_bits)
     env.set_option("stage_spikes", stage)
