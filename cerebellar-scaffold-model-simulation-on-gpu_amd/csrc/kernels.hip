@@ -1369,9 +1369,6 @@ __global__ void k_zero_g(ScState *st, int n)
     }
 }
 
-// Fired-neuron list of one ring row: the spike words are the ballots of spike
-// detection; one block popcounts them, prefix-sums the counts and writes the
-// reference global ids in ascending order.
 // One plasticity rule after exchange step n (launch offset k): Hebb_plasticity /
 // Teacher_plasticity (SynapticPlasticity.cu:38-121), one wavefront per chunk of
 // synapses of one post row, 4 synapses per lane (their loads in flight
@@ -1441,6 +1438,9 @@ __global__ void __launch_bounds__(256) k_plastic(ScInstance I, ScPlastic P, int 
     }
 }
 
+// Fired-neuron list of one ring row: the spike words are the ballots of spike
+// detection; one block popcounts them, prefix-sums the counts and writes the
+// reference global ids in ascending order.
 __global__ void __launch_bounds__(1024) k_compact(const uint64_t *__restrict__ row, int n_groups,
                                                   const ScGroup *__restrict__ groups,
                                                   const ScPop *__restrict__ pops, int32_t *__restrict__ out,
