@@ -510,8 +510,9 @@ class NetworkEnvironment:
         _check(self._lib.scaffold_record_spikes(self._h, None if directory is None else str(directory).encode()))
 
     def step_timer(self, on=-1):
-        """Step-kernel timer: on=1 clears and starts it, 0 stops it, -1 reads it.
-        Returns (milliseconds measured, step launches covered)."""
+        """Step-kernel timer: on=n >= 1 clears and starts it, timing one run of
+        steps in n; 0 stops it, -1 reads it.  Returns (milliseconds measured, step
+        launches covered)."""
         ms = ctypes.c_double()
         n = ctypes.c_int64()
         _check(self._lib.scaffold_step_timer(self._h, on, ctypes.byref(ms), ctypes.byref(n)))

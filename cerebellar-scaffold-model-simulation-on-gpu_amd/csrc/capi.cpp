@@ -1050,7 +1050,7 @@ int scaffold_step_timer(scaffold_env *env, int on, double *ms, int64_t *launches
     clear_error();
     int rc = check_ready(env);
     if (rc) return rc;
-    if (on < -1 || on > 1) return fail(SCAFFOLD_ERR_INVALID_ARG, "on is -1, 0 or 1");
+    if (on < -1 || on > 1000) return fail(SCAFFOLD_ERR_INVALID_ARG, "on is -1, 0, or n >= 1 (one run in n timed)");
     return engine_timer(env->eng, on, ms, launches);
 }
 

@@ -32,6 +32,8 @@ struct TaskWin {
 // statistics and counter advance of the piece following it as direct launches).
 struct KTimer {
     bool on = false;
+    int every = 1;                 // time one run in `every` (the runs counted in `seq`)
+    int64_t seq = 0;
     std::vector<hipEvent_t> pool;  // pairs
     size_t used = 0;
     std::vector<int> piece_k;      // step launches of each measured piece
@@ -794,7 +796,7 @@ static int run_steps(hipStream_t s, RunGraphs &G, const ScInstance *single, cons
     if (out_taken) *out_taken = false;
     if (train_ev_set) *train_ev_set = false;
     if (n_steps <= 0) return SCAFFOLD_OK;
-    const bool timed = kt && kt->on;
+    const bool timer = kt && kt->on;
     const int last_k = n_steps % chunk ? n_steps % chunk : chunk;  // steps of the last run
     const ScOut *last_out = out && out->out && out->T > 0 && out->T <= last_k ? out : nullptr;
     int done = 0;
@@ -803,6 +805,7 @@ static int run_steps(hipStream_t s, RunGraphs &G, const ScInstance *single, cons
         const bool last = done + K == n_steps;
         const ScOut *o = last ? last_out : nullptr;
         const bool graph = use_graph && K == chunk;
+        const bool timed = timer && kt->seq++ % kt->every == 0;  // one run in kt->every
         hipEvent_t a = nullptr, b = nullptr;
         if (timed) {
             int rc = timer_pair(kt, K, &a, &b);
@@ -919,7 +922,7 @@ int engine_timer(Engine *e, int on, double *ms, int64_t *launches)
 {
     if (on >= 0) {
         e->kt.on = on != 0;
-        if (on) { e->kt.ms = 0.0; e->kt.launches = 0; }
+        if (on) { e->kt.ms = 0.0; e->kt.launches = 0; e->kt.every = on; e->kt.seq = 0; }
     }
     if (ms) *ms = e->kt.ms;
     if (launches) *launches = e->kt.launches;

@@ -505,13 +505,14 @@ int scaffold_record_spikes(scaffold_env *env, const char *dir);
 int scaffold_time_steps(scaffold_env *env, int n_steps, double *ms_total,
                         double *ms_step_kernel_mean);
 
-/* Step-kernel timer (no counterpart in the reference): on = 1 clears and starts
- * it, 0 stops it, -1 only reads it.  While it runs, every run of steps
- * (scaffold_loop_n_steps, scaffold_time_steps) records HIP events on the
- * engine's stream around its step kernels: around the step kernels of each run
+/* Step-kernel timer (no counterpart in the reference): on = n >= 1 (at most
+ * 1000) clears and starts it, 0 stops it, -1 only reads it.  While it runs, one
+ * run of steps in n (scaffold_loop_n_steps, scaffold_time_steps; the first one
+ * included) records HIP events on the engine's stream around its step kernels
  * (launched directly, or with "graph" 1 as a replay of a graph that holds the
- * run's step kernels alone; the run's end follows as a direct launch).  *ms receives the measured time, *launches the step
- * kernels it covers (either may be NULL). */
+ * run's step kernels alone; the run's end follows as a direct launch).  *ms
+ * receives the measured time, *launches the step kernels it covers (either may
+ * be NULL). */
 int scaffold_step_timer(scaffold_env *env, int on, double *ms, int64_t *launches);
 
 /* Autotuner (no counterpart in the reference, whose layout is fixed at compile
