@@ -17,6 +17,25 @@ namespace sc {
 // Bit ranges one task reads (per lane slice, or per incoming projection of a team
 // neuron) and the 64-byte segments its gathers touch: the input of the staging
 // plan.  lo > hi: the slice / projection reads nothing.
+// Events a function creates for its own timing, destroyed on every path out of it.
+struct LocalEvents {
+    std::vector<hipEvent_t> ev;
+    LocalEvents() = default;
+    LocalEvents(const LocalEvents &) = delete;
+    LocalEvents &operator=(const LocalEvents &) = delete;
+    ~LocalEvents()
+    {
+        for (hipEvent_t x : ev) (void)hipEventDestroy(x);
+    }
+    int make(hipEvent_t *out)
+    {
+        *out = nullptr;
+        HIP_TRY(hipEventCreate(out));
+        ev.push_back(*out);
+        return SCAFFOLD_OK;
+    }
+};
+
 struct TaskWin {
     int n = 0;
     int32_t delay[SC_MAX_IN] = {}, pre[SC_MAX_IN] = {};
@@ -931,13 +950,15 @@ int engine_timer(Engine *e, int on, double *ms, int64_t *launches)
 
 int engine_time(Engine *e, int n_steps, int use_graph, int graph_steps, double *ms_total, double *ms_kernel_mean)
 {
+    LocalEvents L;
     hipEvent_t a, b;
-    HIP_TRY(hipEventCreate(&a));
-    HIP_TRY(hipEventCreate(&b));
+    int rc = L.make(&a);
+    if (!rc) rc = L.make(&b);
+    if (rc) return rc;
     const int chunk = chunk_for(graph_steps, e->hi.R);
     if (!ms_kernel_mean) {
         HIP_TRY(hipEventRecord(a, e->stream));
-        int rc = run_steps(e->stream, e->graphs, &e->hi, e->di, 1, e->w, n_steps, use_graph, chunk, &e->kt);
+        rc = run_steps(e->stream, e->graphs, &e->hi, e->di, 1, e->w, n_steps, use_graph, chunk, &e->kt);
         if (rc) return rc;
         HIP_TRY(hipEventRecord(b, e->stream));
     } else {
@@ -949,8 +970,9 @@ int engine_time(Engine *e, int n_steps, int use_graph, int graph_steps, double *
         while (done < n_steps) {
             const int K = std::min(chunk, n_steps - done);
             hipEvent_t s0, s1;
-            HIP_TRY(hipEventCreate(&s0));
-            HIP_TRY(hipEventCreate(&s1));
+            rc = L.make(&s0);
+            if (!rc) rc = L.make(&s1);
+            if (rc) return rc;
             ev.push_back(s0);
             ev.push_back(s1);
             HIP_TRY(hipEventRecord(s0, e->stream));
@@ -971,17 +993,12 @@ int engine_time(Engine *e, int n_steps, int use_graph, int graph_steps, double *
             sum += x;
         }
         *ms_kernel_mean = sum / n_steps;
-        for (auto &x : ev) (void)hipEventDestroy(x);
     }
     HIP_TRY(hipEventSynchronize(b));
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, a, b));
     if (ms_total) *ms_total = ms;
-    int rc = engine_finish(e);  // the step-kernel timer's events of this run
-    if (rc) return rc;
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
-    return SCAFFOLD_OK;
+    return engine_finish(e);  // the step-kernel timer's events of this run
 }
 
 int engine_get_state(Engine *e, float *V, float *g, int32_t *refr)
@@ -1080,13 +1097,23 @@ int engine_copy_probe(int64_t bytes, int iters, double *ms)
     int rc = check_device();
     if (rc) return rc;
     const int64_t n = std::max<int64_t>(1, bytes / 16);
-    void *a = nullptr, *b = nullptr;
+    struct Buffers {
+        void *a = nullptr, *b = nullptr;
+        ~Buffers()
+        {
+            (void)hipFree(a);
+            (void)hipFree(b);
+        }
+    } B;
+    void *&a = B.a, *&b = B.b;
     HIP_TRY(hipMalloc(&a, n * 16));
     HIP_TRY(hipMalloc(&b, n * 16));
     HIP_TRY(hipMemset(a, 1, n * 16));
+    LocalEvents L;
     hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    rc = L.make(&e0);
+    if (!rc) rc = L.make(&e1);
+    if (rc) return rc;
     const int grid = (int)std::min<int64_t>((n + 255) / 256, 256 * 64);
     launch_copy(nullptr, a, b, n, grid);
     HIP_TRY(hipEventRecord(e0));
@@ -1096,10 +1123,6 @@ int engine_copy_probe(int64_t bytes, int iters, double *ms)
     float t = 0.f;
     HIP_TRY(hipEventElapsedTime(&t, e0, e1));
     *ms = t / iters;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipFree(a);
-    (void)hipFree(b);
     return SCAFFOLD_OK;
 }
 
@@ -1191,20 +1214,20 @@ int batch_build(Batch **out, std::vector<Engine *> &engines, const std::vector<c
 
 int batch_run(Batch *b, int n_steps, int graph_steps, double *ms)
 {
+    LocalEvents L;
     hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    int rc = L.make(&e0);
+    if (!rc) rc = L.make(&e1);
+    if (rc) return rc;
     HIP_TRY(hipEventRecord(e0, b->stream));
-    int rc = run_steps(b->stream, b->graphs, nullptr, b->di, b->n_inst, b->w, n_steps, 1,
-                       chunk_for(graph_steps, b->min_R));
+    rc = run_steps(b->stream, b->graphs, nullptr, b->di, b->n_inst, b->w, n_steps, 1,
+                   chunk_for(graph_steps, b->min_R));
     if (rc) return rc;
     HIP_TRY(hipEventRecord(e1, b->stream));
     HIP_TRY(hipEventSynchronize(e1));
     float t = 0.f;
     HIP_TRY(hipEventElapsedTime(&t, e0, e1));
     if (ms) *ms = t;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     return SCAFFOLD_OK;
 }
 

@@ -577,13 +577,16 @@ def text_lines_of_rows(env, rows, steps, output):
     return lines
 
 
-def test_output_train_fired_and_text(pkg, orc, tmp_path):
+@pytest.mark.parametrize("graph", [0, 1])
+def test_output_train_fired_and_text(pkg, orc, tmp_path, graph):
     """loop_n_steps with any n_steps: the output train is handed over after every
     step that completes a T_cpu_sim window (GPUScaffoldSimulation.cu:768-783) and
     holds the last completed window; calls that complete none return no train.
     The fired list, the oldest queryable ring row and both text files (device
-    populations, OUTPUT population) agree with the oracle."""
+    populations, OUTPUT population) agree with the oracle.  With graph 1 the full
+    runs replay captured graphs, the train gathered by the run's end launch."""
     env = new_env(pkg, 0)
+    env.set_option("graph", graph)
     env.initialize(50, 25)
     ref = orc.OracleSim.from_env(env)
     ref.initialize()
