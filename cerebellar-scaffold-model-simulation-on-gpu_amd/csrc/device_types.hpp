@@ -246,7 +246,7 @@ struct alignas(16) ScState {
 // after reading the trigger.
 #define SC_STDP_HEBB     0
 #define SC_STDP_TEACHER  1
-#define SC_PLASTIC_CHUNK 256
+#define SC_PLASTIC_CHUNK 128
 #define SC_PLASTIC_WINDOW_MAX 127
 struct ScPlastic {
     const uint32_t *pre;       // population-local presynaptic id
