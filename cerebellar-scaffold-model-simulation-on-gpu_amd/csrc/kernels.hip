@@ -1371,8 +1371,11 @@ __global__ void k_zero_g(ScState *st, int n)
 
 // One plasticity rule after exchange step n (launch offset k): Hebb_plasticity /
 // Teacher_plasticity (SynapticPlasticity.cu:38-121), one wavefront per chunk of
-// synapses of one post row, 4 synapses per lane (their loads in flight
-// together).  dw is summed in float in the
+// synapses of one post row, SC_PLASTIC_CHUNK / 64 synapses per lane.  The loads
+// form four dependent rounds: the chunk's descriptor with the ring position, the
+// trigger's spike word, every lane's synapse records (lanes past the chunk's end
+// load its last record and store nothing), then the window counts and
+// multipliers.  dw is summed in float in the
 // reference's order: A0; A1_J if the presynaptic neuron spiked at n - delay; A1_I
 // if the row's trigger spiked -- the postsynaptic neuron at n (Hebb) or the
 // teacher at n - t_delay (Teacher); with that spike, A2_JI once for every spike
@@ -1385,57 +1388,62 @@ __global__ void __launch_bounds__(256) k_plastic(ScInstance I, ScPlastic P, int 
 {
     const int c = uniform((int)blockIdx.x * 4 + (int)(threadIdx.x / SC_WAVE));
     const int lane = threadIdx.x & (SC_WAVE - 1);
+    if (c < P.n_chunks) {
+        const int32_t trig = uniform(P.ch_trig[c]);
+        const uint32_t b0 = (uint32_t)uniform((int)P.ch_begin[c]), b1 = (uint32_t)uniform((int)P.ch_begin[c + 1]);
+        const int R = I.R;
+        const int row = (I.ctl->row + k) % R;
+        int s_post = 0;
+        if (trig >= 0) {
+            const int r = P.rule == SC_STDP_HEBB ? row : row_back(row, P.t_delay, R);
+            s_post = (int)((I.ring[(int64_t)r * I.Wr + (trig >> 6)] >> (trig & 63)) & 1ull);
+        }
+        s_post = uniform(s_post);
+        if (!P.skip_quiet || s_post) {
+            constexpr int U = SC_PLASTIC_CHUNK / SC_WAVE;
+            uint32_t vi[U], pr[U], h[U];
+            float w[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t i = min(b0 + lane + u * SC_WAVE, b1 - 1u);  // a chunk holds >= 1 synapse
+                vi[u] = P.vidx[i];
+                pr[u] = P.pre[i];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                h[u] = P.hist[pr[u]];
+                w[u] = P.val[vi[u]];
+            }
+            // every lane computes (its loads are then needed on every lane and stay
+            // ahead of the stores together); lanes past the chunk's end store nothing
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                // coefficient c for this synapse: a constant, or the weight-dependent
+                // value (float)(ac + aa * (ab - (double)w)) of the multiplier before the update
+                const auto coef = [&](int c) {
+                    return ((P.affine >> c) & 1) ? (float)(P.ac[c] + P.aa[c] * (P.ab[c] - (double)w[u])) : P.coef[c];
+                };
+                float dw = 0.f;
+                if (P.present & 1) dw += coef(0);
+                if ((P.present & 4) && (h[u] >> 7)) dw += coef(2);
+                if ((P.present & 2) && s_post) dw += coef(1);
+                if ((P.present & 8) && s_post) {
+                    const float a2 = coef(3);  // the same value for every s of the window (a function of w only)
+                    for (int q = 0; q < (int)(h[u] & 127u); ++q) dw += a2;
+                }
+                float nw = w[u] + dw;
+                if (P.rule == SC_STDP_HEBB) nw = nw < 0.f ? 0.f : nw > 1.f ? 1.f : nw;
+                else nw = nw < 0.f ? 0.f : nw;
+                w[u] = ldexpf(rintf(ldexpf(nw, -P.grid_exp)), P.grid_exp);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (b0 + lane + u * SC_WAVE < b1) P.val[vi[u]] = w[u];
+        }
+    }
     // the window counts of this step are complete (the step kernel ran before this
     // launch): the next step updates them from these
     if (c == 0 && lane == 0) I.ctl->recount = 0;
-    if (c >= P.n_chunks) return;
-    const int R = I.R;
-    const int row = (I.ctl->row + k) % R;
-    const int32_t trig = P.ch_trig[c];
-    int s_post = 0;
-    if (trig >= 0) {
-        const int r = P.rule == SC_STDP_HEBB ? row : row_back(row, P.t_delay, R);
-        s_post = (int)((I.ring[(int64_t)r * I.Wr + (trig >> 6)] >> (trig & 63)) & 1ull);
-    }
-    s_post = uniform(s_post);
-    if (P.skip_quiet && !s_post) return;
-    const uint32_t b0 = P.ch_begin[c], b1 = P.ch_begin[c + 1];
-    constexpr int U = SC_PLASTIC_CHUNK / SC_WAVE;
-    uint32_t h[U], vi[U];
-    float w[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const uint32_t i = b0 + lane + u * SC_WAVE;
-        h[u] = 0u;
-        vi[u] = 0u;
-        if (i < b1) {
-            vi[u] = P.vidx[i];
-            h[u] = P.hist[P.pre[i]];
-        }
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) w[u] = b0 + lane + u * SC_WAVE < b1 ? P.val[vi[u]] : 0.f;
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        if (b0 + lane + u * SC_WAVE >= b1) continue;
-        // coefficient c for this synapse: a constant, or the weight-dependent
-        // value (float)(ac + aa * (ab - (double)w)) of the multiplier before the update
-        const auto coef = [&](int c) {
-            return ((P.affine >> c) & 1) ? (float)(P.ac[c] + P.aa[c] * (P.ab[c] - (double)w[u])) : P.coef[c];
-        };
-        float dw = 0.f;
-        if (P.present & 1) dw += coef(0);
-        if ((P.present & 4) && (h[u] >> 7)) dw += coef(2);
-        if ((P.present & 2) && s_post) dw += coef(1);
-        if ((P.present & 8) && s_post) {
-            const float a2 = coef(3);  // the same value for every s of the window (a function of w only)
-            for (int q = 0; q < (int)(h[u] & 127u); ++q) dw += a2;
-        }
-        float nw = w[u] + dw;
-        if (P.rule == SC_STDP_HEBB) nw = nw < 0.f ? 0.f : nw > 1.f ? 1.f : nw;
-        else nw = nw < 0.f ? 0.f : nw;
-        P.val[vi[u]] = ldexpf(rintf(ldexpf(nw, -P.grid_exp)), P.grid_exp);
-    }
 }
 
 // Fired-neuron list of one ring row: the spike words are the ballots of spike
