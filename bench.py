@@ -372,6 +372,8 @@ def main():
     ap.add_argument("--batch-profile-steps", type=int, default=0,
                     help="with --only-batched: build the batch alone and advance it this many steps, one launch "
                          "per call and no graph capture (for counter passes)")
+    ap.add_argument("--only-variant", default=None, choices=("rule_text", "idx16", "plastic"),
+                    help="only run this variant of the plain 96.7k drop-in loop and print its result")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run the calibration copy kernel on this many MiB and exit")
     args = ap.parse_args()
@@ -389,6 +391,10 @@ def main():
         print(json.dumps(batched(pkg, args.batch_instances, args.batch_steps, 50, layout=args.batch_layout,
                                  profile_steps=args.batch_profile_steps,
                                  threads=min(16, os.cpu_count() or 4))))
+        return
+    if args.only_variant:
+        kw = {"rule_text": {"connectome": 0}, "idx16": {"index_bits": 16}, "plastic": {"plastic": True}}
+        print(json.dumps(variant(pkg, args.steps, args.warmup, **kw[args.only_variant])))
         return
     if args.copy_probe_mb:
         ms = pkg.copy_probe(args.copy_probe_mb << 20, 20)
