@@ -727,7 +727,8 @@ int engine_upload_stims(Engine *e, const std::vector<ScStim> &stims)
 // Steps in runs of at most `chunk` (< ring rows, so the end of a run still finds
 // every row of the run): the step kernels (and the plasticity kernels after
 // each), then the run's end (statistics, the output train when the run is the
-// last of a hand-off, the counter advance).  Full runs replay a captured hipGraph.
+// last of a hand-off, the counter advance).  Runs are launched directly; with the
+// `graph` option full runs replay a captured hipGraph (batches always do).
 static void launch_step_kernels(hipStream_t s, const ScInstance *single, const ScInstance *insts, const DevWork &w,
                                 int K)
 {

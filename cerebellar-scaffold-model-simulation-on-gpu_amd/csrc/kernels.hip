@@ -21,8 +21,9 @@
 // boundary orders consecutive steps.  k_run_end closes a run of steps: it
 // popcounts the run's spike words into per-population counts, sums the
 // out-degrees of the spiking neurons (synaptic events), gathers the output spike
-// train and moves the device step counters.  Runs of steps are captured once as a
-// hipGraph and replayed.
+// train (unless k_out_train, launched just after the last step, already has: a
+// directly launched call then returns on an event after the train, before the
+// statistics) and moves the device step counters.
 //
 // Blocks whose gathers read a bounded range of presynaptic neurons first copy
 // those spike words ("windows", planned per block by the engine) into LDS with
