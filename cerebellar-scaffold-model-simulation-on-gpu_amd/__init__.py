@@ -246,6 +246,8 @@ class NetworkEnvironment:
                  num_of_plasticities=0):
         self._lib = load_library()
         self._h = ctypes.c_void_p()
+        self._freq_buf = None     # set_input_freq's rate array, one per stimulus (from initialize)
+        self._train_view = None   # (address, array view) of the library's output-train buffer
         _check(self._lib.scaffold_create_network_env(ctypes.byref(self._h), num_of_dev, num_of_neuron_types,
                                                      num_of_connectivity_types, num_of_inputs,
                                                      num_of_plasticities))
@@ -367,6 +369,10 @@ class NetworkEnvironment:
         _check(self._lib.scaffold_initialize_network_env(self._h, T_print, T_cpu_sim))
         info = self.info()
         self._train_shape = (info.T_cpu_sim, info.n_output_neurons)  # fixed until finalize
+        self._train_view = None
+        if info.n_stimuli > 0:
+            self._freq_buf = np.zeros(info.n_stimuli, dtype=np.float32)
+            self._freq_ptr = self._freq_buf.ctypes.data
 
     def loop_n_steps(self, start_step, n_steps, trial=0, want_output=True):
         """Advance n_steps from start_step.  With want_output, returns the output
@@ -380,12 +386,20 @@ class NetworkEnvironment:
         T, n_out = self._train_shape
         if T * n_out == 0:
             return np.zeros((T, 0), dtype=np.uint8)
-        buf = (ctypes.c_uint8 * (T * n_out)).from_address(out.value)
-        return np.frombuffer(buf, dtype=np.uint8).reshape(T, n_out).copy()
+        if self._train_view is None or self._train_view[0] != out.value:
+            buf = (ctypes.c_uint8 * (T * n_out)).from_address(out.value)
+            self._train_view = (out.value, np.frombuffer(buf, dtype=np.uint8).reshape(T, n_out))
+        return self._train_view[1].copy()
 
     def set_input_freq(self, freq_list):
-        f = np.ascontiguousarray(freq_list, dtype=np.float32)
-        _check(self._lib.scaffold_set_input_freq(self._h, f.ctypes.data))
+        """SetInputFreq: one rate (Hz) per registered stimulus, in registration order."""
+        buf = self._freq_buf
+        if buf is None or len(freq_list) != buf.size:
+            f = np.ascontiguousarray(freq_list, dtype=np.float32)
+            _check(self._lib.scaffold_set_input_freq(self._h, f.ctypes.data))
+            return
+        buf[:] = freq_list
+        _check(self._lib.scaffold_set_input_freq(self._h, self._freq_ptr))
 
     def reset(self, trial=0):
         _check(self._lib.scaffold_reset_network_env(self._h, trial))
