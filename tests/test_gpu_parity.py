@@ -886,8 +886,8 @@ def test_autotune_then_bit_exact(pkg, orc):
     env.finalize()
 
 
-@pytest.mark.parametrize("index_bits,precision,extra", [(32, 0, ()), (16, 1, ()),
-                                                         (32, 0, (("nontemporal", 1), ("graph", 0)))])
+@pytest.mark.parametrize("index_bits,precision,extra", [(32, 0, (("graph", 1),)), (16, 1, ()),
+                                                         (32, 0, (("nontemporal", 1),))])
 def test_plasticity_bit_exact(pkg, orc, index_bits, precision, extra):
     """Synaptic plasticity on the GPU against the oracle, 200 steps: the
     reference's parallel-fibre-to-Purkinje Teacher rule (A1_J 0, A2_JI -0.01,
@@ -896,8 +896,8 @@ def test_plasticity_bit_exact(pkg, orc, index_bits, precision, extra):
     Golgi-to-granule (lane-mode slices).  Rasters every step, the state and every
     updated multiplier must match bit for bit; 16-bit columns move the team rows'
     multipliers into chunk order, BF16 elsewhere leaves the plastic post
-    populations in f32; the third case uses the nontemporal step kernels and no
-    graph replays."""
+    populations in f32; the first case replays its runs as captured graphs (step
+    and plasticity kernels), the third uses the nontemporal step kernels."""
     env = new_env(pkg, 0, plasticities=2)
     idx = {env.projection(i).type: i for i in range(env.info().n_projections)}
     pf_pc, bc_pc, goc_grc = idx[0], idx[11], idx[15]
