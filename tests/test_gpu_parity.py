@@ -663,18 +663,20 @@ def expected_trial_files(env, rows, T, T_print, n_steps):
     return dev, host
 
 
-@pytest.mark.parametrize("T_print,T,n_steps,calls", [
-    (50, 25, 250, None),                                   # main.cu's loop: one call per window
-    (25, 25, 200, (7, 11, 7, 30, 3, 1, 48, 25, 50, 18)),  # ring of 2*T_cpu_sim rows, calls across windows
-    (100, 25, 275, None),                                  # the last T_print window incomplete
+@pytest.mark.parametrize("T_print,T,n_steps,calls,graph", [
+    (50, 25, 250, None, 0),                                   # main.cu's loop: one call per window
+    (25, 25, 200, (7, 11, 7, 30, 3, 1, 48, 25, 50, 18), 0),  # ring of 2*T_cpu_sim rows, calls across windows
+    (25, 25, 200, (7, 11, 7, 30, 3, 1, 48, 25, 50, 18), 1),  # the same with runs replayed as graphs
+    (100, 25, 275, None, 0),                                  # the last T_print window incomplete
 ])
-def test_recorded_spike_files(pkg, orc, tmp_path, T_print, T, n_steps, calls):
+def test_recorded_spike_files(pkg, orc, tmp_path, T_print, T, n_steps, calls, graph):
     """PRINT mode: scaffold_record_spikes writes the printer- and host-thread files
     while the trial runs, ResetNetworkEnv merges them; two trials (SetInputFreq
     before each call, as main.cu:40-55) against the files rendered from the oracle's
     raster.  After the last reset the four files are empty; recording stopped
     writes nothing."""
     env = new_env(pkg, 0)
+    env.set_option("graph", graph)
     env.initialize(T_print, T)
     ref = orc.OracleSim.from_env(env)
     ref.initialize()
