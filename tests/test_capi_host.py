@@ -265,7 +265,8 @@ def test_options(pkg):
 def test_queries_before_initialization(pkg):
     env = new_env(pkg, 0)
     for call in (lambda: env.loop_n_steps(0, 25), lambda: env.state(), lambda: env.counts(0),
-                 lambda: env.reset(0), lambda: env.set_state(V=np.zeros(1, np.float32))):
+                 lambda: env.reset(0), lambda: env.set_state(V=np.zeros(1, np.float32)),
+                 lambda: env.step_timer(1)):
         with pytest.raises(pkg.ScaffoldError) as e:
             call()
         assert e.value.code == pkg.ERR_STATE

@@ -844,6 +844,43 @@ def test_batch_matches_single(pkg, orc, index_bits):
         e.finalize()
 
 
+@pytest.mark.parametrize("graph", [0, 1])
+def test_step_timer_samples_runs(pkg, orc, graph):
+    """scaffold_step_timer(n) times one run of steps in n (the first included) and
+    counts the step launches of the runs it timed; 0 stops it, -1 reads it, values
+    outside -1..1000 are refused.  Timed runs leave the results as they are."""
+    env = new_env(pkg, 0)
+    env.set_option("graph", graph)
+    env.initialize(50, 25)
+    ref = orc.OracleSim.from_env(env)
+    ref.initialize()
+    for bad in (-2, 1001):
+        with pytest.raises(pkg.ScaffoldError) as e:
+            env.step_timer(bad)
+        assert e.value.code == pkg.ERR_INVALID_ARG
+    t = 0
+    env.step_timer(1)
+    for n in (25, 30):   # runs of 25; 25 + 5
+        env.loop_n_steps(t, n, 0)
+        t += n
+    ms, launches = env.step_timer(-1)
+    assert launches == 55 and 0.0 < ms / launches < 1.0, (ms, launches)
+    env.step_timer(4)
+    for _ in range(8):   # one run of 25 steps per call; runs 0 and 4 are timed
+        env.loop_n_steps(t, 25, 0)
+        t += 25
+    ms, launches = env.step_timer(0)
+    assert launches == 50 and 0.0 < ms / launches < 1.0, (ms, launches)
+    env.loop_n_steps(t, 25, 0)
+    t += 25
+    assert env.step_timer(-1) == (ms, launches)   # stopped: nothing more is timed
+    ref.run(0, t)
+    for s in range(t - 49, t):
+        assert np.array_equal(env.spike_row(s), ref.spike_row(s)), f"step {s}"
+    assert_same_state(env, ref, "timed runs")
+    env.finalize()
+
+
 def test_long_run_chunking_and_totals(pkg, orc):
     """Runs longer than the spike ring (statistics per run chunk) and the running
     totals equal the oracle's spike counts."""
