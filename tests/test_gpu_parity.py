@@ -881,6 +881,33 @@ def test_step_timer_samples_runs(pkg, orc, graph):
     env.finalize()
 
 
+def test_timing_entry_points(pkg, orc):
+    """The timing calls the bench and the autotuner use: scaffold_time_layout
+    (before initialization, on a private engine), scaffold_time_steps with and
+    without the per-kernel mean (the steps count as simulated steps, results as
+    loop_n_steps would give), scaffold_batch_time, and the copy probe."""
+    env = new_env(pkg, 0)
+    ms, km = env.time_layout(50)
+    assert 0.0 < km <= ms < 1.0, (ms, km)
+    env.initialize(50, 25)
+    ref = orc.OracleSim.from_env(env)
+    ref.initialize()
+    total = env.time_steps(40)
+    total2, kmean = env.time_steps(60, kernel_mean=True)
+    assert total > 0.0 and 0.0 < kmean * 60 <= total2, (total, total2, kmean)
+    assert env.info().current_step == 100
+    ref.run(0, 100)
+    for s in range(51, 100):
+        assert np.array_equal(env.spike_row(s), ref.spike_row(s)), f"step {s}"
+    assert_same_state(env, ref, "timed steps")
+    b = pkg.Batch([env])
+    assert b.time(25) > 0.0
+    b.destroy()
+    assert env.info().current_step == 125
+    env.finalize()
+    assert pkg.copy_probe(1 << 20, 5) > 0.0
+
+
 def test_long_run_chunking_and_totals(pkg, orc):
     """Runs longer than the spike ring (statistics per run chunk) and the running
     totals equal the oracle's spike counts."""
