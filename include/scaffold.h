@@ -249,9 +249,13 @@ int scaffold_set_input_stim_ids(scaffold_env *env, int neuron_index,
  *                       launches measured as fast or faster at every size: the
  *                       host keeps ahead of 10 us step kernels either way, and
  *                       graph-launched step kernels ran ~2 % longer)
- *   "graph_steps"       exchange steps per run: a run's statistics, output train
- *                       and counter advance are one launch at its end (the length
- *                       of a captured graph with "graph" 1)
+ *   "graph_steps"       exchange steps per run: a run's statistics and counter
+ *                       advance are one launch at its end (the length of a
+ *                       captured graph with "graph" 1; a batch's runs are always
+ *                       replayed as graphs).  The output train of a call is
+ *                       gathered by that launch in a graph, else by a launch of
+ *                       its own after the call's last step, and the call returns
+ *                       once the train is written.
  *   "reset_g"           1: also zero conductances in scaffold_reset_network_env
  *                       (the reference keeps them, GPUScaffoldSimulation.cu:802-854)
  *   "stage_spikes"      1 (default): blocks whose gathers scatter over a bounded
