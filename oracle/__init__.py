@@ -1,7 +1,7 @@
 """TEST INFRASTRUCTURE -- ctypes binding of the CPU oracle (oracle/scaffold_oracle.c).
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
-module, and only as the checker.  The product never imports it.
+module, and only to check the engine's results.  The product never imports it.
 
 OracleSim.from_env() copies an environment's description (populations,
 projections as CSR with the multipliers the engine applies, stimuli) through the

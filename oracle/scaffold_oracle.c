@@ -1,8 +1,8 @@
 /*
  * scaffold_oracle.c -- TEST INFRASTRUCTURE, NOT PART OF THE PRODUCT.
  *
- * CPU restatement of the reference's per-exchange-step algorithm, used only as
- * the checker by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline.
+ * CPU restatement of the reference's per-exchange-step algorithm, used only to
+ * check the engine by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline.
  * Build: oracle/Makefile (gcc -O2 -ffp-contract=off, strict IEEE, no FMA).
  *
  * Pinning (checks that would catch a plausible mistake in each part):

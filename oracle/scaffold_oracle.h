@@ -3,7 +3,7 @@
  *
  * A plain-C, single-threaded CPU restatement of the reference simulator's
  * per-exchange-step algorithm.  Only the tests, the smoke check and the
- * benchmark's CPU-baseline leg load it, and only as the checker.  The product
+ * benchmark's CPU-baseline leg load it, and only to check results.  The product
  * (the HIP engine) never links or calls it.
  *
  * Every function cites the reference lines it restates (paths relative to the
